@@ -1,0 +1,14 @@
+"""hyperfox_amd: MI355X-native HDG element-by-element hot path.
+
+The compute path is libhfx_hip.so (HIP kernels for gfx950 behind a C ABI,
+declared in include/hfx_hip.h).  This package is a thin ctypes layer over it
+for the tests and the benchmark; it has no CPU fallback: importing
+``hyperfox_amd.lib`` raises if the shared library has not been built.
+"""
+from ._lib import lib, LIB_PATH, HfxError  # noqa: F401
+from .hdg import Mesh, HDGProblem, refel_sizes, refel_tables, cubature  # noqa: F401
+
+MODEL_LAPLACE = 0
+MODEL_DIFFUSION_SOURCE = 1
+MODEL_CDR_SOURCE = 2
+PC_NONE, PC_JACOBI, PC_BJACOBI, PC_LEARNED = 0, 1, 2, 3

@@ -1,0 +1,65 @@
+"""ctypes binding of libhfx_hip.so (C ABI in include/hfx_hip.h)."""
+import ctypes
+import os
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libhfx_hip.so')
+
+
+class HfxError(RuntimeError):
+    """An hfx_* entry point returned a nonzero status."""
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError('libhfx_hip.so is not built (%s); run __graft_entry__.build()' % LIB_PATH)
+    return ctypes.CDLL(LIB_PATH)
+
+
+lib = _load()
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.c_double
+_LL = ctypes.c_longlong
+
+_SIGS = {
+    'hfx_last_error': ([], ctypes.c_char_p),
+    'hfx_device_count': ([_P], _I),
+    'hfx_refel_sizes': ([_I, _I, _P], _I),
+    'hfx_refel_tables': ([_I, _I] + [_P] * 9, _I),
+    'hfx_cubature': ([_I, _I, _P, _P], _I),
+    'hfx_mesh_structured': ([_I, _I, _I, _D, ctypes.c_ulonglong, _P], _I),
+    'hfx_mesh_from_cells': ([_I, _I, _I, _P, _I, _P, _P], _I),
+    'hfx_mesh_sizes': ([_P, _P], _I),
+    'hfx_mesh_get': ([_P] * 8, _I),
+    'hfx_mesh_free': ([_P], None),
+    'hfx_hdg_create': ([_I, _I, _I, _P], _I),
+    'hfx_hdg_destroy': ([_P], None),
+    'hfx_hdg_set_mesh': ([_P, _P], _I),
+    'hfx_hdg_set_model': ([_P, _I], _I),
+    'hfx_hdg_set_fields': ([_P, _P, _I, _P, _I, _P, _P], _I),
+    'hfx_hdg_ip_coords': ([_P, _P], _I),
+    'hfx_hdg_set_source': ([_P, _P, _P], _I),
+    'hfx_hdg_set_launch': ([_P, _I, _I], _I),
+    'hfx_hdg_assemble': ([_P], _I),
+    'hfx_hdg_local_matrices': ([_P, _I, _I, _P, _P], _I),
+    'hfx_hdg_get_condensed': ([_P] * 7, _I),
+    'hfx_hdg_trace_sizes': ([_P, _P], _I),
+    'hfx_hdg_get_trace_csr': ([_P] * 5, _I),
+    'hfx_hdg_solve': ([_P, _I, _D, _I, _I, _P, _P], _I),
+    'hfx_hdg_get_solution': ([_P] * 4, _I),
+    'hfx_hdg_time': ([_P, _I, _I, _I, _P], _I),
+    'hfx_element_lds_bytes': ([_I, _I, _P], _I),
+}
+
+for _name, (_args, _res) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.argtypes = _args
+    _f.restype = _res
+
+
+def check(status):
+    """Raise HfxError with hfx_last_error() if status is nonzero."""
+    if status != 0:
+        raise HfxError(lib.hfx_last_error().decode())
+    return status

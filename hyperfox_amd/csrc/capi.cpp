@@ -1,0 +1,555 @@
+// C ABI of libhfx_hip.so (declared in include/hfx_hip.h).
+#include "../../include/hfx_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "hdg_element.h"
+#include "hfx_internal.h"
+#include "krylov.h"
+#include "mesh.h"
+#include "refel.h"
+#include "trace_system.h"
+
+namespace {
+thread_local std::string g_err;
+}
+
+#define HFX_TRY try {
+#define HFX_END                       \
+  }                                   \
+  catch (const std::exception& e) {   \
+    g_err = e.what();                 \
+    return -1;                        \
+  }                                   \
+  return 0;
+
+struct hfx_mesh {
+  std::unique_ptr<hfx::RefEl> re;
+  hfx::Mesh m;
+};
+
+struct hfx_hdg {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int dim = 0, order = 0;
+  std::unique_ptr<hfx::RefEl> re;
+  hfx::Mesh mesh;
+  bool has_mesh = false, has_fields = false, assembled = false, solved = false;
+  int kind = 1, ndD = 0, tau_sides = 1;
+  bool hasD = false, hasB = false, hasF = false, hasR = false;
+  int nt = 0, grid_cap = 0;
+  // device data
+  hfx::DevBuf<double> tabs, coords, Dn, bn, tau, f_ip, r_ip, g;
+  hfx::DevBuf<int> faceNodes, cells, c2f, f2c;
+  hfx::DevBuf<int8_t> perm, invperm;
+  hfx::DevBuf<uint8_t> is_dir;
+  hfx::DevBuf<double> S, S0, U, Q, U0, Q0, rhs, lambda, dinv, u, q;
+  hfx::BsrPattern pat;
+  hfx::BsrDev A;
+  // last solve options
+  int last_pc = HFX_PC_BJACOBI, last_maxits = 10000, last_restart = 30;
+  double last_rtol = 1e-10;
+
+  void set_device() const { HFX_HIP(hipSetDevice(device)); }
+  hfx::ElemArgs elem_args() {
+    hfx::ElemArgs a;
+    a.nEl = mesh.nCells;
+    a.kind = kind;
+    a.useDiff = (kind != HFX_MODEL_CDR_SOURCE) || hasD;
+    a.ndD = hasD ? ndD : 0;
+    a.hasB = hasB;
+    a.hasF = hasF;
+    a.hasR = hasR;
+    a.tau_sides = tau_sides;
+    a.tabs = tabs.p;
+    a.faceNodes = faceNodes.p;
+    a.coords = coords.p;
+    a.cells = cells.p;
+    a.cell2face = c2f.p;
+    a.face2cell = f2c.p;
+    a.face_perm = perm.p;
+    a.Dn = Dn.p;
+    a.bn = bn.p;
+    a.tau = tau.p;
+    a.f_ip = f_ip.p;
+    a.r_ip = r_ip.p;
+    a.S = S.p; a.S0 = S0.p; a.U = U.p; a.Q = Q.p; a.U0 = U0.p; a.Q0 = Q0.p;
+    return a;
+  }
+  int threads() const {
+    if (nt) return nt;
+    return (dim == 3 && order >= 3) ? 256 : (dim == 3 || order >= 4 ? 128 : 64);
+  }
+  void run_element() {
+    hfx::ElemArgs a = elem_args();
+    hfx::launch_element(dim, order, a, threads(), grid_cap, stream);
+  }
+  void run_scatter() {
+    hfx::launch_scatter(A, mesh.nFpc, S.p, S0.p, invperm.p, is_dir.p, g.p, rhs.p, stream);
+  }
+};
+
+namespace {
+
+void pack_tables(const hfx::RefEl& re, std::vector<double>& t) {
+  const int d = re.dim;
+  const hfx::RefEl& fe = *re.face;
+  t.clear();
+  t.insert(t.end(), re.phi.begin(), re.phi.end());
+  t.insert(t.end(), re.dphi.begin(), re.dphi.end());
+  t.insert(t.end(), fe.phi.begin(), fe.phi.end());
+  t.insert(t.end(), fe.dphi.begin(), fe.dphi.end());
+  t.insert(t.end(), re.cub.w.begin(), re.cub.w.end());
+  t.insert(t.end(), fe.cub.w.begin(), fe.cub.w.end());
+  (void)d;
+}
+
+void require(bool c, const char* msg) {
+  if (!c) throw std::runtime_error(msg);
+}
+
+}  // namespace
+
+extern "C" const char* hfx_last_error(void) { return g_err.c_str(); }
+
+extern "C" int hfx_device_count(int* n) {
+  HFX_TRY
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+  *n = c;
+  HFX_END
+}
+
+extern "C" int hfx_refel_sizes(int dim, int order, int* out) {
+  HFX_TRY
+  hfx::RefEl re(dim, order);
+  require(dim >= 2, "hfx_refel_sizes: dimension must be 2 or 3");
+  out[0] = re.nN;
+  out[1] = re.nF;
+  out[2] = re.nNf();
+  out[3] = re.nip();
+  out[4] = re.face->nip();
+  out[5] = re.nN * (dim + 1) + re.nF * re.nNf();
+  out[6] = re.nF * re.nNf();
+  HFX_END
+}
+
+extern "C" int hfx_refel_tables(int dim, int order, double* nodes, int* face_nodes, double* ipx, double* ipw,
+                                double* phi, double* dphi, double* fipx, double* fipw, double* fphi) {
+  HFX_TRY
+  require(dim >= 2, "hfx_refel_tables: dimension must be 2 or 3");
+  hfx::RefEl re(dim, order);
+  if (nodes) std::copy(re.nodes.begin(), re.nodes.end(), nodes);
+  if (face_nodes)
+    for (int f = 0; f < re.nF; ++f)
+      for (int j = 0; j < re.nNf(); ++j) face_nodes[f * re.nNf() + j] = re.faceNodes[f][j];
+  if (ipx) std::copy(re.cub.pts.begin(), re.cub.pts.end(), ipx);
+  if (ipw) std::copy(re.cub.w.begin(), re.cub.w.end(), ipw);
+  if (phi) std::copy(re.phi.begin(), re.phi.end(), phi);
+  if (dphi) std::copy(re.dphi.begin(), re.dphi.end(), dphi);
+  if (fipx) std::copy(re.face->cub.pts.begin(), re.face->cub.pts.end(), fipx);
+  if (fipw) std::copy(re.face->cub.w.begin(), re.face->cub.w.end(), fipw);
+  if (fphi) std::copy(re.face->phi.begin(), re.face->phi.end(), fphi);
+  HFX_END
+}
+
+extern "C" int hfx_cubature(int dim, int order, double* pts, double* w) {
+  try {
+    hfx::Cubature c = hfx::make_cubature(dim, order);
+    if (pts) std::copy(c.pts.begin(), c.pts.end(), pts);
+    if (w) std::copy(c.w.begin(), c.w.end(), w);
+    return c.nip;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+extern "C" int hfx_mesh_structured(int dim, int order, int n, double jitter, unsigned long long seed, hfx_mesh** out) {
+  HFX_TRY
+  auto m = std::unique_ptr<hfx_mesh>(new hfx_mesh());
+  m->re.reset(new hfx::RefEl(dim, order));
+  m->m = hfx::structured_mesh(*m->re, n, jitter, seed);
+  *out = m.release();
+  HFX_END
+}
+
+extern "C" int hfx_mesh_from_cells(int dim, int order, int nNodes, const double* coords, int nCells, const int* cells,
+                                   hfx_mesh** out) {
+  HFX_TRY
+  auto m = std::unique_ptr<hfx_mesh>(new hfx_mesh());
+  m->re.reset(new hfx::RefEl(dim, order));
+  m->m = hfx::mesh_from_cells(*m->re, nNodes, coords, nCells, cells);
+  *out = m.release();
+  HFX_END
+}
+
+extern "C" int hfx_mesh_sizes(const hfx_mesh* m, long long* out) {
+  HFX_TRY
+  require(m, "hfx_mesh_sizes: null mesh");
+  out[0] = m->m.nNodes;
+  out[1] = m->m.nCells;
+  out[2] = m->m.nFaces;
+  out[3] = (long long)m->m.boundary.size();
+  out[4] = (long long)m->m.nFaces * m->m.nNf;
+  HFX_END
+}
+
+extern "C" int hfx_mesh_get(const hfx_mesh* mm, double* coords, int* cells, int* faces, int* cell2face,
+                            int* face2cell, int* boundary, signed char* face_perm) {
+  HFX_TRY
+  require(mm, "hfx_mesh_get: null mesh");
+  const hfx::Mesh& m = mm->m;
+  if (coords) std::copy(m.coords.begin(), m.coords.end(), coords);
+  if (cells) std::copy(m.cells.begin(), m.cells.end(), cells);
+  if (faces) std::copy(m.faces.begin(), m.faces.end(), faces);
+  if (cell2face) std::copy(m.cell2face.begin(), m.cell2face.end(), cell2face);
+  if (face2cell) std::copy(m.face2cell.begin(), m.face2cell.end(), face2cell);
+  if (boundary) std::copy(m.boundary.begin(), m.boundary.end(), boundary);
+  if (face_perm) std::memcpy(face_perm, m.face_perm.data(), m.face_perm.size());
+  HFX_END
+}
+
+extern "C" void hfx_mesh_free(hfx_mesh* m) { delete m; }
+
+extern "C" int hfx_hdg_create(int device, int dim, int order, hfx_hdg** out) {
+  HFX_TRY
+  int n = 0;
+  HFX_HIP(hipGetDeviceCount(&n));
+  require(device >= 0 && device < n, "hfx_hdg_create: no such GPU");
+  auto h = std::unique_ptr<hfx_hdg>(new hfx_hdg());
+  h->device = device;
+  h->dim = dim;
+  h->order = order;
+  h->re.reset(new hfx::RefEl(dim, order));
+  (void)hfx::element_lds_bytes(dim, order);  // throws for unsupported (dim, order)
+  h->set_device();
+  HFX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  std::vector<double> t;
+  pack_tables(*h->re, t);
+  h->tabs.upload(t, h->stream);
+  std::vector<int> fnodes;
+  for (auto& f : h->re->faceNodes) fnodes.insert(fnodes.end(), f.begin(), f.end());
+  h->faceNodes.upload(fnodes, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  *out = h.release();
+  HFX_END
+}
+
+extern "C" void hfx_hdg_destroy(hfx_hdg* h) {
+  if (!h) return;
+  try {
+    h->set_device();
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+  } catch (...) {
+  }
+  hipStream_t s = h->stream;
+  delete h;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
+  HFX_TRY
+  require(h && mm, "hfx_hdg_set_mesh: null handle");
+  require(mm->m.dim == h->dim && mm->re->order == h->order, "hfx_hdg_set_mesh: mesh dimension/order mismatch");
+  h->set_device();
+  h->mesh = mm->m;
+  h->mesh.order = h->order;
+  const hfx::Mesh& m = h->mesh;
+  hipStream_t s = h->stream;
+  h->coords.upload(m.coords, s);
+  h->cells.upload(m.cells, s);
+  h->c2f.upload(m.cell2face, s);
+  h->f2c.upload(m.face2cell, s);
+  h->perm.upload(m.face_perm, s);
+  std::vector<int8_t> inv(m.face_perm.size());
+  for (size_t blk = 0; blk < m.face_perm.size() / m.nNf; ++blk)
+    for (int j = 0; j < m.nNf; ++j) inv[blk * m.nNf + m.face_perm[blk * m.nNf + j]] = (int8_t)j;
+  h->invperm.upload(inv, s);
+  h->pat = hfx::build_bsr_pattern(m.nCells, m.nFaces, m.nFpc, m.nNf, m.cell2face.data(), m.face2cell.data());
+  h->A.nb = h->pat.nb;
+  h->A.bs = h->pat.bs;
+  h->A.nnzb = (int)h->pat.col.size();
+  h->A.rowptr.upload(h->pat.rowptr, s);
+  h->A.col.upload(h->pat.col, s);
+  h->A.rowf.upload(h->pat.rowf, s);
+  h->A.diag.upload(h->pat.diag, s);
+  h->A.contrib.upload(h->pat.contrib, s);
+  h->A.face_cells.upload(h->pat.face_cells, s);
+  h->A.vals.alloc((size_t)h->A.nnzb * m.nNf * m.nNf);
+  std::vector<uint8_t> dir(m.nFaces, 0);
+  for (int f : m.boundary) dir[f] = 1;
+  h->is_dir.upload(dir, s);
+  const int nN = m.nN, nL = m.nFpc * m.nNf, nQ = nN * h->dim;
+  h->S.alloc((size_t)m.nCells * nL * nL);
+  h->S0.alloc((size_t)m.nCells * nL);
+  h->U.alloc((size_t)m.nCells * nN * nL);
+  h->Q.alloc((size_t)m.nCells * nQ * nL);
+  h->U0.alloc((size_t)m.nCells * nN);
+  h->Q0.alloc((size_t)m.nCells * nQ);
+  h->rhs.alloc((size_t)m.nFaces * m.nNf);
+  h->lambda.alloc((size_t)m.nFaces * m.nNf);
+  h->u.alloc((size_t)m.nCells * nN);
+  h->q.alloc((size_t)m.nCells * nQ);
+  h->g.alloc((size_t)m.nFaces * m.nNf);
+  HFX_HIP(hipMemsetAsync(h->g.p, 0, h->g.n * sizeof(double), s));
+  HFX_HIP(hipStreamSynchronize(s));
+  h->has_mesh = true;
+  h->has_fields = h->assembled = h->solved = false;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_model(hfx_hdg* h, int kind) {
+  HFX_TRY
+  require(h, "hfx_hdg_set_model: null handle");
+  require(kind >= 0 && kind <= 2, "hfx_hdg_set_model: unknown model kind");
+  h->kind = kind;
+  h->assembled = h->solved = false;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_fields(hfx_hdg* h, const double* tau, int tau_sides, const double* D, int ndD,
+                                  const double* b, const double* dirichlet) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_set_fields: set the mesh first");
+  require(tau != nullptr, "hfx_hdg_set_fields: the field map must have a Tau field");
+  require(tau_sides == 1 || tau_sides == 2, "hfx_hdg_set_fields: the Tau field must be single or double valued");
+  require(D == nullptr || ndD == 1 || ndD == h->dim * h->dim,
+          "hfx_hdg_set_fields: DiffusionTensor must be scalar or dim x dim per node");
+  h->set_device();
+  const hfx::Mesh& m = h->mesh;
+  hipStream_t s = h->stream;
+  h->tau_sides = tau_sides;
+  h->tau.upload(tau, (size_t)m.nFaces * m.nNf * tau_sides, s);
+  h->hasD = D != nullptr;
+  h->ndD = h->hasD ? ndD : 0;
+  if (D) h->Dn.upload(D, (size_t)m.nNodes * ndD, s);
+  h->hasB = b != nullptr;
+  if (b) h->bn.upload(b, (size_t)m.nNodes * h->dim, s);
+  if (dirichlet) h->g.upload(dirichlet, (size_t)m.nFaces * m.nNf, s);
+  else HFX_HIP(hipMemsetAsync(h->g.p, 0, h->g.n * sizeof(double), s));
+  HFX_HIP(hipStreamSynchronize(s));
+  h->has_fields = true;
+  h->assembled = h->solved = false;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_ip_coords(hfx_hdg* h, double* out) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_ip_coords: set the mesh first");
+  h->set_device();
+  const size_t n = (size_t)h->mesh.nCells * h->re->nip() * h->dim;
+  hfx::DevBuf<double> d;
+  d.alloc(n);
+  hfx::launch_ip_coords(h->dim, h->order, h->mesh.nCells, h->tabs.p, h->coords.p, h->cells.p, d.p, h->stream);
+  d.download(out, n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_source(hfx_hdg* h, const double* f_ip, const double* r_ip) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_set_source: set the mesh first");
+  h->set_device();
+  const size_t n = (size_t)h->mesh.nCells * h->re->nip();
+  h->hasF = f_ip != nullptr;
+  h->hasR = r_ip != nullptr;
+  if (f_ip) h->f_ip.upload(f_ip, n, h->stream);
+  if (r_ip) h->r_ip.upload(r_ip, n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  h->assembled = h->solved = false;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_launch(hfx_hdg* h, int threads, int grid_cap) {
+  HFX_TRY
+  require(h, "hfx_hdg_set_launch: null handle");
+  require(threads == 0 || threads == 64 || threads == 128 || threads == 256,
+          "hfx_hdg_set_launch: threads must be 0, 64, 128 or 256");
+  require(grid_cap >= 0, "hfx_hdg_set_launch: grid cap must be >= 0");
+  h->nt = threads;
+  h->grid_cap = grid_cap;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_assemble(hfx_hdg* h) {
+  HFX_TRY
+  require(h && h->has_mesh && h->has_fields, "hfx_hdg_assemble: the solver must be initialized and allocated before assembling");
+  h->set_device();
+  h->run_element();
+  h->run_scatter();
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  h->assembled = true;
+  h->solved = false;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_local_matrices(hfx_hdg* h, int first, int count, double* K, double* F) {
+  HFX_TRY
+  require(h && h->has_mesh && h->has_fields, "hfx_hdg_local_matrices: mesh and fields must be set");
+  require(first >= 0 && count >= 0 && first + count <= h->mesh.nCells, "hfx_hdg_local_matrices: cell range out of bounds");
+  if (count == 0) return 0;
+  h->set_device();
+  const int n = h->re->nN * (h->dim + 1) + h->re->nF * h->re->nNf();
+  hfx::DevBuf<double> dK, dF;
+  dK.alloc((size_t)count * n * n);
+  dF.alloc((size_t)count * n);
+  hfx::ElemArgs a = h->elem_args();
+  a.el_offset = first;
+  a.nEl = count;
+  a.Kloc = dK.p;
+  a.Floc = dF.p;
+  hfx::launch_element(h->dim, h->order, a, h->threads(), h->grid_cap, h->stream);
+  dK.download(K, (size_t)count * n * n, h->stream);
+  dF.download(F, (size_t)count * n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  HFX_END
+}
+
+extern "C" int hfx_hdg_get_condensed(hfx_hdg* h, double* S, double* S0, double* U, double* Q, double* U0, double* Q0) {
+  HFX_TRY
+  require(h && h->assembled, "hfx_hdg_get_condensed: assemble first");
+  h->set_device();
+  if (S) h->S.download(S, h->S.n, h->stream);
+  if (S0) h->S0.download(S0, h->S0.n, h->stream);
+  if (U) h->U.download(U, h->U.n, h->stream);
+  if (Q) h->Q.download(Q, h->Q.n, h->stream);
+  if (U0) h->U0.download(U0, h->U0.n, h->stream);
+  if (Q0) h->Q0.download(Q0, h->Q0.n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  HFX_END
+}
+
+extern "C" int hfx_hdg_trace_sizes(hfx_hdg* h, long long* out) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_trace_sizes: set the mesh first");
+  out[0] = h->A.nrows();
+  out[1] = (long long)h->A.nnzb * h->A.bs * h->A.bs;
+  out[2] = h->A.nnzb;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_get_trace_csr(hfx_hdg* h, long long* rowptr, int* cols, double* vals, double* rhs) {
+  HFX_TRY
+  require(h && h->assembled, "hfx_hdg_get_trace_csr: assemble first");
+  h->set_device();
+  const int bs = h->A.bs;
+  std::vector<double> bv((size_t)h->A.nnzb * bs * bs);
+  h->A.vals.download(bv.data(), bv.size(), h->stream);
+  if (rhs) h->rhs.download(rhs, h->rhs.n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  const auto& P = h->pat;
+  long long o = 0;
+  for (int F = 0; F < P.nb; ++F)
+    for (int r = 0; r < bs; ++r) {
+      if (rowptr) rowptr[(long long)F * bs + r] = o;
+      for (int k = P.rowptr[F]; k < P.rowptr[F + 1]; ++k)
+        for (int c = 0; c < bs; ++c) {
+          if (cols) cols[o] = P.col[k] * bs + c;
+          if (vals) vals[o] = bv[((size_t)k * bs + r) * bs + c];
+          ++o;
+        }
+    }
+  if (rowptr) rowptr[(long long)P.nb * bs] = o;
+  HFX_END
+}
+
+namespace {
+
+void do_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, hfx::KrylovInfo& info) {
+  const long long n = h->A.nrows();
+  hipStream_t s = h->stream;
+  std::function<void(const double*, double*)> M;
+  if (pc == HFX_PC_JACOBI) {
+    h->dinv.alloc(n);
+    hfx::bsr_inv_point_diag(h->A, h->dinv.p, s);
+    M = [h, n, s](const double* in, double* out) { hfx::apply_point_diag(n, h->dinv.p, in, out, s); };
+  } else if (pc == HFX_PC_BJACOBI) {
+    h->dinv.alloc((size_t)h->A.nb * h->A.bs * h->A.bs);
+    hfx::bsr_invert_diag(h->A, h->dinv.p, s);
+    M = [h, s](const double* in, double* out) { hfx::apply_block_diag(h->A.nb, h->A.bs, h->dinv.p, in, out, s); };
+  } else if (pc != HFX_PC_NONE) {
+    throw std::runtime_error("hfx_hdg_solve: unsupported preconditioner for this entry point");
+  }
+  HFX_HIP(hipMemsetAsync(h->lambda.p, 0, n * sizeof(double), s));
+  hfx::KrylovOpts o;
+  o.rtol = rtol;
+  o.maxits = maxits;
+  o.restart = restart;
+  hfx::fgmres(h->A, h->rhs.p, h->lambda.p, M, o, info, s);
+}
+
+}  // namespace
+
+extern "C" int hfx_hdg_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, int* info, double* relres) {
+  HFX_TRY
+  require(h && h->assembled, "hfx_hdg_solve: system must be assembled before solving");
+  require(rtol > 0 && maxits > 0, "hfx_hdg_solve: rtol and maxits must be positive");
+  h->set_device();
+  hfx::KrylovInfo ki;
+  do_solve(h, pc, rtol, maxits, restart, ki);
+  const hfx::Mesh& m = h->mesh;
+  hfx::launch_reconstruct(m.nCells, m.nN, m.nN * h->dim, m.nFpc, m.nNf, h->U.p, h->U0.p, h->Q.p, h->Q0.p, h->c2f.p,
+                          h->perm.p, h->lambda.p, h->u.p, h->q.p, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  if (info) {
+    info[0] = ki.iters;
+    info[1] = ki.reason;
+  }
+  if (relres) relres[0] = ki.true_relres;
+  h->last_pc = pc;
+  h->last_rtol = rtol;
+  h->last_maxits = maxits;
+  h->last_restart = restart;
+  h->solved = true;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_get_solution(hfx_hdg* h, double* u, double* q, double* lambda) {
+  HFX_TRY
+  require(h && h->solved, "hfx_hdg_get_solution: solve first");
+  h->set_device();
+  if (u) h->u.download(u, h->u.n, h->stream);
+  if (q) h->q.download(q, h->q.n, h->stream);
+  if (lambda) h->lambda.download(lambda, h->lambda.n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  HFX_END
+}
+
+extern "C" int hfx_hdg_time(hfx_hdg* h, int what, int warmup, int reps, double* ms) {
+  HFX_TRY
+  require(h && h->has_mesh && h->has_fields, "hfx_hdg_time: mesh and fields must be set");
+  require(what >= 0 && what <= 2 && warmup >= 0 && reps >= 1, "hfx_hdg_time: bad arguments");
+  require(what != 2 || h->solved, "hfx_hdg_time: solve once before timing the solve");
+  h->set_device();
+  hipEvent_t e0, e1;
+  HFX_HIP(hipEventCreate(&e0));
+  HFX_HIP(hipEventCreate(&e1));
+  for (int i = 0; i < warmup + reps; ++i) {
+    HFX_HIP(hipEventRecord(e0, h->stream));
+    if (what == 0) h->run_element();
+    else if (what == 1) { h->run_element(); h->run_scatter(); }
+    else { hfx::KrylovInfo ki; do_solve(h, h->last_pc, h->last_rtol, h->last_maxits, h->last_restart, ki); }
+    HFX_HIP(hipEventRecord(e1, h->stream));
+    HFX_HIP(hipEventSynchronize(e1));
+    float t = 0.f;
+    HFX_HIP(hipEventElapsedTime(&t, e0, e1));
+    if (i >= warmup) ms[i - warmup] = t;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  h->assembled = true;
+  HFX_END
+}
+
+extern "C" int hfx_element_lds_bytes(int dim, int order, long long* bytes) {
+  HFX_TRY
+  *bytes = (long long)hfx::element_lds_bytes(dim, order);
+  HFX_END
+}

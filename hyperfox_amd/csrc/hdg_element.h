@@ -1,0 +1,71 @@
+// Fused per-element HDG assembly + static condensation on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hfx {
+
+// Compile-time sizes of the order-P simplex in dimension D.
+template <int D, int P>
+struct ESz {
+  static constexpr int NN = D == 2 ? (P + 1) * (P + 2) / 2 : (P + 1) * (P + 2) * (P + 3) / 6;
+  static constexpr int NNF = D == 2 ? P + 1 : (P + 1) * (P + 2) / 2;
+  static constexpr int NIP = D == 2 ? (P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : P == 4 ? 16 : 25)
+                                    : (P == 1 ? 4 : P == 2 ? 14 : P == 3 ? 24 : P == 4 ? 46 : 81);
+  static constexpr int NIPF = D == 2 ? P + 1 : (P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : P == 4 ? 16 : 25);
+  static constexpr int NF = D + 1;
+  static constexpr int NL = NF * NNF;
+  static constexpr int NQ = NN * D;
+  static constexpr int NLOC = NN * (D + 1) + NL;
+  // packed reference tables: PHI[NIP][NN] DPHI[NIP][NN][D] FPHI[NIPF][NNF]
+  // FDPHI[NIPF][NNF][D-1] W[NIP] FW[NIPF]
+  static constexpr int T_PHI = 0;
+  static constexpr int T_DPHI = T_PHI + NIP * NN;
+  static constexpr int T_FPHI = T_DPHI + NIP * NN * D;
+  static constexpr int T_FDPHI = T_FPHI + NIPF * NNF;
+  static constexpr int T_W = T_FDPHI + NIPF * NNF * (D - 1);
+  static constexpr int T_FW = T_W + NIP;
+  static constexpr int TAB = T_FW + NIPF;
+};
+
+// Arguments of one element-kernel launch (all device pointers).
+struct ElemArgs {
+  int nEl = 0;
+  int kind = 0;        // 0 Laplace, 1 diffusion-source, 2 convection-diffusion-reaction-source
+  int useDiff = 1;     // diffusion operator present
+  int ndD = 0;         // 0: identity, 1: scalar per node, D*D: column-major tensor per node
+  int hasB = 0, hasF = 0, hasR = 0;
+  int tau_sides = 1;
+  int el_offset = 0;   // first element handled by this launch
+  const double* tabs = nullptr;
+  const int* faceNodes = nullptr;  // NF*NNF
+  const double* coords = nullptr;
+  const int* cells = nullptr;
+  const int* cell2face = nullptr;
+  const int* face2cell = nullptr;
+  const int8_t* face_perm = nullptr;
+  const double* Dn = nullptr;
+  const double* bn = nullptr;
+  const double* tau = nullptr;
+  const double* f_ip = nullptr;
+  const double* r_ip = nullptr;
+  // condensed outputs (element-major, row-major blocks)
+  double *S = nullptr, *S0 = nullptr, *U = nullptr, *Q = nullptr, *U0 = nullptr, *Q0 = nullptr;
+  // optional: full local matrices (NLOC x NLOC) and RHS instead of condensation
+  double *Kloc = nullptr, *Floc = nullptr;
+};
+
+// Launch the element kernel for (dim, order) on `count` elements starting at
+// a.el_offset; `nt` threads per element (64, 128 or 256).  Throws on error.
+void launch_element(int dim, int order, const ElemArgs& a, int nt, int grid_cap, hipStream_t s);
+
+// Dynamic LDS bytes the kernel needs for (dim, order).
+size_t element_lds_bytes(int dim, int order);
+
+// Physical coordinates of the element IPs: out[(e*NIP + q)*D + k].
+void launch_ip_coords(int dim, int order, int nEl, const double* tabs, const double* coords, const int* cells,
+                      double* out, hipStream_t s);
+
+}  // namespace hfx

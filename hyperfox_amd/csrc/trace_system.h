@@ -1,0 +1,51 @@
+// Global trace system on the GPU: block-sparse (BSR) layout with one
+// nNf x nNf dense block per (face, face) coupling, assembled by a
+// deterministic gather from the condensed element blocks, and the
+// element-wise reconstruction of u and q from the trace solution.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "hfx_internal.h"
+
+namespace hfx {
+
+// Host-built BSR pattern of the trace matrix: the block row of face F couples
+// with every face of every cell containing F (HDGSolver.cpp:117-164).
+struct BsrPattern {
+  int nb = 0, bs = 0;                 // block rows (= faces), block size (= nNf)
+  std::vector<int> rowptr, col, rowf;  // nb+1, nnzb, nnzb (block row of each block)
+  std::vector<int> diag;               // nb: index of the diagonal block
+  // contribution of the condensed element blocks to block k:
+  //   c[4k+0] = e0, c[4k+1] = a0*16 + b0, c[4k+2] = e1 (or -1), c[4k+3] = a1
+  std::vector<int> contrib;
+  // per face: the (cell, local face) pairs that hold it: fe[4F..4F+3] = e0 a0 e1 a1
+  std::vector<int> face_cells;
+};
+
+// Build the pattern from cell2face/face2cell (nF faces per cell).
+BsrPattern build_bsr_pattern(int nCells, int nFaces, int nFpc, int bs, const int* cell2face, const int* face2cell);
+
+// Device BSR matrix (row-major dense blocks).
+struct BsrDev {
+  int nb = 0, bs = 0, nnzb = 0;
+  DevBuf<int> rowptr, col, rowf, diag, contrib, face_cells;
+  DevBuf<double> vals;
+  long long nrows() const { return (long long)nb * bs; }
+};
+
+// vals/rhs from condensed S, S0 with Dirichlet rows set to identity/g on the
+// faces flagged in is_dir (DirichletModel Set/Set, HDGSolver.cpp:493-502).
+// invperm[(e*nFpc + lf)*bs + pos] = cell-local node j at face position pos.
+void launch_scatter(const BsrDev& A, int nFpc, const double* S, const double* S0, const int8_t* invperm,
+                    const uint8_t* is_dir, const double* g, double* rhs, hipStream_t s);
+
+// u = U lambda_e + U0, q = Q lambda_e + Q0 (HDGSolver.cpp:741-775).
+void launch_reconstruct(int nEl, int nN, int nQ, int nFpc, int bs, const double* U, const double* U0, const double* Q,
+                        const double* Q0, const int* cell2face, const int8_t* face_perm, const double* lambda, double* u,
+                        double* q, hipStream_t s);
+
+}  // namespace hfx

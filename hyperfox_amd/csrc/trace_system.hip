@@ -1,0 +1,180 @@
+// Global trace system: BSR pattern, deterministic gather-scatter of the
+// condensed element blocks, and element reconstruction.  See trace_system.h.
+#include <algorithm>
+#include <stdexcept>
+
+#include "trace_system.h"
+
+namespace hfx {
+
+BsrPattern build_bsr_pattern(int nCells, int nFaces, int nFpc, int bs, const int* cell2face, const int* face2cell) {
+  BsrPattern P;
+  P.nb = nFaces;
+  P.bs = bs;
+  P.rowptr.assign(nFaces + 1, 0);
+  P.face_cells.assign((size_t)4 * nFaces, -1);
+  auto local_face = [&](int e, int F) {
+    for (int l = 0; l < nFpc; ++l)
+      if (cell2face[(size_t)e * nFpc + l] == F) return l;
+    return -1;
+  };
+  std::vector<int> cols;
+  std::vector<std::vector<int>> rows(nFaces);
+  for (int F = 0; F < nFaces; ++F) {
+    cols.clear();
+    for (int s = 0; s < 2; ++s) {
+      int e = face2cell[2 * (size_t)F + s];
+      if (e < 0) continue;
+      int lf = local_face(e, F);
+      if (lf < 0) throw std::runtime_error("bsr: face2cell and cell2face disagree");
+      P.face_cells[4 * (size_t)F + 2 * s] = e;
+      P.face_cells[4 * (size_t)F + 2 * s + 1] = lf;
+      for (int l = 0; l < nFpc; ++l) cols.push_back(cell2face[(size_t)e * nFpc + l]);
+    }
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    rows[F] = cols;
+    P.rowptr[F + 1] = P.rowptr[F] + (int)cols.size();
+  }
+  const int nnzb = P.rowptr[nFaces];
+  P.col.resize(nnzb);
+  P.rowf.resize(nnzb);
+  P.contrib.assign((size_t)4 * nnzb, -1);
+  P.diag.assign(nFaces, -1);
+  for (int F = 0; F < nFaces; ++F) {
+    int k = P.rowptr[F];
+    const int e0 = P.face_cells[4 * (size_t)F], a0 = P.face_cells[4 * (size_t)F + 1];
+    const int e1 = P.face_cells[4 * (size_t)F + 2], a1 = P.face_cells[4 * (size_t)F + 3];
+    for (int G : rows[F]) {
+      P.col[k] = G;
+      P.rowf[k] = F;
+      int* c = &P.contrib[4 * (size_t)k];
+      if (G == F) {
+        P.diag[F] = k;
+        c[0] = e0; c[1] = a0 * 16 + a0; c[2] = e1; c[3] = e1 >= 0 ? a1 : 0;
+      } else {
+        int b = e0 >= 0 ? local_face(e0, G) : -1;
+        if (b >= 0) { c[0] = e0; c[1] = a0 * 16 + b; }
+        else {
+          b = local_face(e1, G);
+          if (b < 0) throw std::runtime_error("bsr: coupling without a common cell");
+          c[0] = e1; c[1] = a1 * 16 + b;
+        }
+        c[2] = -1; c[3] = 0;
+      }
+      ++k;
+    }
+  }
+  return P;
+}
+
+namespace {
+
+template <int BS>
+__global__ void scatter_kernel(int nnzb, int nFpc, const int* __restrict__ col, const int* __restrict__ rowf,
+                               const int* __restrict__ contrib, const double* __restrict__ S,
+                               const int8_t* __restrict__ invperm, const uint8_t* __restrict__ is_dir,
+                               double* __restrict__ vals) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nnzb * BS * BS) return;
+  const int k = (int)(t / (BS * BS));
+  const int r = (int)((t / BS) % BS), c = (int)(t % BS);
+  const int F = rowf[k];
+  const int NL = nFpc * BS;
+  double v;
+  if (is_dir && is_dir[F]) {
+    v = (col[k] == F && r == c) ? 1.0 : 0.0;
+  } else {
+    const int e0 = contrib[4 * k], ab = contrib[4 * k + 1], e1 = contrib[4 * k + 2], a1 = contrib[4 * k + 3];
+    const int a0 = ab >> 4, b0 = ab & 15;
+    const int lr = a0 * BS + invperm[((size_t)e0 * nFpc + a0) * BS + r];
+    const int lc = b0 * BS + invperm[((size_t)e0 * nFpc + b0) * BS + c];
+    v = S[((size_t)e0 * NL + lr) * NL + lc];
+    if (e1 >= 0) {
+      const int lr1 = a1 * BS + invperm[((size_t)e1 * nFpc + a1) * BS + r];
+      const int lc1 = a1 * BS + invperm[((size_t)e1 * nFpc + a1) * BS + c];
+      v += S[((size_t)e1 * NL + lr1) * NL + lc1];
+    }
+  }
+  vals[t] = v;
+}
+
+__global__ void rhs_kernel(int nb, int bs, int nFpc, const int* __restrict__ face_cells, const double* __restrict__ S0,
+                           const int8_t* __restrict__ invperm, const uint8_t* __restrict__ is_dir,
+                           const double* __restrict__ g, double* __restrict__ rhs) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nb * bs) return;
+  const int F = (int)(t / bs), r = (int)(t % bs);
+  const int NL = nFpc * bs;
+  double v = 0.0;
+  if (is_dir && is_dir[F]) {
+    v = g ? g[t] : 0.0;
+  } else {
+    for (int s = 0; s < 2; ++s) {
+      const int e = face_cells[4 * F + 2 * s], a = face_cells[4 * F + 2 * s + 1];
+      if (e < 0) continue;
+      v += S0[(size_t)e * NL + a * bs + invperm[((size_t)e * nFpc + a) * bs + r]];
+    }
+  }
+  rhs[t] = v;
+}
+
+__global__ void reconstruct_kernel(int nEl, int nN, int nQ, int nFpc, int bs, const double* __restrict__ U,
+                                   const double* __restrict__ U0, const double* __restrict__ Q,
+                                   const double* __restrict__ Q0, const int* __restrict__ cell2face,
+                                   const int8_t* __restrict__ face_perm, const double* __restrict__ lambda,
+                                   double* __restrict__ u, double* __restrict__ q) {
+  const int rows = nN + nQ;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nEl * rows) return;
+  const int e = (int)(t / rows), i = (int)(t % rows);
+  const int NL = nFpc * bs;
+  const double* M = i < nN ? U + ((size_t)e * nN + i) * NL : Q + ((size_t)e * nQ + (i - nN)) * NL;
+  double acc = i < nN ? U0[(size_t)e * nN + i] : Q0[(size_t)e * nQ + (i - nN)];
+  for (int f = 0; f < nFpc; ++f) {
+    const int F = cell2face[(size_t)e * nFpc + f];
+    const int8_t* pp = face_perm + ((size_t)e * nFpc + f) * bs;
+    for (int j = 0; j < bs; ++j) acc += M[f * bs + j] * lambda[(size_t)F * bs + pp[j]];
+  }
+  if (i < nN) u[(size_t)e * nN + i] = acc;
+  else q[(size_t)e * nQ + (i - nN)] = acc;
+}
+
+}  // namespace
+
+void launch_scatter(const BsrDev& A, int nFpc, const double* S, const double* S0, const int8_t* invperm,
+                    const uint8_t* is_dir, const double* g, double* rhs, hipStream_t s) {
+  const long long n = (long long)A.nnzb * A.bs * A.bs;
+  const int nt = 256;
+#define HFX_BS(B)                                                                                               \
+  case B:                                                                                                       \
+    hipLaunchKernelGGL((scatter_kernel<B>), dim3(ceil_div(n, nt)), dim3(nt), 0, s, A.nnzb, nFpc, A.col.p,      \
+                       A.rowf.p, A.contrib.p, S, invperm, is_dir, A.vals.p);                                    \
+    break;
+  if (n > 0) {
+    switch (A.bs) {
+      HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(10)
+      default: throw std::runtime_error("scatter: unsupported block size");
+    }
+    HFX_HIP(hipGetLastError());
+  }
+#undef HFX_BS
+  const long long nr = A.nrows();
+  if (nr > 0) {
+    hipLaunchKernelGGL(rhs_kernel, dim3(ceil_div(nr, nt)), dim3(nt), 0, s, A.nb, A.bs, nFpc, A.face_cells.p, S0,
+                       invperm, is_dir, g, rhs);
+    HFX_HIP(hipGetLastError());
+  }
+}
+
+void launch_reconstruct(int nEl, int nN, int nQ, int nFpc, int bs, const double* U, const double* U0, const double* Q,
+                        const double* Q0, const int* cell2face, const int8_t* face_perm, const double* lambda, double* u,
+                        double* q, hipStream_t s) {
+  const long long n = (long long)nEl * (nN + nQ);
+  if (n <= 0) return;
+  hipLaunchKernelGGL(reconstruct_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, nEl, nN, nQ, nFpc, bs, U, U0, Q, Q0,
+                     cell2face, face_perm, lambda, u, q);
+  HFX_HIP(hipGetLastError());
+}
+
+}  // namespace hfx

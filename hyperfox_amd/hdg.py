@@ -1,0 +1,198 @@
+"""Python handles over the C ABI of libhfx_hip.so (include/hfx_hip.h).
+
+These mirror the reference's set-up sequence (Mesh -> fields -> Solver
+initialize/allocate/assemble/solve, src/solver/HDGSolver.cpp:5-779) for the
+tests and the benchmark.  All compute runs in the HIP library.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def refel_sizes(dim, order):
+    out = np.zeros(7, dtype=np.int32)
+    check(lib.hfx_refel_sizes(dim, order, _ptr(out)))
+    keys = ('nN', 'nF', 'nNf', 'nIP', 'nIPf', 'n_local', 'lenL')
+    return dict(zip(keys, (int(v) for v in out)))
+
+
+def refel_tables(dim, order):
+    s = refel_sizes(dim, order)
+    nN, nF, nNf, nIP, nIPf = s['nN'], s['nF'], s['nNf'], s['nIP'], s['nIPf']
+    t = dict(nodes=np.zeros((nN, dim)), face_nodes=np.zeros((nF, nNf), dtype=np.int32),
+             ipx=np.zeros((nIP, dim)), ipw=np.zeros(nIP), phi=np.zeros((nIP, nN)),
+             dphi=np.zeros((nIP, nN, dim)), fipx=np.zeros((nIPf, dim - 1)), fipw=np.zeros(nIPf),
+             fphi=np.zeros((nIPf, nNf)))
+    check(lib.hfx_refel_tables(dim, order, *[_ptr(t[k]) for k in
+                                             ('nodes', 'face_nodes', 'ipx', 'ipw', 'phi', 'dphi',
+                                              'fipx', 'fipw', 'fphi')]))
+    return t
+
+
+def cubature(dim, order):
+    n = lib.hfx_cubature(dim, order, None, None)
+    if n < 0:
+        check(-1)
+    pts = np.zeros((n, dim))
+    w = np.zeros(n)
+    lib.hfx_cubature(dim, order, _ptr(pts), _ptr(w))
+    return pts, w
+
+
+class Mesh:
+    """Simplex mesh with the reference's topology conventions."""
+
+    def __init__(self, handle, dim, order):
+        self._h = handle
+        self.dim = dim
+        self.order = order
+        out = np.zeros(5, dtype=np.int64)
+        check(lib.hfx_mesh_sizes(self._h, _ptr(out)))
+        self.n_nodes, self.n_cells, self.n_faces, self.n_boundary, self.n_trace = (int(v) for v in out)
+        self.sizes = refel_sizes(dim, order)
+        self._arrays = None
+
+    @classmethod
+    def structured(cls, dim, order, n, jitter=0.0, seed=0):
+        h = ctypes.c_void_p()
+        check(lib.hfx_mesh_structured(dim, order, n, float(jitter), ctypes.c_ulonglong(seed), ctypes.byref(h)))
+        return cls(h, dim, order)
+
+    @classmethod
+    def from_cells(cls, dim, order, nodes, cells):
+        nodes = _f64(nodes)
+        cells = _i32(cells)
+        h = ctypes.c_void_p()
+        check(lib.hfx_mesh_from_cells(dim, order, nodes.shape[0], _ptr(nodes), cells.shape[0], _ptr(cells),
+                                      ctypes.byref(h)))
+        return cls(h, dim, order)
+
+    def arrays(self):
+        if self._arrays is None:
+            s = self.sizes
+            a = dict(coords=np.zeros((self.n_nodes, self.dim)),
+                     cells=np.zeros((self.n_cells, s['nN']), dtype=np.int32),
+                     faces=np.zeros((self.n_faces, s['nNf']), dtype=np.int32),
+                     cell2face=np.zeros((self.n_cells, s['nF']), dtype=np.int32),
+                     face2cell=np.zeros((self.n_faces, 2), dtype=np.int32),
+                     boundary=np.zeros(self.n_boundary, dtype=np.int32),
+                     face_perm=np.zeros((self.n_cells, s['nF'], s['nNf']), dtype=np.int8))
+            check(lib.hfx_mesh_get(self._h, *[_ptr(a[k]) for k in
+                                             ('coords', 'cells', 'faces', 'cell2face', 'face2cell',
+                                              'boundary', 'face_perm')]))
+            self._arrays = a
+        return self._arrays
+
+    def __del__(self):
+        if getattr(self, '_h', None):
+            lib.hfx_mesh_free(self._h)
+            self._h = None
+
+
+class HDGProblem:
+    """One HDG problem resident on a GPU."""
+
+    def __init__(self, dim, order, device=0):
+        self.dim = dim
+        self.order = order
+        self.sizes = refel_sizes(dim, order)
+        h = ctypes.c_void_p()
+        check(lib.hfx_hdg_create(device, dim, order, ctypes.byref(h)))
+        self._h = h
+        self.mesh = None
+
+    def __del__(self):
+        if getattr(self, '_h', None):
+            lib.hfx_hdg_destroy(self._h)
+            self._h = None
+
+    def set_mesh(self, mesh):
+        check(lib.hfx_hdg_set_mesh(self._h, mesh._h))
+        self.mesh = mesh
+
+    def set_model(self, kind):
+        check(lib.hfx_hdg_set_model(self._h, kind))
+
+    def set_fields(self, tau, tau_sides=1, D=None, ndD=0, b=None, dirichlet=None):
+        self._keep = [_f64(tau), _f64(D), _f64(b), _f64(dirichlet)]
+        t, d, bb, g = self._keep
+        check(lib.hfx_hdg_set_fields(self._h, _ptr(t), tau_sides, _ptr(d), ndD if d is not None else 0,
+                                     _ptr(bb), _ptr(g)))
+
+    def ip_coords(self):
+        out = np.zeros((self.mesh.n_cells, self.sizes['nIP'], self.dim))
+        check(lib.hfx_hdg_ip_coords(self._h, _ptr(out)))
+        return out
+
+    def set_source(self, f_ip=None, r_ip=None):
+        f = _f64(f_ip)
+        r = _f64(r_ip)
+        check(lib.hfx_hdg_set_source(self._h, _ptr(f), _ptr(r)))
+
+    def set_launch(self, threads=0, grid_cap=0):
+        check(lib.hfx_hdg_set_launch(self._h, threads, grid_cap))
+
+    def assemble(self):
+        check(lib.hfx_hdg_assemble(self._h))
+
+    def local_matrices(self, first, count):
+        n = self.sizes['n_local']
+        K = np.zeros((count, n, n))
+        F = np.zeros((count, n))
+        check(lib.hfx_hdg_local_matrices(self._h, first, count, _ptr(K), _ptr(F)))
+        return K, F
+
+    def condensed(self):
+        s, ne, d = self.sizes, self.mesh.n_cells, self.dim
+        nN, nL = s['nN'], s['lenL']
+        out = dict(S=np.zeros((ne, nL, nL)), S0=np.zeros((ne, nL)), U=np.zeros((ne, nN, nL)),
+                   Q=np.zeros((ne, nN * d, nL)), U0=np.zeros((ne, nN)), Q0=np.zeros((ne, nN * d)))
+        check(lib.hfx_hdg_get_condensed(self._h, *[_ptr(out[k]) for k in ('S', 'S0', 'U', 'Q', 'U0', 'Q0')]))
+        return out
+
+    def trace_sizes(self):
+        out = np.zeros(3, dtype=np.int64)
+        check(lib.hfx_hdg_trace_sizes(self._h, _ptr(out)))
+        return tuple(int(v) for v in out)
+
+    def trace_csr(self):
+        n, nnz, _ = self.trace_sizes()
+        rowptr = np.zeros(n + 1, dtype=np.int64)
+        cols = np.zeros(nnz, dtype=np.int32)
+        vals = np.zeros(nnz)
+        rhs = np.zeros(n)
+        check(lib.hfx_hdg_get_trace_csr(self._h, _ptr(rowptr), _ptr(cols), _ptr(vals), _ptr(rhs)))
+        return rowptr, cols, vals, rhs
+
+    def solve(self, pc=2, rtol=1e-10, maxits=10000, restart=30):
+        info = np.zeros(2, dtype=np.int32)
+        rel = np.zeros(1)
+        check(lib.hfx_hdg_solve(self._h, pc, rtol, maxits, restart, _ptr(info), _ptr(rel)))
+        return int(info[0]), int(info[1]), float(rel[0])
+
+    def solution(self):
+        s, ne, d = self.sizes, self.mesh.n_cells, self.dim
+        u = np.zeros((ne, s['nN']))
+        q = np.zeros((ne, s['nN'] * d))
+        lam = np.zeros(self.mesh.n_trace)
+        check(lib.hfx_hdg_get_solution(self._h, _ptr(u), _ptr(q), _ptr(lam)))
+        return u, q, lam
+
+    def time(self, what, warmup=1, reps=3):
+        ms = np.zeros(reps)
+        check(lib.hfx_hdg_time(self._h, what, warmup, reps, _ptr(ms)))
+        return ms

@@ -1,0 +1,139 @@
+"""Synthetic HDG workloads (manufactured solutions on structured simplex meshes).
+
+Each workload returns the inputs the reference's drivers build by hand in
+tests/regression/HDG/*.cpp: the mesh, the model kind, the field map
+(Tau, DiffusionTensor, Velocity, Dirichlet) and the source at the element
+integration points.
+"""
+import math
+
+import numpy as np
+
+from .hdg import Mesh
+
+PI = math.pi
+
+
+def face_normals(m):
+    """Unit normal of every (straight) face, outward from its first cell."""
+    a = m.arrays()
+    x = a['coords']
+    faces, f2c, cells = a['faces'], a['face2cell'], a['cells']
+    dim = x.shape[1]
+    fv = x[faces[:, :dim]]                       # face vertices
+    if dim == 2:
+        t = fv[:, 1] - fv[:, 0]
+        n = np.stack([t[:, 1], -t[:, 0]], axis=1)
+    else:
+        n = np.cross(fv[:, 1] - fv[:, 0], fv[:, 2] - fv[:, 0])
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    cv = x[cells[f2c[:, 0], :dim + 1]]           # first cell's vertices
+    centroid = cv.mean(axis=1)
+    s = np.sign(np.einsum('fd,fd->f', fv[:, 0] - centroid, n))
+    return n * s[:, None]
+
+
+def dirichlet_from(m, u_exact):
+    a = m.arrays()
+    g = np.zeros(a['faces'].shape)
+    xb = a['coords'][a['faces'][a['boundary']]]
+    g[a['boundary']] = u_exact(xb)
+    return g
+
+
+def u_sin2d(x):
+    return np.sin(PI * x[..., 0]) * np.sin(PI * x[..., 1])
+
+
+def f_sin2d(x, D=1.0):
+    return 2.0 * PI * PI * D * u_sin2d(x)
+
+
+def u_c3(x):
+    X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+    return np.exp(X + Y + Z) * np.sin(PI * X) * np.sin(PI * Y) * np.sin(PI * Z)
+
+
+def grad_lap_c3(x):
+    X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
+    e = np.exp(X + Y + Z)
+    s = [np.sin(PI * X), np.sin(PI * Y), np.sin(PI * Z)]
+    c = [np.cos(PI * X), np.cos(PI * Y), np.cos(PI * Z)]
+    g = []
+    lap = 0.0
+    for k in range(3):
+        o = [s[j] for j in range(3) if j != k]
+        # d/dx_k (e s_k) = e (s_k + pi c_k);  d2 = e (s_k + 2 pi c_k - pi^2 s_k)
+        g.append(e * (s[k] + PI * c[k]) * o[0] * o[1])
+        lap = lap + e * (s[k] + 2 * PI * c[k] - PI * PI * s[k]) * o[0] * o[1]
+    return np.stack(g, axis=-1), lap
+
+
+def vel_c3(x):
+    return np.stack([np.sin(2 * PI * x[..., 1]), np.sin(2 * PI * x[..., 2]), np.sin(2 * PI * x[..., 0])], axis=-1)
+
+
+def f_c3(x, D):
+    g, lap = grad_lap_c3(x)
+    b = vel_c3(x)
+    return np.einsum('...k,...k->...', b, g) - D * lap
+
+
+def upwind_tau(m, b_nodes, D, ell):
+    """Double-valued tau: |b.n| + D/ell on the upwind side, D/ell on the other
+    (tests/regression/HDG/TestHDGConvectionDiffusionReactionSource.cpp:182-202)."""
+    a = m.arrays()
+    n = face_normals(m)
+    bn = np.einsum('fjd,fd->fj', b_nodes[a['faces']], n)
+    tau = np.empty(a['faces'].shape + (2,))
+    base = D / ell
+    tau[..., 0] = np.where(bn > 0, np.abs(bn) + base, base)
+    tau[..., 1] = np.where(bn > 0, base, np.abs(bn) + base)
+    return tau
+
+
+class Workload:
+    """Inputs of one HDG problem, shared by the GPU path and the oracle."""
+
+    def __init__(self, name, dim, p, mesh, kind, tau, tau_sides=1, D=None, ndD=0, b=None, dirichlet=None,
+                 f_fn=None, u_exact=None, h=None):
+        self.name, self.dim, self.p, self.mesh, self.kind = name, dim, p, mesh, kind
+        self.tau, self.tau_sides, self.D, self.ndD, self.b = tau, tau_sides, D, ndD, b
+        self.dirichlet, self.f_fn, self.u_exact, self.h = dirichlet, f_fn, u_exact, h
+
+    def source_at(self, xip):
+        return None if self.f_fn is None else np.ascontiguousarray(self.f_fn(xip))
+
+
+def diffusion_source_2d(n, p, jitter=0.1, seed=1234, name=None):
+    """Unit square, n x n squares split in two, D = I, f = 2 pi^2 sin(pi x) sin(pi y),
+    u = sin(pi x) sin(pi y), tau = 1 (the C1/C2 workloads)."""
+    m = Mesh.structured(2, p, n, jitter, seed)
+    tau = np.ones((m.n_faces, m.sizes['nNf']))
+    return Workload(name or 'diffusion-source-2d-n%d-p%d' % (n, p), 2, p, m, 1, tau, 1,
+                    dirichlet=dirichlet_from(m, u_sin2d), f_fn=f_sin2d, u_exact=u_sin2d, h=1.0 / n)
+
+
+def convection_diffusion_3d(n, p, D=1e-2, jitter=0.05, seed=4321, name=None):
+    """Unit cube, n^3 cubes split in six tetrahedra, D = 1e-2, b = (sin 2pi y,
+    sin 2pi z, sin 2pi x), u = e^{x+y+z} sin sin sin, upwind double-valued tau
+    with ell = h (the C3 workload)."""
+    m = Mesh.structured(3, p, n, jitter, seed)
+    x = m.arrays()['coords']
+    b = vel_c3(x)
+    h = 1.0 / n
+    tau = upwind_tau(m, b, D, h)
+    Dn = np.full(m.n_nodes, D)
+    return Workload(name or 'convection-diffusion-3d-n%d-p%d' % (n, p), 3, p, m, 2, tau, 2, D=Dn, ndD=1, b=b,
+                    dirichlet=dirichlet_from(m, u_c3), f_fn=lambda xx: f_c3(xx, D), u_exact=u_c3, h=h)
+
+
+def setup_gpu(w, prob):
+    """Feed a workload to an HDGProblem; returns the IP source table used."""
+    prob.set_mesh(w.mesh)
+    prob.set_model(w.kind)
+    prob.set_fields(w.tau, w.tau_sides, w.D, w.ndD, w.b, w.dirichlet)
+    xip = prob.ip_coords()
+    f = w.source_at(xip)
+    prob.set_source(f, None)
+    return f

@@ -1,0 +1,163 @@
+/* C ABI of libhfx_hip.so: the MI355X (gfx950) HDG element-by-element hot path.
+ *
+ * Every entry point returns 0 on success and -1 on failure; the failure text is
+ * available from hfx_last_error() in the calling thread.  Plain pointers and
+ * sizes only.  All double arrays are FP64, all index arrays int32 unless noted,
+ * all 2-D arrays row-major.  Host pointers are copied; the library owns its
+ * device memory.  Calls on one handle are stream-ordered and return after the
+ * work they describe has completed.
+ *
+ * The groups below replace, in the reference (julienfausty/HyperFox):
+ *   mesh      src/mesh/Mesh.cpp:31-46, 183-537 (setMesh, computeFaces, orderings)
+ *   refel     src/element/ReferenceElement.cpp:5-26, Cubature.cpp:52-59
+ *   hdg       src/solver/HDGSolver.cpp:5-106 (allocate), 166-675 (assemble),
+ *             677-779 (solve + reconstruction), with the HDG models of
+ *             src/model/HDG*.cpp:* and operators of src/operator/HDG*.cpp:*
+ *   la        src/resolution/LinAlgebraInterface.h:23-162 (the plug point),
+ *             behaviour of src/resolution/PetscInterface.cpp:5-265
+ */
+#ifndef HFX_HIP_H
+#define HFX_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Last error message of the calling thread ("" if none). */
+const char* hfx_last_error(void);
+
+/* Number of visible GPUs (0 when no GPU).  Does not create a context. */
+int hfx_device_count(int* n);
+
+/* ------------------------------------------------------------ reference element */
+
+/* out[0..6] = nN, nF (faces per cell), nNf, nIP, nIPf, n_local, lenL
+ * (ReferenceElement::getNumNodes/.../Cubature::getNumIPs, HDGOperator.cpp:5-15). */
+int hfx_refel_sizes(int dim, int order, int* out);
+
+/* Host tables of the reference element (ReferenceElement getters,
+ * ReferenceElement.h:29-119).  Any pointer may be NULL.  nodes nN*dim,
+ * face_nodes nF*nNf, ipx nIP*dim, ipw nIP, phi nIP*nN, dphi nIP*nN*dim,
+ * fipx nIPf*(dim-1), fipw nIPf, fphi nIPf*nNf. */
+int hfx_refel_tables(int dim, int order, double* nodes, int* face_nodes, double* ipx, double* ipw, double* phi,
+                     double* dphi, double* fipx, double* fipw, double* fphi);
+
+/* Cubature on the dim-simplex exact to `order` (Cubature::Cubature,
+ * Cubature.cpp:5-26): returns the point count (or -1), fills pts/w if non-NULL. */
+int hfx_cubature(int dim, int order, double* pts, double* w);
+
+/* ----------------------------------------------------------------------- mesh */
+
+typedef struct hfx_mesh hfx_mesh;
+
+/* Unit square (dim 2, n x n squares, 2 triangles each) or cube (dim 3, n^3
+ * cubes, 6 tetrahedra each) with interior vertices jittered by
+ * U(-jitter*h, jitter*h) from a splitmix64 stream seeded with `seed`;
+ * high-order nodes by the affine map of tools/convertGmsh2H5HO.cpp:367-397. */
+int hfx_mesh_structured(int dim, int order, int n, double jitter, unsigned long long seed, hfx_mesh** out);
+
+/* Mesh from high-order cells (HDF5Io::load + Mesh::setMesh,
+ * HDF5Io.cpp:13-62, Mesh.cpp:31-46): cells nCells*nN global node ids in the
+ * reference node order, coords nNodes*dim. */
+int hfx_mesh_from_cells(int dim, int order, int nNodes, const double* coords, int nCells, const int* cells,
+                        hfx_mesh** out);
+
+/* out[0..4] = nNodes, nCells, nFaces, nBoundaryFaces, trace DOFs. */
+int hfx_mesh_sizes(const hfx_mesh* m, long long* out);
+
+/* Copies of the topology (Mesh getters, Mesh.cpp:145-161, 292-295, 435-448,
+ * 513-515).  Any pointer may be NULL.  faces nFaces*nNf (first cell's local
+ * face-node order), cell2face nCells*nF, face2cell nFaces*2 (-1 = boundary),
+ * boundary ascending face ids, face_perm nCells*nF*nNf int8 (face2CellMap of
+ * HDGSolver.cpp:258-275). */
+int hfx_mesh_get(const hfx_mesh* m, double* coords, int* cells, int* faces, int* cell2face, int* face2cell,
+                 int* boundary, signed char* face_perm);
+
+void hfx_mesh_free(hfx_mesh* m);
+
+/* ------------------------------------------------------- HDG device bulk path */
+
+typedef struct hfx_hdg hfx_hdg;
+
+/* Model kinds: HDGLaplaceModel, HDGDiffusionSource,
+ * HDGConvectionDiffusionReactionSource (src/model/HDG*.cpp). */
+enum { HFX_MODEL_LAPLACE = 0, HFX_MODEL_DIFFUSION_SOURCE = 1, HFX_MODEL_CDR_SOURCE = 2 };
+
+/* Trace preconditioners. */
+enum { HFX_PC_NONE = 0, HFX_PC_JACOBI = 1, HFX_PC_BJACOBI = 2, HFX_PC_LEARNED = 3 };
+
+/* Create a problem on GPU `device` for simplices of `dim` and `order`
+ * (HDGSolver construction + initialize, Solver.cpp:5-12). */
+int hfx_hdg_create(int device, int dim, int order, hfx_hdg** out);
+void hfx_hdg_destroy(hfx_hdg* h);
+
+/* Upload the mesh (Solver::setMesh + HDGSolver::allocate's pattern, HDGSolver.cpp:117-164). */
+int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* m);
+
+/* Model kind (Solver::setModel). */
+int hfx_hdg_set_model(hfx_hdg* h, int kind);
+
+/* Field map (Solver::setFieldMap; field layouts checked as in HDGSolver.cpp:24-73):
+ *   tau       nFaces*nNf*tau_sides ("Tau" face field; tau_sides 2 = double valued,
+ *             side 0 belongs to face2cell[0])
+ *   D         nNodes*ndD or NULL ("DiffusionTensor" node field, ndD = 1 or dim*dim
+ *             column-major); NULL = identity for kinds 0/1, no diffusion for kind 2
+ *   b         nNodes*dim or NULL ("Velocity" node field)
+ *   dirichlet nFaces*nNf or NULL ("Dirichlet" face field, used on boundary faces). */
+int hfx_hdg_set_fields(hfx_hdg* h, const double* tau, int tau_sides, const double* D, int ndD, const double* b,
+                       const double* dirichlet);
+
+/* Physical IP coordinates computed on the GPU, out nCells*nIP*dim, so the
+ * caller can evaluate its source/reaction functions once per problem
+ * (replaces the per-IP callbacks of Source.cpp:5-22, Reaction.cpp:5-20). */
+int hfx_hdg_ip_coords(hfx_hdg* h, double* out);
+
+/* Source and reaction values at the element IPs (nCells*nIP each, or NULL). */
+int hfx_hdg_set_source(hfx_hdg* h, const double* f_ip, const double* r_ip);
+
+/* Launch shape of the element kernel: threads per element (64/128/256) and
+ * a cap on the grid (0 = one workgroup per element). */
+int hfx_hdg_set_launch(hfx_hdg* h, int threads, int grid_cap);
+
+/* Elemental matrices + static condensation + Dirichlet rows + trace-matrix
+ * scatter, all on the GPU (HDGSolver::assemble, HDGSolver.cpp:166-675). */
+int hfx_hdg_assemble(hfx_hdg* h);
+
+/* Assembly-only: local matrices (n_local^2, reference unknown order
+ * [u | q (i*dim+k) | lambda]) and RHS of cells [first, first+count)
+ * (FEModel::getLocalMatrix/getLocalRHS after Model::compute, HDGModel.cpp:35-51). */
+int hfx_hdg_local_matrices(hfx_hdg* h, int first, int count, double* K, double* F);
+
+/* Condensed blocks of every cell before the Dirichlet rows are applied,
+ * row-major: S lenL^2, S0 lenL, U nN*lenL, Q (nN*dim)*lenL, U0 nN, Q0 nN*dim. */
+int hfx_hdg_get_condensed(hfx_hdg* h, double* S, double* S0, double* U, double* Q, double* U0, double* Q0);
+
+/* out[0..2] = trace DOFs, scalar nnz, block nnz. */
+int hfx_hdg_trace_sizes(hfx_hdg* h, long long* out);
+
+/* Trace system as CSR (rows sorted by column) and its RHS (after assemble). */
+int hfx_hdg_get_trace_csr(hfx_hdg* h, long long* rowptr, int* cols, double* vals, double* rhs);
+
+/* Trace solve (LinAlgebraInterface::solve, HDGSolver.cpp:699-708): restarted
+ * FGMRES in FP64 with preconditioner `pc`, zero initial guess, stopping when
+ * the true residual ||b - A lambda||_2 / ||b||_2 <= rtol.  info[0] =
+ * iterations, info[1] = converged reason (2 rtol, 3 zero rhs, -3 max its,
+ * -9 nan); relres[0] = final true relative residual. */
+int hfx_hdg_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, int* info, double* relres);
+
+/* Solution fields after solve (HDGSolver.cpp:741-775): u nCells*nN ("Solution"),
+ * q nCells*nN*dim ("Flux"), lambda nFaces*nNf ("Trace").  Any may be NULL. */
+int hfx_hdg_get_solution(hfx_hdg* h, double* u, double* q, double* lambda);
+
+/* Timed repetitions with HIP events on the problem's stream:
+ * what = 0 element kernel only, 1 element kernel + scatter (assembly+condense),
+ * 2 trace solve with the last solve's options.  ms receives `reps` values. */
+int hfx_hdg_time(hfx_hdg* h, int what, int warmup, int reps, double* ms);
+
+/* Dynamic LDS bytes and threads used by the element kernel of (dim, order). */
+int hfx_element_lds_bytes(int dim, int order, long long* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

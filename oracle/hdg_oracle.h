@@ -1,0 +1,114 @@
+/* FP64 CPU oracle for the HDG element-by-element path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path is
+ * compared against, and the CPU baseline the benchmark times beside it.  Only
+ * the tests, the smoke check and the benchmark's CPU-baseline leg may load it;
+ * the product library never links or calls it.
+ *
+ * It restates, in plain C++ with no external libraries, the reference's:
+ *   ReferenceElement / Cubature      (src/element/ReferenceElement.cpp, Cubature.cpp)
+ *   Operator::calcJacobians & co     (src/operator/Operator.cpp:14-84)
+ *   HDGBase / HDGDiffusion / HDGConvection / Convection / Mass / Source / Reaction
+ *   HDG models                       (src/model/HDG*.cpp)
+ *   HDGSolver::calcElementalMatrices / applyBoundaryConditions / assembleSystem / solve
+ *                                    (src/solver/HDGSolver.cpp:176-779)
+ *   PetscInterface KSP semantics     (GMRES(restart) + left Jacobi, x0 = 0)
+ *
+ * Parity pinning of each function is listed in the tests that exercise it
+ * (tests/test_oracle_*.py).
+ */
+#ifndef HFX_HDG_ORACLE_H
+#define HFX_HDG_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Last error message of the calling thread ("" if none). */
+const char* orc_last_error(void);
+
+/* Sizes of the order-p simplex: out[0..6] = nN, nF, nNf, nIP, nIPf, n_local, lenL. */
+int orc_sizes(int dim, int p, int* out);
+
+/* Reference tables.  Any pointer may be NULL.
+ * nodes nN*dim, face_nodes nF*nNf, ipx nIP*dim, ipw nIP, phi nIP*nN,
+ * dphi nIP*nN*dim, fipx nIPf*(dim-1), fipw nIPf, fphi nIPf*nNf. */
+int orc_refel(int dim, int p, double* nodes, int* face_nodes, double* ipx, double* ipw, double* phi,
+              double* dphi, double* fipx, double* fipw, double* fphi);
+
+/* Cubature rule of the given order on the dim-simplex: returns nIP, fills
+ * pts (nIP*dim) and w (nIP) when non-NULL. */
+int orc_cubature(int dim, int order, double* pts, double* w);
+
+/* Model description shared by the local and global entry points.
+ * kind: 0 HDGLaplaceModel, 1 HDGDiffusionSource, 2 HDGConvectionDiffusionReactionSource.
+ * D: per element node, NULL (identity for kinds 0/1, absent for kind 2),
+ *    ndD = 1 (scalar) or dim*dim (column-major tensor).
+ * b: per element node dim values or NULL.
+ * f_ip, r_ip: source / reaction at the element IPs (nIP) or NULL. */
+typedef struct {
+  int kind;
+  int ndD;
+} orc_model;
+
+/* Local matrix (n x n, row-major) and RHS (n) of one element, with unknowns
+ * ordered [u (nN) | q (nN*dim, (i*dim+k)) | lambda (nF*nNf, cell-local)].
+ * x: nN*dim node coords; tau: nF*nNf in cell-local face-node order. */
+int orc_local_system(int dim, int p, orc_model model, const double* x, const double* D, const double* b,
+                     const double* tau, const double* f_ip, const double* r_ip, double* K, double* F);
+
+/* Physical coordinates of the element IPs (nIP*dim). */
+int orc_ip_coords(int dim, int p, const double* x, double* out);
+
+/* Static condensation of one local system by Householder QR, as
+ * HDGSolver.cpp:331-348.  Outputs are row-major:
+ * S lenL*lenL, S0 lenL, U lenU*lenL, Q lenQ*lenL, U0 lenU, Q0 lenQ. */
+int orc_condense(int dim, int p, const double* K, const double* F, double* S, double* S0, double* U, double* Q,
+                 double* U0, double* Q0);
+
+/* Whole-mesh pipeline: elemental matrices, Dirichlet BCs, CSR trace system.
+ * Mesh arrays follow the reference's Mesh: cells nEl*nN (global node ids),
+ * faces nFaces*nNf, cell2face nEl*nF, face2cell nFaces*2.
+ * Node fields: D nNodes*ndD (or NULL), b nNodes*dim (or NULL).
+ * tau: nFaces*nNf*tau_sides (tau_sides = 1, or 2 for a double-valued field,
+ *      side 0 belonging to face2cell[0]).
+ * f_ip, r_ip: nEl*nIP or NULL.  dirichlet: nFaces*nNf values (used on
+ * boundary faces; NULL means 0).
+ * Returns an opaque handle; free with orc_system_free. */
+typedef struct orc_system orc_system;
+orc_system* orc_system_build(int dim, int p, orc_model model, int nNodes, const double* coords, int nEl,
+                             const int* cells, int nFaces, const int* faces, const int* cell2face,
+                             const int* face2cell, const double* D, const double* b, const double* tau,
+                             int tau_sides, const double* f_ip, const double* r_ip, const double* dirichlet,
+                             int nthreads);
+void orc_system_free(orc_system* s);
+/* sizes: out[0]=ndofs, out[1]=nnz */
+int orc_system_sizes(const orc_system* s, long long* out);
+int orc_system_csr(const orc_system* s, long long* rowptr, int* cols, double* vals, double* rhs);
+/* Per-element condensed blocks after BCs (row-major, element-major). */
+int orc_system_condensed(const orc_system* s, double* S, double* S0, double* U, double* Q, double* U0, double* Q0);
+
+/* Solve the trace system.  method 0: GMRES(restart) with left Jacobi, zero
+ * initial guess, stop on the preconditioned residual <= rtol*||P^-1 b||
+ * (PETSc's KSPGMRES + PCJACOBI defaults); method 1: dense LU (small systems).
+ * info[0]=iterations, info[1]=converged(1)/not(0); relres[0] = true
+ * ||b-Ax||/||b||. */
+int orc_system_solve(orc_system* s, int method, double rtol, int maxits, int restart, double* lambda, int* info,
+                     double* relres);
+
+/* Element reconstruction u = U lambda_e + U0, q = Q lambda_e + Q0. */
+int orc_system_reconstruct(const orc_system* s, const double* lambda, double* u, double* q);
+
+/* Wall time (s) of one [elemental matrices + condensation + BC + scatter]
+ * pass over the first nEl_sample elements (the CPU baseline). */
+double orc_system_time_assembly(orc_system* s, int nEl_sample, int nthreads);
+
+/* Dense-matrix GMRES test hook for the LinAlgebraInterface known-answer
+ * systems: solves A x = b (A n x n row-major) with GMRES(restart)+Jacobi. */
+int orc_dense_gmres(int n, const double* A, const double* b, double rtol, int maxits, int restart, double* x,
+                    int* iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

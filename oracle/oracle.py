@@ -1,0 +1,191 @@
+"""ctypes binding of the FP64 CPU oracle (liboracle_hdg.so).
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+benchmark's CPU-baseline leg, never by the product (hyperfox_amd).  See
+hdg_oracle.h for what it restates from the reference and where.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'liboracle_hdg.so')
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Model(ctypes.Structure):
+    _fields_ = [('kind', ctypes.c_int), ('ndD', ctypes.c_int)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError('oracle not built: run make -C oracle')
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, D, LL = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_longlong
+    sig = {
+        'orc_last_error': ([], ctypes.c_char_p),
+        'orc_sizes': ([I, I, P], I),
+        'orc_refel': ([I, I] + [P] * 9, I),
+        'orc_cubature': ([I, I, P, P], I),
+        'orc_local_system': ([I, I, Model] + [P] * 8, I),
+        'orc_ip_coords': ([I, I, P, P], I),
+        'orc_condense': ([I, I] + [P] * 8, I),
+        'orc_system_build': ([I, I, Model, I, P, I, P, I, P, P, P, P, P, P, I, P, P, P, I], P),
+        'orc_system_free': ([P], None),
+        'orc_system_sizes': ([P, P], I),
+        'orc_system_csr': ([P] * 5, I),
+        'orc_system_condensed': ([P] * 7, I),
+        'orc_system_solve': ([P, I, D, I, I, P, P, P], I),
+        'orc_system_reconstruct': ([P, P, P, P], I),
+        'orc_system_time_assembly': ([P, I, I], D),
+        'orc_dense_gmres': ([I, P, P, D, I, I, P, P], I),
+    }
+    for n, (a, r) in sig.items():
+        f = getattr(L, n)
+        f.argtypes = a
+        f.restype = r
+    return L
+
+
+lib = _load()
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _chk(st):
+    if st != 0:
+        raise OracleError(lib.orc_last_error().decode())
+
+
+def sizes(dim, p):
+    out = np.zeros(7, dtype=np.int32)
+    _chk(lib.orc_sizes(dim, p, _p(out)))
+    return dict(zip(('nN', 'nF', 'nNf', 'nIP', 'nIPf', 'n_local', 'lenL'), (int(v) for v in out)))
+
+
+def refel(dim, p):
+    s = sizes(dim, p)
+    nN, nF, nNf, nIP, nIPf = s['nN'], s['nF'], s['nNf'], s['nIP'], s['nIPf']
+    t = dict(nodes=np.zeros((nN, dim)), face_nodes=np.zeros((nF, nNf), dtype=np.int32),
+             ipx=np.zeros((nIP, dim)), ipw=np.zeros(nIP), phi=np.zeros((nIP, nN)),
+             dphi=np.zeros((nIP, nN, dim)), fipx=np.zeros((nIPf, dim - 1)), fipw=np.zeros(nIPf),
+             fphi=np.zeros((nIPf, nNf)))
+    _chk(lib.orc_refel(dim, p, *[_p(t[k]) for k in ('nodes', 'face_nodes', 'ipx', 'ipw', 'phi', 'dphi',
+                                                      'fipx', 'fipw', 'fphi')]))
+    return t
+
+
+def cubature(dim, order):
+    n = lib.orc_cubature(dim, order, None, None)
+    if n < 0:
+        raise OracleError(lib.orc_last_error().decode())
+    pts, w = np.zeros((n, dim)), np.zeros(n)
+    lib.orc_cubature(dim, order, _p(pts), _p(w))
+    return pts, w
+
+
+def local_system(dim, p, kind, x, D=None, ndD=0, b=None, tau=None, f_ip=None, r_ip=None):
+    s = sizes(dim, p)
+    n = s['n_local']
+    K, F = np.zeros((n, n)), np.zeros(n)
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) if v is not None else None for v in (x, D, b, tau, f_ip, r_ip)]
+    _chk(lib.orc_local_system(dim, p, Model(kind, ndD if D is not None else 0), *[_p(a) for a in arrs], _p(K), _p(F)))
+    return K, F
+
+
+def ip_coords(dim, p, x):
+    s = sizes(dim, p)
+    out = np.zeros((s['nIP'], dim))
+    _chk(lib.orc_ip_coords(dim, p, _p(np.ascontiguousarray(x, dtype=np.float64)), _p(out)))
+    return out
+
+
+def condense(dim, p, K, F):
+    s = sizes(dim, p)
+    nN, nL = s['nN'], s['lenL']
+    out = dict(S=np.zeros((nL, nL)), S0=np.zeros(nL), U=np.zeros((nN, nL)), Q=np.zeros((nN * dim, nL)),
+               U0=np.zeros(nN), Q0=np.zeros(nN * dim))
+    _chk(lib.orc_condense(dim, p, _p(np.ascontiguousarray(K)), _p(np.ascontiguousarray(F)),
+                          *[_p(out[k]) for k in ('S', 'S0', 'U', 'Q', 'U0', 'Q0')]))
+    return out
+
+
+class System:
+    """Whole-mesh oracle pipeline (HDGSolver.cpp:176-779 restated)."""
+
+    def __init__(self, dim, p, kind, mesh_arrays, tau, tau_sides=1, D=None, ndD=0, b=None, f_ip=None, r_ip=None,
+                 dirichlet=None, nthreads=0):
+        m = mesh_arrays
+        self.dim, self.p = dim, p
+        self.s = sizes(dim, p)
+        self.n_cells = m['cells'].shape[0]
+        self._keep = [np.ascontiguousarray(v, dtype=np.float64) if v is not None else None
+                      for v in (m['coords'], D, b, tau, f_ip, r_ip, dirichlet)]
+        self._ikeep = [np.ascontiguousarray(m[k], dtype=np.int32) for k in ('cells', 'faces', 'cell2face', 'face2cell')]
+        co, Dd, bb, tt, ff, rr, gg = self._keep
+        ce, fa, c2f, f2c = self._ikeep
+        h = lib.orc_system_build(dim, p, Model(kind, ndD if D is not None else 0), co.shape[0], _p(co), ce.shape[0],
+                                 _p(ce), fa.shape[0], _p(fa), _p(c2f), _p(f2c), _p(Dd), _p(bb), _p(tt), tau_sides,
+                                 _p(ff), _p(rr), _p(gg), nthreads)
+        if not h:
+            raise OracleError(lib.orc_last_error().decode())
+        self._h = h
+        out = np.zeros(2, dtype=np.int64)
+        lib.orc_system_sizes(self._h, _p(out))
+        self.ndofs, self.nnz = int(out[0]), int(out[1])
+
+    def __del__(self):
+        if getattr(self, '_h', None):
+            lib.orc_system_free(self._h)
+            self._h = None
+
+    def csr(self):
+        rowptr = np.zeros(self.ndofs + 1, dtype=np.int64)
+        cols = np.zeros(self.nnz, dtype=np.int32)
+        vals = np.zeros(self.nnz)
+        rhs = np.zeros(self.ndofs)
+        lib.orc_system_csr(self._h, _p(rowptr), _p(cols), _p(vals), _p(rhs))
+        return rowptr, cols, vals, rhs
+
+    def condensed(self):
+        s, ne, d = self.s, self.n_cells, self.dim
+        nN, nL = s['nN'], s['lenL']
+        out = dict(S=np.zeros((ne, nL, nL)), S0=np.zeros((ne, nL)), U=np.zeros((ne, nN, nL)),
+                   Q=np.zeros((ne, nN * d, nL)), U0=np.zeros((ne, nN)), Q0=np.zeros((ne, nN * d)))
+        lib.orc_system_condensed(self._h, *[_p(out[k]) for k in ('S', 'S0', 'U', 'Q', 'U0', 'Q0')])
+        return out
+
+    def solve(self, method=0, rtol=1e-12, maxits=20000, restart=30):
+        lam = np.zeros(self.ndofs)
+        info = np.zeros(2, dtype=np.int32)
+        rel = np.zeros(1)
+        _chk(lib.orc_system_solve(self._h, method, rtol, maxits, restart, _p(lam), _p(info), _p(rel)))
+        return lam, int(info[0]), bool(info[1]), float(rel[0])
+
+    def reconstruct(self, lam):
+        s, ne, d = self.s, self.n_cells, self.dim
+        u = np.zeros((ne, s['nN']))
+        q = np.zeros((ne, s['nN'] * d))
+        _chk(lib.orc_system_reconstruct(self._h, _p(np.ascontiguousarray(lam)), _p(u), _p(q)))
+        return u, q
+
+    def time_assembly(self, n_sample, nthreads=0):
+        t = lib.orc_system_time_assembly(self._h, int(n_sample), int(nthreads))
+        if t < 0:
+            raise OracleError(lib.orc_last_error().decode())
+        return t
+
+
+def dense_gmres(A, b, rtol=1e-12, maxits=1000, restart=30):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(b.shape[0])
+    it = np.zeros(1, dtype=np.int32)
+    _chk(lib.orc_dense_gmres(b.shape[0], _p(A), _p(b), rtol, maxits, restart, _p(x), _p(it)))
+    return x, int(it[0])

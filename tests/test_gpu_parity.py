@@ -1,0 +1,187 @@
+"""HIP path (libhfx_hip.so on the MI355X) against the FP64 oracle.
+
+Bars: local matrices and condensed blocks to 1e-12 relative (both FP64 on
+the same inputs), the CSR trace system to 1e-12, solutions to 1e-10
+relative with the true FP64 residual <= 1e-10."""
+import os
+
+import numpy as np
+import pytest
+
+import hyperfox_amd as hf
+import hyperfox_amd.problems as pr
+import oracle as orc
+from conftest import GOLDEN
+from hdg_helpers import l2_projection_error, rel
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(2, p) for p in range(1, 6)] + [(3, p) for p in range(1, 4)]
+
+
+def small_workload(dim, p, kind, seed=5):
+    """Jittered mesh with every model feature switched on for `kind`."""
+    n = 3 if dim == 2 else 2
+    m = hf.Mesh.structured(dim, p, n, 0.15 if dim == 2 else 0.05, seed)
+    a = m.arrays()
+    rng = np.random.default_rng(seed + 17 * p + dim)
+    nNf = m.sizes['nNf']
+    x = a['coords']
+    if kind == 2:
+        b = np.stack([np.sin(2 * x[:, 1] + 0.3), np.cos(x[:, 0])] + ([np.sin(x[:, 0] * x[:, 1])] if dim == 3 else []), 1)
+        tau = pr.upwind_tau(m, b, 0.1, 1.0 / n) + rng.uniform(0, 0.2, (m.n_faces, nNf, 2))
+        D = 0.1 + 0.05 * np.sin(3 * x[:, 0])
+        return pr.Workload('cdr', dim, p, m, 2, tau, 2, D=D, ndD=1, b=b,
+                           dirichlet=rng.normal(size=(m.n_faces, nNf)), f_fn=lambda xx: np.cos(xx[..., 0] + xx[..., 1]))
+    tau = rng.uniform(0.5, 2.0, (m.n_faces, nNf))
+    if kind == 1:
+        Dm = np.zeros((x.shape[0], dim, dim))
+        for k in range(dim):
+            Dm[:, k, k] = 1.0 + 0.3 * np.sin(x[:, k])
+        Dm[:, 0, 1] = Dm[:, 1, 0] = 0.2 * np.cos(x[:, 1])
+        Dcol = Dm.transpose(0, 2, 1).reshape(x.shape[0], dim * dim)
+        return pr.Workload('ds', dim, p, m, 1, tau, 1, D=Dcol, ndD=dim * dim, dirichlet=rng.normal(size=(m.n_faces, nNf)),
+                           f_fn=lambda xx: np.sin(xx[..., 0]) * np.exp(xx[..., 1]))
+    return pr.Workload('lap', dim, p, m, 0, tau, 1, dirichlet=rng.normal(size=(m.n_faces, nNf)))
+
+
+def run_gpu(w):
+    prob = hf.HDGProblem(w.dim, w.p)
+    f = pr.setup_gpu(w, prob)
+    return prob, f
+
+
+def oracle_system(w, f):
+    return orc.System(w.dim, w.p, w.kind, w.mesh.arrays(), w.tau, tau_sides=w.tau_sides, D=w.D, ndD=w.ndD, b=w.b,
+                      f_ip=f, dirichlet=w.dirichlet)
+
+
+def oracle_local(w, f, e):
+    a = w.mesh.arrays()
+    c = a['cells'][e]
+    X = a['coords'][c]
+    nNf = w.mesh.sizes['nNf']
+    tau = np.zeros((w.mesh.sizes['nF'], nNf))
+    for lf in range(w.mesh.sizes['nF']):
+        F = a['cell2face'][e, lf]
+        side = 1 if (w.tau_sides == 2 and a['face2cell'][F, 0] != e) else 0
+        t = w.tau[F] if w.tau_sides == 1 else w.tau[F][:, side]
+        tau[lf] = t[a['face_perm'][e, lf]]
+    D = None if w.D is None else np.asarray(w.D).reshape(w.mesh.n_nodes, -1)[c]
+    b = None if w.b is None else w.b[c]
+    fe = None if f is None else f[e]
+    return orc.local_system(w.dim, w.p, w.kind, X, D=D, ndD=w.ndD, b=b, tau=tau.reshape(-1), f_ip=fe)
+
+
+@pytest.mark.parametrize('dim,p', CASES)
+def test_ip_coords(dim, p):
+    w = small_workload(dim, p, 1)
+    prob, _ = run_gpu(w)
+    xg = prob.ip_coords()
+    a = w.mesh.arrays()
+    for e in range(0, w.mesh.n_cells, 5):
+        assert np.allclose(xg[e], orc.ip_coords(dim, p, a['coords'][a['cells'][e]]), atol=1e-14)
+
+
+@pytest.mark.parametrize('kind', [0, 1, 2])
+@pytest.mark.parametrize('dim,p', CASES)
+def test_local_matrices_match_oracle(dim, p, kind):
+    w = small_workload(dim, p, kind)
+    prob, f = run_gpu(w)
+    K, F = prob.local_matrices(0, w.mesh.n_cells)
+    for e in range(w.mesh.n_cells):
+        Ko, Fo = oracle_local(w, f, e)
+        assert rel(K[e], Ko) < 1e-12, (e, rel(K[e], Ko))
+        if kind != 0:
+            assert rel(F[e], Fo) < 1e-12
+        else:
+            assert np.all(F[e] == 0)
+
+
+@pytest.mark.parametrize('kind', [0, 1, 2])
+@pytest.mark.parametrize('dim,p', CASES)
+def test_condensed_blocks_match_oracle(dim, p, kind):
+    w = small_workload(dim, p, kind)
+    prob, f = run_gpu(w)
+    prob.assemble()
+    g = prob.condensed()
+    for e in range(w.mesh.n_cells):
+        Ko, Fo = oracle_local(w, f, e)
+        c = orc.condense(dim, p, Ko, Fo)
+        for k in ('S', 'U', 'Q'):
+            assert rel(g[k][e], c[k]) < 1e-11, (e, k, rel(g[k][e], c[k]))
+        for k in ('S0', 'U0', 'Q0'):
+            assert np.allclose(g[k][e], c[k], rtol=1e-10, atol=1e-12 * max(1, np.abs(c[k]).max())), (e, k)
+
+
+@pytest.mark.parametrize('kind', [0, 1, 2])
+@pytest.mark.parametrize('dim,p', CASES)
+def test_trace_system_matches_oracle(dim, p, kind):
+    w = small_workload(dim, p, kind)
+    prob, f = run_gpu(w)
+    prob.assemble()
+    rp, cols, vals, rhs = prob.trace_csr()
+    s = oracle_system(w, f)
+    rpo, colso, valso, rhso = s.csr()
+    assert np.array_equal(rp, rpo) and np.array_equal(cols, colso)
+    assert rel(vals, valso) < 1e-12
+    assert rel(rhs, rhso) < 1e-12
+
+
+@pytest.mark.parametrize('pc', [hf.PC_NONE, hf.PC_JACOBI, hf.PC_BJACOBI])
+@pytest.mark.parametrize('kind', [1, 2])
+@pytest.mark.parametrize('dim,p', [(2, 1), (2, 3), (2, 5), (3, 1), (3, 3)])
+def test_solution_matches_oracle(dim, p, kind, pc):
+    w = small_workload(dim, p, kind)
+    prob, f = run_gpu(w)
+    prob.assemble()
+    its, reason, relres = prob.solve(pc=pc, rtol=1e-12, maxits=5000)
+    assert reason == 2 and relres <= 1e-12
+    u, q, lam = prob.solution()
+    s = oracle_system(w, f)
+    lo, _, _, _ = s.solve(method=1)
+    uo, qo = s.reconstruct(lo)
+    assert rel(lam, lo) < 1e-10
+    assert rel(u, uo) < 1e-10
+    assert rel(q, qo) < 1e-10
+
+
+def test_constant_dirichlet_light_tri2():
+    z = np.load(os.path.join(GOLDEN, 'meshes', 'lightTri2.npz'))
+    m = hf.Mesh.from_cells(2, 2, z['nodes'], z['cells'])
+    nNf = m.sizes['nNf']
+    prob = hf.HDGProblem(2, 2)
+    prob.set_mesh(m)
+    prob.set_model(hf.MODEL_LAPLACE)
+    prob.set_fields(np.ones((m.n_faces, nNf)), 1, dirichlet=np.full((m.n_faces, nNf), 3.0))
+    prob.assemble()
+    its, reason, relres = prob.solve(pc=hf.PC_BJACOBI, rtol=1e-12)
+    u, q, lam = prob.solution()
+    assert np.allclose(u, 3.0, atol=1e-12, rtol=0)
+    assert np.allclose(q, 0.0, atol=1e-12, rtol=0)
+    assert np.allclose(lam, 3.0, atol=1e-12, rtol=0)
+
+
+@pytest.mark.parametrize('dim,h,p', [(2, '1e-1', 1), (2, '1e-1', 3), (3, '2e-1', 2), (3, '2e-1', 3)])
+def test_laplace_regression_meshes(dim, h, p):
+    z = np.load(os.path.join(GOLDEN, 'meshes', 'regression_dim-%d_h-%s_ord-%d.npz' % (dim, h, p)))
+    m = hf.Mesh.from_cells(dim, p, z['nodes'], z['cells'])
+    a = m.arrays()
+    nNf = m.sizes['nNf']
+    g = np.zeros((m.n_faces, nNf))
+    sx = lambda x: np.sin(x[..., 0]) * np.exp(x[..., 1])
+    g[a['boundary']] = sx(a['coords'][a['faces'][a['boundary']]])
+    tau = np.ones((m.n_faces, nNf))
+    prob = hf.HDGProblem(dim, p)
+    prob.set_mesh(m)
+    prob.set_model(hf.MODEL_LAPLACE)
+    prob.set_fields(tau, 1, dirichlet=g)
+    prob.assemble()
+    its, reason, relres = prob.solve(pc=hf.PC_BJACOBI, rtol=1e-12)
+    u, q, lam = prob.solution()
+    err = l2_projection_error(dim, p, a, u, sx(a['coords'][a['cells']]))
+    assert err < 1e-2
+    s = orc.System(dim, p, 0, a, tau, dirichlet=g)
+    lo, _, _, _ = s.solve(method=0, rtol=1e-14, maxits=20000)
+    uo, _ = s.reconstruct(lo)
+    assert rel(u, uo) < 1e-9

@@ -1,0 +1,124 @@
+"""Reference element and cubature: known answers of the reference's own tests
+(tests/unittests/element/TestCubature.cpp, TestReferenceElement.cpp) checked
+on both the product's host tables (libhfx_hip.so) and the oracle."""
+import itertools
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import hyperfox_amd as hf
+import oracle as orc
+
+# IP counts per (dim, order): TestCubature.cpp:43-47
+NIP = {2: [1, 1, 3, 6, 6, 7, 12, 15, 16, 19, 25], 3: [1, 1, 4, 8, 14, 14, 24, 35, 46, 59, 81]}
+VOL = {1: 2.0, 2: 2.0, 3: 4.0 / 3.0}
+
+
+def simplex_monomial_integral(exps):
+    """Exact integral of prod x_k^e_k over the bi-unit simplex (x_k >= -1,
+    sum x_k <= 2-d), from the unit-simplex moments i!j!k!/(i+j+k+d)!."""
+    d = len(exps)
+    total = Fraction(0)
+    for ks in itertools.product(*[range(e + 1) for e in exps]):
+        c = Fraction(1)
+        for e, k in zip(exps, ks):
+            c *= math.comb(e, k) * 2 ** k * (-1) ** (e - k)
+        num = 1
+        for k in ks:
+            num *= math.factorial(k)
+        total += c * Fraction(num, math.factorial(sum(ks) + d))
+    return float(total * 2 ** d)
+
+
+@pytest.mark.parametrize('impl', ['product', 'oracle'])
+@pytest.mark.parametrize('dim', [2, 3])
+def test_ip_counts_and_volume(impl, dim):
+    cub = hf.cubature if impl == 'product' else orc.cubature
+    for order, n in enumerate(NIP[dim]):
+        pts, w = cub(dim, order)
+        assert pts.shape == (n, dim)
+        assert abs(w.sum() - VOL[dim]) < 1e-13
+
+
+@pytest.mark.parametrize('impl', ['product', 'oracle'])
+@pytest.mark.parametrize('dim,maxo', [(1, 20), (2, 20), (3, 10)])
+def test_cubature_polynomial_exactness(impl, dim, maxo):
+    cub = hf.cubature if impl == 'product' else orc.cubature
+    for order in range(maxo + 1):
+        pts, w = cub(dim, order)
+        for exps in itertools.product(range(order + 1), repeat=dim):
+            if sum(exps) > order:
+                continue
+            if dim == 1:
+                exact = (1.0 - (-1.0) ** (exps[0] + 1)) / (exps[0] + 1)
+            else:
+                exact = simplex_monomial_integral(exps)
+            val = np.sum(w * np.prod(pts ** np.array(exps), axis=1))
+            assert abs(val - exact) <= 1e-12 * max(1.0, abs(exact)), (dim, order, exps, val, exact)
+
+
+def test_gll_golden_values():
+    """Node database values of ReferenceElement.cpp:700-760 (1-D Lobatto nodes
+    in the order -1, 1, interior ascending), as seen in the 2-D edge nodes."""
+    gold = {3: [-0.447213595499957939282, 0.447213595499957939282],
+            4: [-0.6546536707079771437983, 0.0, 0.6546536707079771437983],
+            5: [-0.765055323929464692851, -0.2852315164806450963142, 0.2852315164806450963142,
+                0.765055323929464692851]}
+    for p, vals in gold.items():
+        for tabs in (hf.refel_tables(2, p), orc.refel(2, p)):
+            # edge {0,1} of the triangle carries the 1-D nodes on y = -1
+            edge = tabs['nodes'][3:3 + p - 1, 0]
+            assert np.allclose(edge, vals, atol=1e-15, rtol=0)
+
+
+@pytest.mark.parametrize('dim,pmax', [(2, 5), (3, 5)])
+def test_node_counts_and_faces(dim, pmax):
+    for p in range(1, pmax + 1):
+        t = hf.refel_tables(dim, p)
+        nN = math.comb(p + dim, dim)
+        assert t['nodes'].shape[0] == nN
+        x = t['nodes']
+        # barycentric coordinate opposite each reference face vanishes on it
+        lam = np.concatenate([-(x.sum(1, keepdims=True) + dim - 2) / 2.0, (x + 1) / 2.0], axis=1)
+        opposite = {2: [2, 0, 1], 3: [2, 0, 1, 3]}[dim]
+        for f, fn in enumerate(t['face_nodes']):
+            assert np.all(np.abs(lam[fn, opposite[f]]) < 1e-13)
+            assert len(set(fn.tolist())) == len(fn)
+        assert np.all(lam > -1e-13)
+
+
+def _monomials(dim, p):
+    return [e for e in itertools.product(range(p + 1), repeat=dim) if sum(e) <= p]
+
+
+@pytest.mark.parametrize('impl', ['product', 'oracle'])
+@pytest.mark.parametrize('dim,pmax', [(2, 5), (3, 5)])
+def test_lagrange_reproduces_monomials(impl, dim, pmax):
+    """TestReferenceElement.cpp:146-236: sum_j m(x_j) phi_j = m and
+    sum_j m(x_j) grad phi_j = grad m at every IP, for deg(m) <= p."""
+    for p in range(1, pmax + 1):
+        t = hf.refel_tables(dim, p) if impl == 'product' else orc.refel(dim, p)
+        X, ip = t['nodes'], t['ipx']
+        for e in _monomials(dim, p):
+            e = np.array(e)
+            mv = np.prod(X ** e, axis=1)
+            assert np.allclose(t['phi'] @ mv, np.prod(ip ** e, axis=1), atol=1e-12)
+            for k in range(dim):
+                if e[k] == 0:
+                    g = np.zeros(ip.shape[0])
+                else:
+                    ek = e.copy()
+                    ek[k] -= 1
+                    g = e[k] * np.prod(ip ** ek, axis=1)
+                assert np.allclose(np.einsum('qn,n->q', t['dphi'][:, :, k], mv), g, atol=1e-11)
+
+
+@pytest.mark.parametrize('dim,pmax', [(2, 5), (3, 5)])
+def test_product_tables_match_oracle(dim, pmax):
+    for p in range(1, pmax + 1):
+        a, b = hf.refel_tables(dim, p), orc.refel(dim, p)
+        for k in a:
+            assert a[k].shape == b[k].shape
+            assert np.allclose(a[k], b[k], atol=1e-13, rtol=0), (dim, p, k)
