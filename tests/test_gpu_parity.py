@@ -135,8 +135,11 @@ def test_solution_matches_oracle(dim, p, kind, pc):
     w = small_workload(dim, p, kind)
     prob, f = run_gpu(w)
     prob.assemble()
-    its, reason, relres = prob.solve(pc=pc, rtol=1e-12, maxits=5000)
-    assert reason == 2 and relres <= 1e-12
+    # solution error <= cond(A) * residual: solve to 1e-13 so the 1e-10 bar
+    # on the solution is a statement about the discretisation, not about when
+    # the Krylov loop stopped
+    its, reason, relres = prob.solve(pc=pc, rtol=1e-13, maxits=20000)
+    assert reason == 2 and relres <= 1e-13
     u, q, lam = prob.solution()
     s = oracle_system(w, f)
     lo, _, _, _ = s.solve(method=1)
