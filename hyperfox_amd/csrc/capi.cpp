@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -44,8 +45,10 @@ struct hfx_hdg {
   int kind = 1, ndD = 0, tau_sides = 1;
   bool hasD = false, hasB = false, hasF = false, hasR = false;
   int nt = 0, grid_cap = 0;
+  bool affine = false;     // every cell straight-sided (constant Jacobian)
+  int kernel_choice = 0;   // 0 auto, 1 general, 2 affine
   // device data
-  hfx::DevBuf<double> tabs, coords, Dn, bn, tau, f_ip, r_ip, g;
+  hfx::DevBuf<double> tabs, tabs_aff, coords, Dn, bn, tau, f_ip, r_ip, g;
   hfx::DevBuf<int> faceNodes, cells, c2f, f2c;
   hfx::DevBuf<int8_t> perm, invperm;
   hfx::DevBuf<uint8_t> is_dir;
@@ -86,9 +89,26 @@ struct hfx_hdg {
     if (nt) return nt;
     return (dim == 3 && order >= 3) ? 256 : (dim == 3 || order >= 4 ? 128 : 64);
   }
+  bool use_affine() const {
+    if (kernel_choice == 1) return false;
+    const bool ok = affine && !(hasD && ndD == dim * dim);
+    if (kernel_choice == 2 && !ok)
+      throw std::runtime_error("affine element kernel needs straight-sided cells and a scalar diffusion field");
+    return ok;
+  }
+  int affine_waves() const {
+    for (int w : {4, 3, 2, 1})
+      if (hfx::affine_lds_bytes(dim, order, false, w) + 4096 <= 160 * 1024) return w;
+    return 1;
+  }
   void run_element() {
     hfx::ElemArgs a = elem_args();
-    hfx::launch_element(dim, order, a, threads(), grid_cap, stream);
+    if (use_affine()) {
+      a.tabs = tabs_aff.p;
+      hfx::launch_affine(dim, order, false, affine_waves(), a, grid_cap, stream);
+    } else {
+      hfx::launch_element(dim, order, a, threads(), grid_cap, stream);
+    }
   }
   void run_scatter() {
     hfx::launch_scatter(A, mesh.nFpc, S.p, S0.p, invperm.p, is_dir.p, g.p, rhs.p, stream);
@@ -108,6 +128,80 @@ void pack_tables(const hfx::RefEl& re, std::vector<double>& t) {
   t.insert(t.end(), re.cub.w.begin(), re.cub.w.end());
   t.insert(t.end(), fe.cub.w.begin(), fe.cub.w.end());
   (void)d;
+}
+
+// Reference tables of the affine element kernel (order of ALay in
+// hdg_affine.hip): PHI, DPHI, FPHI, W, FW, Ahat_a = Mhat^-1 Chat_a,
+// G_g = Mhat^-1[:, fn_g] Mfhat.  Built in long double.
+void pack_affine_tables(const hfx::RefEl& re, std::vector<double>& t) {
+  const int d = re.dim, nN = re.nN, nip = re.nip();
+  const hfx::RefEl& fe = *re.face;
+  const int nNf = fe.nN, nipf = fe.nip(), nF = re.nF;
+  typedef long double ld;
+  std::vector<ld> M(nN * nN, 0), Mi(nN * nN, 0);
+  for (int q = 0; q < nip; ++q)
+    for (int i = 0; i < nN; ++i)
+      for (int j = 0; j < nN; ++j) M[i * nN + j] += (ld)re.cub.w[q] * re.phi[q * nN + i] * re.phi[q * nN + j];
+  for (int i = 0; i < nN; ++i) Mi[i * nN + i] = 1;
+  for (int k = 0; k < nN; ++k) {  // SPD: Gauss-Jordan without pivoting
+    ld inv = 1.0L / M[k * nN + k];
+    for (int j = 0; j < nN; ++j) { M[k * nN + j] *= inv; Mi[k * nN + j] *= inv; }
+    for (int r = 0; r < nN; ++r) {
+      if (r == k) continue;
+      ld f = M[r * nN + k];
+      for (int j = 0; j < nN; ++j) { M[r * nN + j] -= f * M[k * nN + j]; Mi[r * nN + j] -= f * Mi[k * nN + j]; }
+    }
+  }
+  t.clear();
+  t.insert(t.end(), re.phi.begin(), re.phi.end());
+  t.insert(t.end(), re.dphi.begin(), re.dphi.end());
+  t.insert(t.end(), fe.phi.begin(), fe.phi.end());
+  t.insert(t.end(), re.cub.w.begin(), re.cub.w.end());
+  t.insert(t.end(), fe.cub.w.begin(), fe.cub.w.end());
+  for (int a = 0; a < d; ++a) {
+    std::vector<ld> C(nN * nN, 0);
+    for (int q = 0; q < nip; ++q)
+      for (int i = 0; i < nN; ++i)
+        for (int j = 0; j < nN; ++j)
+          C[i * nN + j] += (ld)re.cub.w[q] * re.dphi[(q * nN + i) * d + a] * re.phi[q * nN + j];
+    for (int i = 0; i < nN; ++i)
+      for (int j = 0; j < nN; ++j) {
+        ld s = 0;
+        for (int l = 0; l < nN; ++l) s += Mi[i * nN + l] * C[l * nN + j];
+        t.push_back((double)s);
+      }
+  }
+  std::vector<ld> Mf(nNf * nNf, 0);
+  for (int q = 0; q < nipf; ++q)
+    for (int i = 0; i < nNf; ++i)
+      for (int j = 0; j < nNf; ++j) Mf[i * nNf + j] += (ld)fe.cub.w[q] * fe.phi[q * nNf + i] * fe.phi[q * nNf + j];
+  for (int g = 0; g < nF; ++g)
+    for (int j = 0; j < nN; ++j)
+      for (int b = 0; b < nNf; ++b) {
+        ld s = 0;
+        for (int a2 = 0; a2 < nNf; ++a2) s += Mi[j * nN + re.faceNodes[g][a2]] * Mf[a2 * nNf + b];
+        t.push_back((double)s);
+      }
+}
+
+// True when every cell's nodes are the affine image of its vertices (so
+// the Jacobian is constant per cell and faces are flat).
+bool mesh_is_affine(const hfx::RefEl& re, const hfx::Mesh& m) {
+  const int d = re.dim;
+  for (int e = 0; e < m.nCells; ++e) {
+    const int* c = &m.cells[(size_t)e * re.nN];
+    double h = 0.0;
+    for (int v = 1; v <= d; ++v)
+      for (int k = 0; k < d; ++k) h = std::max(h, std::fabs(m.coords[(size_t)c[v] * d + k] - m.coords[(size_t)c[0] * d + k]));
+    for (int i = d + 1; i < re.nN; ++i)
+      for (int k = 0; k < d; ++k) {
+        double x = m.coords[(size_t)c[0] * d + k];
+        for (int a = 0; a < d; ++a)
+          x += 0.5 * (re.nodes[i * d + a] + 1.0) * (m.coords[(size_t)c[a + 1] * d + k] - m.coords[(size_t)c[0] * d + k]);
+        if (std::fabs(x - m.coords[(size_t)c[i] * d + k]) > 1e-10 * h) return false;
+      }
+  }
+  return true;
 }
 
 void require(bool c, const char* msg) {
@@ -234,6 +328,8 @@ extern "C" int hfx_hdg_create(int device, int dim, int order, hfx_hdg** out) {
   std::vector<double> t;
   pack_tables(*h->re, t);
   h->tabs.upload(t, h->stream);
+  pack_affine_tables(*h->re, t);
+  h->tabs_aff.upload(t, h->stream);
   std::vector<int> fnodes;
   for (auto& f : h->re->faceNodes) fnodes.insert(fnodes.end(), f.begin(), f.end());
   h->faceNodes.upload(fnodes, h->stream);
@@ -261,6 +357,7 @@ extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
   h->set_device();
   h->mesh = mm->m;
   h->mesh.order = h->order;
+  h->affine = mesh_is_affine(*h->re, h->mesh);
   const hfx::Mesh& m = h->mesh;
   hipStream_t s = h->stream;
   h->coords.upload(m.coords, s);
@@ -551,5 +648,26 @@ extern "C" int hfx_hdg_time(hfx_hdg* h, int what, int warmup, int reps, double* 
 extern "C" int hfx_element_lds_bytes(int dim, int order, long long* bytes) {
   HFX_TRY
   *bytes = (long long)hfx::element_lds_bytes(dim, order);
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_kernel(hfx_hdg* h, int which) {
+  HFX_TRY
+  require(h, "hfx_hdg_set_kernel: null handle");
+  require(which >= 0 && which <= 2, "hfx_hdg_set_kernel: 0 (auto), 1 (general) or 2 (affine)");
+  h->kernel_choice = which;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_kernel_info(hfx_hdg* h, int* out) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_kernel_info: set the mesh first");
+  const bool aff = h->use_affine();
+  out[0] = aff ? 2 : 1;
+  out[1] = aff ? 64 * h->affine_waves() : h->threads();
+  out[2] = h->affine ? 1 : 0;
+  long long lds = aff ? (long long)hfx::affine_lds_bytes(h->dim, h->order, false, h->affine_waves())
+                      : (long long)hfx::element_lds_bytes(h->dim, h->order);
+  out[3] = (int)lds;
   HFX_END
 }

@@ -64,6 +64,11 @@ void launch_element(int dim, int order, const ElemArgs& a, int nt, int grid_cap,
 // Dynamic LDS bytes the kernel needs for (dim, order).
 size_t element_lds_bytes(int dim, int order);
 
+// Affine-cell kernel (hdg_affine.hip): one wavefront per cell, `waves`
+// wavefronts per workgroup; scalar or identity diffusion only.
+void launch_affine(int dim, int order, bool tensor_d, int waves, const ElemArgs& a, int grid_cap, hipStream_t s);
+size_t affine_lds_bytes(int dim, int order, bool tensor_d, int waves);
+
 // Physical coordinates of the element IPs: out[(e*NIP + q)*D + k].
 void launch_ip_coords(int dim, int order, int nEl, const double* tabs, const double* coords, const int* cells,
                       double* out, hipStream_t s);

@@ -22,47 +22,51 @@ namespace hfx {
 
 namespace {
 
-// Gauss-Jordan elimination with partial pivoting on an LDS matrix of NR rows
-// and NC columns (row stride NC); the left NR x NR block is reduced to I and
-// the right-hand columns become its inverse times themselves.
+// Gauss-Jordan elimination with partial pivoting, without row swaps, on an
+// LDS matrix of NR rows and NC columns (row stride NC, NR <= 64).  Step k
+// picks the unused row p of largest |A[p][k]| (every wave computes the same
+// pivot with a wave-wide argmax, so no broadcast barrier is needed) and
+// eliminates column k from every other row.  On return row perm[k] holds
+// variable k: its right-hand columns (NR..NC-1) are divided by the pivot,
+// i.e. X[k][c] = A[perm[k]][c].  One workgroup barrier per step.
 template <int NR, int NC, int NT>
-__device__ void gauss_jordan(double* A, int* piv_slot) {
+__device__ void gauss_jordan(double* A, int* perm) {
+  static_assert(NR <= 64, "gauss_jordan: at most 64 rows");
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  unsigned long long used = 0ull;
   for (int k = 0; k < NR; ++k) {
-    if (tid == 0) {
-      int p = k;
-      double best = fabs(A[k * NC + k]);
-      for (int r = k + 1; r < NR; ++r) {
-        double v = fabs(A[r * NC + k]);
-        if (v > best) { best = v; p = r; }
-      }
-      *piv_slot = p;
+    double v = -1.0;
+    if (lane < NR && !((used >> lane) & 1ull)) {
+      v = fabs(A[lane * NC + k]);
+      if (!(v >= 0.0)) v = 0.0;  // NaN: keep the argmax total and uniform
     }
-    __syncthreads();
-    const int p = *piv_slot;
-    if (p != k)
-      for (int c = k + tid; c < NC; c += NT) {
-        double t = A[k * NC + c];
-        A[k * NC + c] = A[p * NC + c];
-        A[p * NC + c] = t;
-      }
-    __syncthreads();
-    const double inv = 1.0 / A[k * NC + k];
-    __syncthreads();
-    for (int c = k + 1 + tid; c < NC; c += NT) A[k * NC + c] *= inv;
-    __syncthreads();
-    // eliminate column k from every other row (columns > k only)
-    constexpr int W = NC;  // upper bound on active columns
-    const int ncol = NC - k - 1;
-    for (int t = tid; t < NR * ncol; t += NT) {
-      int r = t / ncol;
-      int c = k + 1 + t % ncol;
-      if (r == k) continue;
-      A[r * NC + c] -= A[r * NC + k] * A[k * NC + c];
+    int idx = lane;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(v, off, 64);
+      const int oi = __shfl_xor(idx, off, 64);
+      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
     }
-    (void)W;
+    const int p = idx;
+    used |= 1ull << p;
+    if (tid == 0) perm[k] = p;
+    const double pinv = 1.0 / A[p * NC + k];
+    constexpr int TOT = NR * NC;
+    for (int t = tid; t < TOT; t += NT) {
+      const int r = t / NC, c = t % NC;
+      if (r == p || c <= k) continue;
+      A[r * NC + c] -= (A[r * NC + k] * pinv) * A[p * NC + c];
+    }
     __syncthreads();
   }
+  // normalise the right-hand columns of each pivot row
+  constexpr int NRHS = NC - NR;
+  for (int t = tid; t < NR * NRHS; t += NT) {
+    const int k = t / NRHS, c = NR + t % NRHS;
+    const int r = perm[k];
+    A[r * NC + c] /= A[r * NC + k];
+  }
+  __syncthreads();
 }
 
 template <int D, int P, int NT>
@@ -134,7 +138,7 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
   __shared__ int fn[NF * NNF];
   __shared__ int pos[NF * NN];
   __shared__ int vopp[NF];
-  __shared__ int piv;
+  __shared__ int perm1[NN], perm2[NN];
   const int tid = threadIdx.x;
 
   for (int t = tid; t < Z::TAB; t += NT) sm[L::O_TAB + t] = a.tabs[t];
@@ -400,19 +404,19 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
     }
 
     // ---------------------------------- [M | Cg | I] -> [I | A | M^-1]
-    gauss_jordan<NN, NC1, NT>(AUG1, &piv);
+    gauss_jordan<NN, NC1, NT>(AUG1, perm1);
     // K = Suu - sum_k Suq_k A_k ;  Z_k = Suq_k M^-1
     for (int t = tid; t < NN * NN; t += NT) {
       const int i = t / NN, j = t % NN;
       double s = SUU[t];
       for (int k = 0; k < D; ++k)
-        for (int l = 0; l < NN; ++l) s -= SUQ[(k * NN + i) * NN + l] * AUG1[l * NC1 + NN + k * NN + j];
+        for (int l = 0; l < NN; ++l) s -= SUQ[(k * NN + i) * NN + l] * AUG1[perm1[l] * NC1 + NN + k * NN + j];
       AUG2[i * NC2 + j] = s;
     }
     for (int t = tid; t < D * NN * NN; t += NT) {
       const int k = t / (NN * NN), i = (t / NN) % NN, j = t % NN;
       double s = 0.0;
-      for (int l = 0; l < NN; ++l) s += SUQ[(k * NN + i) * NN + l] * AUG1[l * NC1 + NN + D * NN + j];
+      for (int l = 0; l < NN; ++l) s += SUQ[(k * NN + i) * NN + l] * AUG1[perm1[l] * NC1 + NN + D * NN + j];
       ZB[t] = s;
     }
     __syncthreads();
@@ -430,25 +434,26 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
     for (int i = tid; i < NN; i += NT) AUG2[i * NC2 + NN + NL] = FU[i];
     __syncthreads();
     // ---------------------------------------- [K | -R | Fu] -> [I | U | U0]
-    gauss_jordan<NN, NC2, NT>(AUG2, &piv);
+    gauss_jordan<NN, NC2, NT>(AUG2, perm2);
     // Q_k = -A_k U - M^-1 Sql_k ;  Q0_k = -A_k U0 ; write U, U0, Q, Q0
     double* Ue = a.U + (size_t)e * NN * NL;
     double* Qe = a.Q + (size_t)e * NN * D * NL;
-    for (int t = tid; t < NN * NL; t += NT) Ue[t] = AUG2[(t / NL) * NC2 + NN + t % NL];
-    for (int t = tid; t < NN; t += NT) a.U0[(size_t)e * NN + t] = AUG2[t * NC2 + NN + NL];
+    for (int t = tid; t < NN * NL; t += NT) Ue[t] = AUG2[perm2[t / NL] * NC2 + NN + t % NL];
+    for (int t = tid; t < NN; t += NT) a.U0[(size_t)e * NN + t] = AUG2[perm2[t] * NC2 + NN + NL];
     for (int t = tid; t < D * NN * NL; t += NT) {
       const int k = t / (NN * NL), i = (t / NL) % NN, c = t % NL, f = c / NNF, b = c % NNF;
       double s = 0.0;
-      for (int j = 0; j < NN; ++j) s -= AUG1[i * NC1 + NN + k * NN + j] * AUG2[j * NC2 + NN + c];
+      const int pi = perm1[i];
+      for (int j = 0; j < NN; ++j) s -= AUG1[pi * NC1 + NN + k * NN + j] * AUG2[perm2[j] * NC2 + NN + c];
       for (int aa = 0; aa < NNF; ++aa)
-        s += AUG1[i * NC1 + NN + D * NN + fn[f * NNF + aa]] * FM[((f * NS + 2 + k) * NNF + aa) * NNF + b];
+        s += AUG1[pi * NC1 + NN + D * NN + fn[f * NNF + aa]] * FM[((f * NS + 2 + k) * NNF + aa) * NNF + b];
       QL[t] = s;
       Qe[(size_t)(i * D + k) * NL + c] = s;
     }
     for (int t = tid; t < D * NN; t += NT) {
       const int k = t / NN, i = t % NN;
       double s = 0.0;
-      for (int j = 0; j < NN; ++j) s -= AUG1[i * NC1 + NN + k * NN + j] * AUG2[j * NC2 + NN + NL];
+      for (int j = 0; j < NN; ++j) s -= AUG1[perm1[i] * NC1 + NN + k * NN + j] * AUG2[perm2[j] * NC2 + NN + NL];
       Q0L[t] = s;
       a.Q0[(size_t)e * NN * D + i * D + k] = s;
     }
@@ -460,7 +465,7 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
       double s = 0.0;
       for (int b = 0; b < NNF; ++b) {
         const int j = fn[f * NNF + b];
-        s += FM[((f * NS + 0) * NNF + aa) * NNF + b] * AUG2[j * NC2 + NN + c];
+        s += FM[((f * NS + 0) * NNF + aa) * NNF + b] * AUG2[perm2[j] * NC2 + NN + c];
         for (int k = 0; k < D; ++k) s -= FM[((f * NS + 2 + D + k) * NNF + aa) * NNF + b] * QL[(k * NN + j) * NL + c];
       }
       if (c / NNF == f) {
@@ -474,7 +479,7 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
       double s = 0.0;
       for (int b = 0; b < NNF; ++b) {
         const int j = fn[f * NNF + b];
-        s -= FM[((f * NS + 0) * NNF + aa) * NNF + b] * AUG2[j * NC2 + NN + NL];
+        s -= FM[((f * NS + 0) * NNF + aa) * NNF + b] * AUG2[perm2[j] * NC2 + NN + NL];
         for (int k = 0; k < D; ++k) s += FM[((f * NS + 2 + D + k) * NNF + aa) * NNF + b] * Q0L[k * NN + j];
       }
       a.S0[(size_t)e * NL + r] = s;

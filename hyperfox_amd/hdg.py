@@ -146,6 +146,16 @@ class HDGProblem:
     def set_launch(self, threads=0, grid_cap=0):
         check(lib.hfx_hdg_set_launch(self._h, threads, grid_cap))
 
+    def set_kernel(self, which):
+        """0 automatic, 1 general (per-IP Jacobians), 2 affine-cell kernel."""
+        check(lib.hfx_hdg_set_kernel(self._h, which))
+
+    def kernel_info(self):
+        out = np.zeros(4, dtype=np.int32)
+        check(lib.hfx_hdg_kernel_info(self._h, _ptr(out)))
+        return dict(kernel='affine' if out[0] == 2 else 'general', threads=int(out[1]), mesh_affine=bool(out[2]),
+                    lds_bytes=int(out[3]))
+
     def assemble(self):
         check(lib.hfx_hdg_assemble(self._h))
 

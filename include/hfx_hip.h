@@ -119,6 +119,15 @@ int hfx_hdg_set_source(hfx_hdg* h, const double* f_ip, const double* r_ip);
  * a cap on the grid (0 = one workgroup per element). */
 int hfx_hdg_set_launch(hfx_hdg* h, int threads, int grid_cap);
 
+/* Element kernel selection: 0 automatic (the affine-cell kernel when every
+ * cell is straight-sided and diffusion is scalar, else the general kernel
+ * with per-IP Jacobians), 1 general, 2 affine. */
+int hfx_hdg_set_kernel(hfx_hdg* h, int which);
+
+/* out[0] = kernel in use (1 general, 2 affine), out[1] = threads per
+ * workgroup, out[2] = mesh is affine, out[3] = dynamic LDS bytes. */
+int hfx_hdg_kernel_info(hfx_hdg* h, int* out);
+
 /* Elemental matrices + static condensation + Dirichlet rows + trace-matrix
  * scatter, all on the GPU (HDGSolver::assemble, HDGSolver.cpp:166-675). */
 int hfx_hdg_assemble(hfx_hdg* h);

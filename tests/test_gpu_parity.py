@@ -98,11 +98,18 @@ def test_local_matrices_match_oracle(dim, p, kind):
             assert np.all(F[e] == 0)
 
 
+@pytest.mark.parametrize('kernel', [1, 0])
 @pytest.mark.parametrize('kind', [0, 1, 2])
 @pytest.mark.parametrize('dim,p', CASES)
-def test_condensed_blocks_match_oracle(dim, p, kind):
+def test_condensed_blocks_match_oracle(dim, p, kind, kernel):
+    """kernel 1 forces the general (per-IP Jacobian) kernel; 0 picks the
+    affine-cell kernel on these straight-sided meshes (general for tensor D)."""
     w = small_workload(dim, p, kind)
     prob, f = run_gpu(w)
+    prob.set_kernel(kernel)
+    info = prob.kernel_info()
+    assert info['mesh_affine']
+    assert info['kernel'] == ('general' if (kernel == 1 or kind == 1) else 'affine')
     prob.assemble()
     g = prob.condensed()
     for e in range(w.mesh.n_cells):
