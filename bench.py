@@ -10,6 +10,8 @@ The trace solve (FGMRES + block Jacobi to a true FP64 relative residual of
 1e-10) is timed once after the steps and reported beside the main number.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 24] [--no-cpu]
+       python bench.py --copy-calibration MiB   (a plain device copy of known
+       size, run under rocprofv3 --pmc to scale FETCH_SIZE / WRITE_SIZE)
 """
 import argparse
 import json
@@ -25,6 +27,10 @@ sys.path.insert(0, ROOT)
 METRIC = 'DOF/s assembly+condense; trace-solve time to 1e-10 residual (learned vs plain)'
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# HBM KiB per element-kernel dispatch, (read, write), from rocprofv3 --pmc
+# FETCH_SIZE / WRITE_SIZE runs of this script, scaled by a 1 GiB copy kernel
+# (FETCH_SIZE x2.00, WRITE_SIZE x1.00); the summary is kept under profiles/.
+MEASURED_TRAFFIC_KIB = {(24, 3): (2 * 105748.1, 3191374.5)}
 
 
 def elem_flops(dim, p, nN, nIP, nNf, nIPf, conv):
@@ -70,6 +76,21 @@ def cpu_baseline(n_cpu, p, threads):
                       % (n_cpu, w.mesh.n_cells, best)}
 
 
+def copy_calibration(mib):
+    """Copy `mib` MiB device-to-device 5 times with torch (one plain copy
+    kernel per call): the PMC reading of FETCH_SIZE / WRITE_SIZE for these
+    dispatches against the known 1 x mib read + 1 x mib write gives the scale
+    factor applied to the element kernel's counters."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.arange(n, dtype=torch.float64, device='cuda')
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({'copy_calibration_MiB': mib, 'bytes_read': n * 8, 'bytes_written': n * 8, 'dispatches': 5}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -80,11 +101,18 @@ def main():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-solve', action='store_true')
     ap.add_argument('--cpu-n', type=int, default=12)
+    ap.add_argument('--traffic', type=float, nargs=2, metavar=('FETCH_KIB', 'WRITE_KIB'), default=None,
+                    help='HBM KiB per element-kernel dispatch from rocprofv3 --pmc runs, already scaled')
+    ap.add_argument('--copy-calibration', type=int, default=0, metavar='MiB')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on exactly one GPU (multi-GPU sharding is designed, not built); '
               'use --gpus 1', file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     import hyperfox_amd as hf
     import hyperfox_amd.problems as pr
@@ -117,6 +145,15 @@ def main():
     by = elem_bytes(3, sz['nN'], sz['nNf'], sz['nIP'], nin) * nEl
     achieved = fl / (k_avg * 1e-3) / 1e12
     steps_ms = prob.time(1, warmup=0, reps=max(3, args.steps))
+    info = prob.kernel_info()
+    kname = ('hdg_affine_kernel<3,%d,W=%d>' % (args.p, info['threads'] // 64) if info['kernel'] == 'affine'
+             else 'hdg_element_kernel<3,%d,NT=%d>' % (args.p, info['threads']))
+    if args.traffic is not None:
+        traffic = 1024.0 * (args.traffic[0] + args.traffic[1])
+    elif (args.n, args.p) in MEASURED_TRAFFIC_KIB:
+        traffic = 1024.0 * sum(MEASURED_TRAFFIC_KIB[(args.n, args.p)])
+    else:
+        traffic = None
 
     res = {
         'metric': METRIC, 'value': value, 'unit': 'DOF/s', 'n_gpus': 1, 'steps': args.steps,
@@ -125,8 +162,9 @@ def main():
         'config': {'workload': 'HDG 3D convection-diffusion p=%d, %d^3x6=%d jittered tets, D=1e-2' % (args.p, args.n, nEl),
                    'cells': nEl, 'p': args.p, 'u_dofs': dofs, 'trace_dofs': w.mesh.n_trace},
         'roofline': {'bound': 'mfma', 'achieved': achieved, 'peak': FP64_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                     'frac': achieved / FP64_PEAK_TFLOPS, 'traffic': None,
-                     'kernel': 'hdg_element_kernel<3,3>', 'kernel_ms': k_avg,
+                     'frac': achieved / FP64_PEAK_TFLOPS, 'traffic': traffic,
+                     'kernel': kname, 'kernel_ms': k_avg, 'threads_per_wg': info['threads'],
+                     'lds_bytes_per_wg': info['lds_bytes'],
                      'algorithmic_flops_per_cell': fl / nEl, 'algorithmic_bytes_per_cell': by / nEl,
                      'hbm_GBs': by / (k_avg * 1e-3) / 1e9},
         'event_ms_per_step': float(np.median(steps_ms)),

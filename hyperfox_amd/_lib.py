@@ -52,6 +52,27 @@ _SIGS = {
     'hfx_element_lds_bytes': ([_I, _I, _P], _I),
     'hfx_hdg_set_kernel': ([_P, _I], _I),
     'hfx_hdg_kernel_info': ([_P, _P], _I),
+    'hfx_la_create': ([_I, _P], _I),
+    'hfx_la_free': ([_P], None),
+    'hfx_la_initialize': ([_P], _I),
+    'hfx_la_configure': ([_P, _I, _I, _D, _I, _I], _I),
+    'hfx_la_allocate': ([_P, _I, _P, _P], _I),
+    'hfx_la_add_vals': ([_P, _I, _P, _I, _P, _P], _I),
+    'hfx_la_set_vals': ([_P, _I, _P, _I, _P, _P], _I),
+    'hfx_la_add_rhs': ([_P, _I, _P, _P], _I),
+    'hfx_la_set_rhs': ([_P, _I, _P, _P], _I),
+    'hfx_la_zero_rows': ([_P, _I, _P], _I),
+    'hfx_la_assemble': ([_P], _I),
+    'hfx_la_assemble_flush': ([_P], _I),
+    'hfx_la_solve': ([_P, _P, _I, _P, _P], _I),
+    'hfx_la_ownership': ([_P, _P, _P], _I),
+    'hfx_la_clear': ([_P], _I),
+    'hfx_la_destroy_system': ([_P], _I),
+    'hfx_la_num_dofs': ([_P, _P], _I),
+    'hfx_la_get_values': ([_P, _I, _P, _I, _P, _P], _I),
+    'hfx_la_get_rhs': ([_P, _I, _P, _P], _I),
+    'hfx_la_nnz': ([_P, _P], _I),
+    'hfx_la_get_csr': ([_P] * 5, _I),
 }
 
 for _name, (_args, _res) in _SIGS.items():

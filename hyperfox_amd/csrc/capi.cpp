@@ -21,6 +21,11 @@ namespace {
 thread_local std::string g_err;
 }
 
+namespace hfx {
+// Shared with the linear-algebra entry points (capi_la.cpp).
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace hfx
+
 #define HFX_TRY try {
 #define HFX_END                       \
   }                                   \

@@ -166,6 +166,90 @@ int hfx_hdg_time(hfx_hdg* h, int what, int warmup, int reps, double* ms);
 /* Dynamic LDS bytes and threads used by the element kernel of (dim, order). */
 int hfx_element_lds_bytes(int dim, int order, long long* bytes);
 
+/* ------------------------------------------- linear-algebra plug point (la) */
+
+/* A sparse FP64 system A x = b behind the reference's LinAlgebraInterface
+ * (src/resolution/LinAlgebraInterface.h:23-162), with the behaviour of its
+ * PETSc implementation (src/resolution/PetscInterface.cpp:5-265): the same
+ * call order and misuse errors, ADD/INSERT semantics, row-major value blocks
+ * (MatSetValues' default), zero initial guess.  Values are gathered on the
+ * host and reduced into CSR at assembly; the solve runs on the GPU.  Only
+ * hfx_la_solve touches the GPU. */
+typedef struct hfx_la hfx_la;
+
+/* Solver types of hfx_la_configure (PetscOpts::solverType, PetscOpts.h:14). */
+enum { HFX_LA_GMRES = 0 };
+
+/* Construction (PetscInterface::PetscInterface, PetscInterface.cpp:5-12):
+ * a system that will solve on GPU `device`. */
+int hfx_la_create(int device, hfx_la** out);
+/* Destructor (PetscInterface.cpp:25-31). */
+void hfx_la_free(hfx_la* h);
+
+/* LinAlgebraInterface::initialize (PetscInterface.cpp:48-58). */
+int hfx_la_initialize(hfx_la* h);
+
+/* LinAlgebraInterface::configure with the PetscOpts fields (PetscInterface.cpp:60-86,
+ * PetscOpts.h:9-31): solver_type HFX_LA_GMRES, pc HFX_PC_NONE or HFX_PC_JACOBI,
+ * rtol (on the true residual ||b - A x||_2 / ||b||_2), maxits, GMRES restart
+ * (1..63; PETSc's default is 30).  Fails before initialize. */
+int hfx_la_configure(hfx_la* h, int solver_type, int pc, double rtol, int maxits, int restart);
+
+/* LinAlgebraInterface::allocate (PetscInterface.cpp:89-114): ndofs rows;
+ * diag_nnz/off_nnz per-row counts (both NULL or both set) only size the
+ * value stash.  Fails before initialize + configure.  Resets values. */
+int hfx_la_allocate(hfx_la* h, int ndofs, const int* diag_nnz, const int* off_nnz);
+
+/* addValsMatrix / setValsMatrix (PetscInterface.cpp:116-169): the nr x nc
+ * row-major block vals at rows x cols, ADD or INSERT.  ADD and INSERT cannot
+ * be mixed before a flush.  Entries outside the pattern are created.  The
+ * single-value forms are nr = nc = 1. */
+int hfx_la_add_vals(hfx_la* h, int nr, const int* rows, int nc, const int* cols, const double* vals);
+int hfx_la_set_vals(hfx_la* h, int nr, const int* rows, int nc, const int* cols, const double* vals);
+
+/* addValsRHS / setValsRHS (PetscInterface.cpp:134-187). */
+int hfx_la_add_rhs(hfx_la* h, int nr, const int* rows, const double* vals);
+int hfx_la_set_rhs(hfx_la* h, int nr, const int* rows, const double* vals);
+
+/* zeroOutRows (PetscInterface.cpp:189-196): pending values are flushed, then
+ * the rows' stored values are set to zero (pattern kept, RHS untouched). */
+int hfx_la_zero_rows(hfx_la* h, int nr, const int* rows);
+
+/* assemble / assembleFlush (PetscInterface.cpp:198-220). */
+int hfx_la_assemble(hfx_la* h);
+int hfx_la_assemble_flush(hfx_la* h);
+
+/* solve (PetscInterface.cpp:222-247): FP64 FGMRES on the GPU from a zero
+ * guess; x receives ndofs values (n = length of x).  info[0] = iterations,
+ * info[1] = converged reason (2 rtol, 3 zero rhs, -3 max its, -9 nan);
+ * relres[0] = final true relative residual.  Not converging is not an error
+ * (PETSc's KSPSolve default). */
+int hfx_la_solve(hfx_la* h, double* x, int n, int* info, double* relres);
+
+/* getSolutionOwnership (PetscInterface.cpp:249-255): [low, high) of the last
+ * solve's rows (one GPU: [0, ndofs)). */
+int hfx_la_ownership(hfx_la* h, int* low, int* high);
+
+/* clearSystem (PetscInterface.cpp:257-265): values and RHS to zero, pattern kept. */
+int hfx_la_clear(hfx_la* h);
+
+/* destroySystem (PetscInterface.cpp:33-46): back to the constructed state. */
+int hfx_la_destroy_system(hfx_la* h);
+
+/* getNumDofs (LinAlgebraInterface.h:138). */
+int hfx_la_num_dofs(hfx_la* h, int* n);
+
+/* Read back assembled values, row-major nr x nc (absent entries read 0), and
+ * RHS entries: the MatGetValues / VecGetValues reads of the reference's tests
+ * (tests/unittests/resolution/TestPetscInterface.cpp:31-36). */
+int hfx_la_get_values(hfx_la* h, int nr, const int* rows, int nc, const int* cols, double* out);
+int hfx_la_get_rhs(hfx_la* h, int nr, const int* rows, double* out);
+/* Stored entries of the assembled matrix, and the whole assembled system as
+ * CSR (rowptr ndofs+1, columns sorted within a row) with its RHS; any
+ * pointer may be NULL. */
+int hfx_la_nnz(hfx_la* h, long long* nnz);
+int hfx_la_get_csr(hfx_la* h, long long* rowptr, int* cols, double* vals, double* rhs);
+
 #ifdef __cplusplus
 }
 #endif
