@@ -13,3 +13,4 @@ MODEL_LAPLACE = 0
 MODEL_DIFFUSION_SOURCE = 1
 MODEL_CDR_SOURCE = 2
 PC_NONE, PC_JACOBI, PC_BJACOBI, PC_LEARNED = 0, 1, 2, 3
+KRYLOV_FGMRES, KRYLOV_BICGSTAB = 0, 1

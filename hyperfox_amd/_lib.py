@@ -51,6 +51,7 @@ _SIGS = {
     'hfx_hdg_time': ([_P, _I, _I, _I, _P], _I),
     'hfx_element_lds_bytes': ([_I, _I, _P], _I),
     'hfx_hdg_set_kernel': ([_P, _I], _I),
+    'hfx_hdg_set_krylov': ([_P, _I], _I),
     'hfx_hdg_kernel_info': ([_P, _P], _I),
     'hfx_la_create': ([_I, _P], _I),
     'hfx_la_free': ([_P], None),
@@ -73,6 +74,21 @@ _SIGS = {
     'hfx_la_get_rhs': ([_P, _I, _P, _P], _I),
     'hfx_la_nnz': ([_P, _P], _I),
     'hfx_la_get_csr': ([_P] * 5, _I),
+    'hfx_lpc_create': ([_I, _I, _P], _I),
+    'hfx_lpc_free': ([_P], None),
+    'hfx_lpc_num_params': ([_P, _P], _I),
+    'hfx_lpc_init_params': ([_P, ctypes.c_ulonglong], _I),
+    'hfx_lpc_set_params': ([_P, _P, ctypes.c_longlong], _I),
+    'hfx_lpc_get_params': ([_P, _P, ctypes.c_longlong], _I),
+    'hfx_lpc_add_sample': ([_P, _P], _I),
+    'hfx_lpc_num_samples': ([_P, _P], _I),
+    'hfx_lpc_clear_samples': ([_P], _I),
+    'hfx_lpc_graph_sizes': ([_P, _I, _P], _I),
+    'hfx_lpc_loss_grad': ([_P, _I, _I, ctypes.c_ulonglong, _P, _P], _I),
+    'hfx_lpc_train': ([_P, _P, _I, ctypes.c_float, _I, ctypes.c_ulonglong, _P], _I),
+    'hfx_lpc_outputs': ([_P, _I, _I, _P, _P], _I),
+    'hfx_hdg_set_lpc': ([_P, _P, _I], _I),
+    'hfx_hdg_pc_setup_ms': ([_P, _P], _I),
 }
 
 for _name, (_args, _res) in _SIGS.items():

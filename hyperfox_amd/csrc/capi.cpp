@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -16,6 +17,7 @@
 #include "mesh.h"
 #include "refel.h"
 #include "trace_system.h"
+#include "capi_internal.h"
 
 namespace {
 thread_local std::string g_err;
@@ -34,91 +36,6 @@ void set_last_error(const std::string& msg) { g_err = msg; }
     return -1;                        \
   }                                   \
   return 0;
-
-struct hfx_mesh {
-  std::unique_ptr<hfx::RefEl> re;
-  hfx::Mesh m;
-};
-
-struct hfx_hdg {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int dim = 0, order = 0;
-  std::unique_ptr<hfx::RefEl> re;
-  hfx::Mesh mesh;
-  bool has_mesh = false, has_fields = false, assembled = false, solved = false;
-  int kind = 1, ndD = 0, tau_sides = 1;
-  bool hasD = false, hasB = false, hasF = false, hasR = false;
-  int nt = 0, grid_cap = 0;
-  bool affine = false;     // every cell straight-sided (constant Jacobian)
-  int kernel_choice = 0;   // 0 auto, 1 general, 2 affine
-  // device data
-  hfx::DevBuf<double> tabs, tabs_aff, coords, Dn, bn, tau, f_ip, r_ip, g;
-  hfx::DevBuf<int> faceNodes, cells, c2f, f2c;
-  hfx::DevBuf<int8_t> perm, invperm;
-  hfx::DevBuf<uint8_t> is_dir;
-  hfx::DevBuf<double> S, S0, U, Q, U0, Q0, rhs, lambda, dinv, u, q;
-  hfx::BsrPattern pat;
-  hfx::BsrDev A;
-  // last solve options
-  int last_pc = HFX_PC_BJACOBI, last_maxits = 10000, last_restart = 30;
-  double last_rtol = 1e-10;
-
-  void set_device() const { HFX_HIP(hipSetDevice(device)); }
-  hfx::ElemArgs elem_args() {
-    hfx::ElemArgs a;
-    a.nEl = mesh.nCells;
-    a.kind = kind;
-    a.useDiff = (kind != HFX_MODEL_CDR_SOURCE) || hasD;
-    a.ndD = hasD ? ndD : 0;
-    a.hasB = hasB;
-    a.hasF = hasF;
-    a.hasR = hasR;
-    a.tau_sides = tau_sides;
-    a.tabs = tabs.p;
-    a.faceNodes = faceNodes.p;
-    a.coords = coords.p;
-    a.cells = cells.p;
-    a.cell2face = c2f.p;
-    a.face2cell = f2c.p;
-    a.face_perm = perm.p;
-    a.Dn = Dn.p;
-    a.bn = bn.p;
-    a.tau = tau.p;
-    a.f_ip = f_ip.p;
-    a.r_ip = r_ip.p;
-    a.S = S.p; a.S0 = S0.p; a.U = U.p; a.Q = Q.p; a.U0 = U0.p; a.Q0 = Q0.p;
-    return a;
-  }
-  int threads() const {
-    if (nt) return nt;
-    return (dim == 3 && order >= 3) ? 256 : (dim == 3 || order >= 4 ? 128 : 64);
-  }
-  bool use_affine() const {
-    if (kernel_choice == 1) return false;
-    const bool ok = affine && !(hasD && ndD == dim * dim);
-    if (kernel_choice == 2 && !ok)
-      throw std::runtime_error("affine element kernel needs straight-sided cells and a scalar diffusion field");
-    return ok;
-  }
-  int affine_waves() const {
-    for (int w : {4, 3, 2, 1})
-      if (hfx::affine_lds_bytes(dim, order, false, w) + 4096 <= 160 * 1024) return w;
-    return 1;
-  }
-  void run_element() {
-    hfx::ElemArgs a = elem_args();
-    if (use_affine()) {
-      a.tabs = tabs_aff.p;
-      hfx::launch_affine(dim, order, false, affine_waves(), a, grid_cap, stream);
-    } else {
-      hfx::launch_element(dim, order, a, threads(), grid_cap, stream);
-    }
-  }
-  void run_scatter() {
-    hfx::launch_scatter(A, mesh.nFpc, S.p, S0.p, invperm.p, is_dir.p, g.p, rhs.p, stream);
-  }
-};
 
 namespace {
 
@@ -360,6 +277,9 @@ extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
   require(h && mm, "hfx_hdg_set_mesh: null handle");
   require(mm->m.dim == h->dim && mm->re->order == h->order, "hfx_hdg_set_mesh: mesh dimension/order mismatch");
   h->set_device();
+  static std::atomic<unsigned long long> serial{0};
+  h->mesh_serial = ++serial;
+  h->lpc_state.reset();
   h->mesh = mm->m;
   h->mesh.order = h->order;
   h->affine = mesh_is_affine(*h->re, h->mesh);
@@ -576,18 +496,31 @@ void do_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, hfx::Kry
     h->dinv.alloc((size_t)h->A.nb * h->A.bs * h->A.bs);
     hfx::bsr_invert_diag(h->A, h->dinv.p, s);
     M = [h, s](const double* in, double* out) { hfx::apply_block_diag(h->A.nb, h->A.bs, h->dinv.p, in, out, s); };
+  } else if (pc == HFX_PC_LEARNED) {
+    hfx::lpc_prepare(h);
+    const hfx::LpcInfer* st = h->lpc_state.get();
+    M = [st, s](const double* in, double* out) { hfx::bsr_spmv_f32vals(st->g, st->mvals.p, in, out, s); };
   } else if (pc != HFX_PC_NONE) {
-    throw std::runtime_error("hfx_hdg_solve: unsupported preconditioner for this entry point");
+    throw std::runtime_error("hfx_hdg_solve: unknown preconditioner");
   }
   HFX_HIP(hipMemsetAsync(h->lambda.p, 0, n * sizeof(double), s));
   hfx::KrylovOpts o;
   o.rtol = rtol;
   o.maxits = maxits;
   o.restart = restart;
-  hfx::fgmres(h->A, h->rhs.p, h->lambda.p, M, o, info, s);
+  if (h->krylov == HFX_KRYLOV_BICGSTAB) hfx::bicgstab(h->A, h->rhs.p, h->lambda.p, M, o, info, s);
+  else hfx::fgmres(h->A, h->rhs.p, h->lambda.p, M, o, info, s);
 }
 
 }  // namespace
+
+extern "C" int hfx_hdg_set_krylov(hfx_hdg* h, int method) {
+  HFX_TRY
+  require(h != nullptr, "hfx_hdg_set_krylov: null handle");
+  require(method == HFX_KRYLOV_FGMRES || method == HFX_KRYLOV_BICGSTAB, "hfx_hdg_set_krylov: unknown method");
+  h->krylov = method;
+  HFX_END
+}
 
 extern "C" int hfx_hdg_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, int* info, double* relres) {
   HFX_TRY

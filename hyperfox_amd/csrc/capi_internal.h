@@ -1,0 +1,129 @@
+// Handle types of the C ABI, shared by capi.cpp (mesh, HDG bulk path) and
+// capi_lpc.cpp (learned preconditioner).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <stdexcept>
+#include <vector>
+
+#include "../../include/hfx_hip.h"
+#include "hdg_element.h"
+#include "hfx_internal.h"
+#include "krylov.h"
+#include "lpc.h"
+#include "mesh.h"
+#include "refel.h"
+#include "trace_system.h"
+
+namespace hfx {
+// Inference state of the learned preconditioner for one problem's pattern.
+struct LpcInfer {
+  LpcGraph g;
+  LpcSystem sys;
+  LpcActs acts;
+  DevBuf<float> mvals;
+  bool graph_built = false;
+};
+}  // namespace hfx
+
+struct hfx_hdg;
+namespace hfx {
+// Build the learned preconditioner matrix for the problem's current trace
+// matrix (network attached with hfx_hdg_set_lpc); defined in capi_lpc.cpp.
+void lpc_prepare(hfx_hdg* h);
+}  // namespace hfx
+
+struct hfx_mesh {
+  std::unique_ptr<hfx::RefEl> re;
+  hfx::Mesh m;
+};
+
+struct hfx_hdg {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int dim = 0, order = 0;
+  std::unique_ptr<hfx::RefEl> re;
+  hfx::Mesh mesh;
+  bool has_mesh = false, has_fields = false, assembled = false, solved = false;
+  int kind = 1, ndD = 0, tau_sides = 1;
+  bool hasD = false, hasB = false, hasF = false, hasR = false;
+  int nt = 0, grid_cap = 0;
+  bool affine = false;     // every cell straight-sided (constant Jacobian)
+  int kernel_choice = 0;   // 0 auto, 1 general, 2 affine
+  // device data
+  hfx::DevBuf<double> tabs, tabs_aff, coords, Dn, bn, tau, f_ip, r_ip, g;
+  hfx::DevBuf<int> faceNodes, cells, c2f, f2c;
+  hfx::DevBuf<int8_t> perm, invperm;
+  hfx::DevBuf<uint8_t> is_dir;
+  hfx::DevBuf<double> S, S0, U, Q, U0, Q0, rhs, lambda, dinv, u, q;
+  hfx::BsrPattern pat;
+  hfx::BsrDev A;
+  // learned preconditioner (attached network, per-problem inference state)
+  unsigned long long mesh_serial = 0;  // distinct for every set_mesh in the process
+  hfx_lpc* lpc = nullptr;
+  int lpc_bf16 = 1;
+  std::unique_ptr<hfx::LpcInfer> lpc_state;
+  double last_pc_setup_ms = 0.0;
+  int krylov = HFX_KRYLOV_FGMRES;
+  // last solve options
+  int last_pc = HFX_PC_BJACOBI, last_maxits = 10000, last_restart = 30;
+  double last_rtol = 1e-10;
+
+  void set_device() const { HFX_HIP(hipSetDevice(device)); }
+  hfx::ElemArgs elem_args() {
+    hfx::ElemArgs a;
+    a.nEl = mesh.nCells;
+    a.kind = kind;
+    a.useDiff = (kind != HFX_MODEL_CDR_SOURCE) || hasD;
+    a.ndD = hasD ? ndD : 0;
+    a.hasB = hasB;
+    a.hasF = hasF;
+    a.hasR = hasR;
+    a.tau_sides = tau_sides;
+    a.tabs = tabs.p;
+    a.faceNodes = faceNodes.p;
+    a.coords = coords.p;
+    a.cells = cells.p;
+    a.cell2face = c2f.p;
+    a.face2cell = f2c.p;
+    a.face_perm = perm.p;
+    a.Dn = Dn.p;
+    a.bn = bn.p;
+    a.tau = tau.p;
+    a.f_ip = f_ip.p;
+    a.r_ip = r_ip.p;
+    a.S = S.p; a.S0 = S0.p; a.U = U.p; a.Q = Q.p; a.U0 = U0.p; a.Q0 = Q0.p;
+    return a;
+  }
+  int threads() const {
+    if (nt) return nt;
+    return (dim == 3 && order >= 3) ? 256 : (dim == 3 || order >= 4 ? 128 : 64);
+  }
+  bool use_affine() const {
+    if (kernel_choice == 1) return false;
+    const bool ok = affine && !(hasD && ndD == dim * dim);
+    if (kernel_choice == 2 && !ok)
+      throw std::runtime_error("affine element kernel needs straight-sided cells and a scalar diffusion field");
+    return ok;
+  }
+  int affine_waves() const {
+    for (int w : {4, 3, 2, 1})
+      if (hfx::affine_lds_bytes(dim, order, false, w) + 4096 <= 160 * 1024) return w;
+    return 1;
+  }
+  void run_element() {
+    hfx::ElemArgs a = elem_args();
+    if (use_affine()) {
+      a.tabs = tabs_aff.p;
+      hfx::launch_affine(dim, order, false, affine_waves(), a, grid_cap, stream);
+    } else {
+      hfx::launch_element(dim, order, a, threads(), grid_cap, stream);
+    }
+  }
+  void run_scatter() {
+    hfx::launch_scatter(A, mesh.nFpc, S.p, S0.p, invperm.p, is_dir.p, g.p, rhs.p, stream);
+  }
+};
+

@@ -50,6 +50,7 @@ struct hfx_la {
   int pc = HFX_PC_JACOBI;
   double rtol = 1e-6;
   int maxits = 1000, restart = 30;
+  int solver = HFX_LA_GMRES;
   // assembled matrix: CSR with sorted columns, and the RHS
   std::vector<int> rowptr = std::vector<int>(1, 0), col;
   std::vector<double> val, rhs;
@@ -222,10 +223,12 @@ extern "C" int hfx_la_configure(hfx_la* h, int solver_type, int pc, double rtol,
   LA_TRY
   LA_HANDLE(h);
   if (!h->initialized) throw LaError("configure", "the object was not initialized before being configured.");
-  if (solver_type != HFX_LA_GMRES) throw LaError("configure", "only GMRES is available");
+  if (solver_type != HFX_LA_GMRES && solver_type != HFX_LA_BCGS)
+    throw LaError("configure", "the solver type must be GMRES or BiCGStab");
   if (pc != HFX_PC_NONE && pc != HFX_PC_JACOBI) throw LaError("configure", "the preconditioner must be none or Jacobi");
   if (!(rtol > 0.0) || maxits <= 0 || restart < 1 || restart > 63)
     throw LaError("configure", "rtol must be > 0, maxits > 0 and restart in [1, 63]");
+  h->solver = solver_type;
   h->pc = pc;
   h->rtol = rtol;
   h->maxits = maxits;
@@ -344,7 +347,8 @@ extern "C" int hfx_la_solve(hfx_la* h, double* x, int n, int* info, double* relr
     o.rtol = h->rtol;
     o.maxits = h->maxits;
     o.restart = h->restart;
-    hfx::fgmres(h->A, h->db.p, h->dx.p, M, o, ki, s);
+    if (h->solver == HFX_LA_BCGS) hfx::bicgstab(h->A, h->db.p, h->dx.p, M, o, ki, s);
+    else hfx::fgmres(h->A, h->db.p, h->dx.p, M, o, ki, s);
     h->dx.download(x, h->n, s);
     HFX_HIP(hipStreamSynchronize(s));
   }

@@ -61,4 +61,11 @@ struct Reducer {
 void fgmres(const BsrDev& A, const double* b, double* x, const std::function<void(const double*, double*)>& precond,
             const KrylovOpts& o, KrylovInfo& info, hipStream_t s);
 
+// Right-preconditioned BiCGStab with device-resident scalars (no host round
+// trip per iteration; the recurrence residual is read back every 8
+// iterations).  Convergence is confirmed on the true FP64 residual; a
+// recurrence that converges early or breaks down restarts from it.
+void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<void(const double*, double*)>& precond,
+              const KrylovOpts& o, KrylovInfo& info, hipStream_t s);
+
 }  // namespace hfx

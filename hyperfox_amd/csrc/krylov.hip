@@ -171,6 +171,106 @@ __global__ void combine_kernel(long long n, int k, const double* __restrict__ Z,
   x[t] = acc;
 }
 
+
+// ---------------------------------------------------------------- BiCGStab
+//
+// Scalars live on the device (no host round trip per iteration):
+//   sc[0] rho = (rhat, r)     sc[1] alpha     sc[2] omega     sc[3] rho_prev
+//   sc[4] (rhat, v)           sc[5] (t, s)    sc[6] (t, t)    sc[7] (r, r)
+//   sc[8] breakdown flag (1 when a denominator vanished)
+enum { SC_RHO = 0, SC_ALPHA, SC_OMEGA, SC_RHOP, SC_RV, SC_TS, SC_TT, SC_RR, SC_BRK, SC_N };
+
+// partial[j * G + blk] = sum over the block's slice of a_j . b_j, j < nd
+__global__ void dots_kernel(long long n, int nd, const double* __restrict__ a0, const double* __restrict__ b0,
+                            const double* __restrict__ a1, const double* __restrict__ b1,
+                            double* __restrict__ partial) {
+  __shared__ double red[RT / 64];
+  const int G = gridDim.x;
+  double s0 = 0.0, s1 = 0.0;
+  for (long long i = (long long)blockIdx.x * RT + threadIdx.x; i < n; i += (long long)G * RT) {
+    s0 += a0[i] * b0[i];
+    if (nd > 1) s1 += a1[i] * b1[i];
+  }
+  double t0 = block_sum(s0, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t0;
+  if (nd > 1) {
+    __syncthreads();
+    double t1 = block_sum(s1, red);
+    if (threadIdx.x == 0) partial[G + blockIdx.x] = t1;
+  }
+}
+
+// Reduce nd partial rows (fixed order) and update the scalars:
+//   op 0: rho_prev = rho; rho = d0; rr = d1          (after the r update)
+//   op 1: rv = d0; alpha = rho / rv                   (after v = A p^)
+//   op 2: ts = d0; tt = d1; omega = ts / tt           (after t = A s^)
+__global__ void scalars_kernel(int G, int nd, int op, const double* __restrict__ partial, double* __restrict__ sc) {
+  __shared__ double red[RT / 64];
+  double d[2] = {0.0, 0.0};
+  for (int j = 0; j < nd; ++j) {
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < G; b += RT) acc += partial[(size_t)j * G + b];
+    d[j] = block_sum(acc, red);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (op == 0) {
+    sc[SC_RHOP] = sc[SC_RHO];
+    sc[SC_RHO] = d[0];
+    sc[SC_RR] = d[1];
+    if (d[0] == 0.0) sc[SC_BRK] = 1.0;
+  } else if (op == 1) {
+    sc[SC_RV] = d[0];
+    if (d[0] == 0.0 || !isfinite(d[0])) { sc[SC_BRK] = 1.0; sc[SC_ALPHA] = 0.0; }
+    else sc[SC_ALPHA] = sc[SC_RHO] / d[0];
+  } else {
+    sc[SC_TS] = d[0];
+    sc[SC_TT] = d[1];
+    if (d[1] == 0.0 || !isfinite(d[1])) { sc[SC_BRK] = 1.0; sc[SC_OMEGA] = 0.0; }
+    else sc[SC_OMEGA] = d[0] / d[1];
+    if (sc[SC_OMEGA] == 0.0) sc[SC_BRK] = 1.0;
+  }
+}
+
+// p = r + beta (p - omega v), beta = (rho / rho_prev) (alpha / omega);
+// first = 1: p = r
+__global__ void bicg_p_kernel(long long n, int first, const double* __restrict__ sc, const double* __restrict__ r,
+                              double* __restrict__ p, const double* __restrict__ v) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  if (first) { p[t] = r[t]; return; }
+  const double om = sc[SC_OMEGA], rp = sc[SC_RHOP];
+  const double beta = (rp != 0.0 && om != 0.0) ? (sc[SC_RHO] / rp) * (sc[SC_ALPHA] / om) : 0.0;
+  p[t] = r[t] + beta * (p[t] - om * v[t]);
+}
+
+// s = r - alpha v
+__global__ void bicg_s_kernel(long long n, const double* __restrict__ sc, const double* __restrict__ r,
+                              const double* __restrict__ v, double* __restrict__ s) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) s[t] = r[t] - sc[SC_ALPHA] * v[t];
+}
+
+// x += alpha p^ + omega s^ ; r = s - omega t
+__global__ void bicg_xr_kernel(long long n, const double* __restrict__ sc, const double* __restrict__ ph,
+                               const double* __restrict__ sh, const double* __restrict__ s,
+                               const double* __restrict__ tv, double* __restrict__ x, double* __restrict__ r) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double al = sc[SC_ALPHA], om = sc[SC_OMEGA];
+  x[t] += al * ph[t] + om * sh[t];
+  r[t] = s[t] - om * tv[t];
+}
+
+__global__ void set_scalars_kernel(double* __restrict__ sc) {
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < SC_N; ++i) sc[i] = 0.0;
+    sc[SC_ALPHA] = 1.0;
+    sc[SC_OMEGA] = 1.0;
+    sc[SC_RHO] = 1.0;
+  }
+}
+
 }  // namespace
 
 void bsr_spmv(const BsrDev& A, const double* x, double* y, const double* b, hipStream_t s) {
@@ -354,6 +454,90 @@ void fgmres(const BsrDev& A, const double* b, double* x, const std::function<voi
     }
     ++info.restarts;
     (void)done;
+  }
+  info.iters = it;
+  HFX_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace hfx
+
+namespace hfx {
+
+void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<void(const double*, double*)>& precond,
+              const KrylovOpts& o, KrylovInfo& info, hipStream_t s) {
+  const long long n = A.nrows();
+  info = KrylovInfo();
+  Reducer red;
+  red.init(n);
+  const int G = red.grid;
+  DevBuf<double> r, rh, p, v, ph, sv, sh, tv, sc, part;
+  for (DevBuf<double>* q : {&r, &rh, &p, &v, &ph, &sv, &sh, &tv}) q->alloc(n);
+  sc.alloc(SC_N);
+  part.alloc((size_t)2 * G);
+  const double bnorm = red.norm(n, b, s);
+  if (bnorm == 0.0) {
+    HFX_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
+    HFX_HIP(hipStreamSynchronize(s));
+    info.reason = CONVERGED_ATOL;
+    return;
+  }
+  const double target = o.rtol * bnorm;
+  const int nblk = ceil_div(n, 256);
+  auto apply_M = [&](const double* in, double* out) {
+    if (precond) precond(in, out);
+    else HFX_HIP(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  };
+  auto dots = [&](int nd, const double* a0, const double* b0, const double* a1, const double* b1, int op) {
+    hipLaunchKernelGGL(dots_kernel, dim3(G), dim3(RT), 0, s, n, nd, a0, b0, a1, b1, part.p);
+    HFX_HIP(hipGetLastError());
+    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(RT), 0, s, G, nd, op, part.p, sc.p);
+    HFX_HIP(hipGetLastError());
+  };
+  const int check_every = 8;
+  int it = 0;
+  bool restart = true;
+  while (true) {
+    if (restart) {
+      // r = b - A x (true FP64 residual), rhat = r, rho = (r, r)
+      bsr_spmv(A, x, r.p, b, s);
+      const double rn = red.norm(n, r.p, s);
+      info.true_relres = rn / bnorm;
+      if (!std::isfinite(rn)) { info.reason = DIVERGED_NANORINF; break; }
+      if (rn <= target) { info.reason = CONVERGED_RTOL; break; }
+      if (it >= o.maxits) { info.reason = DIVERGED_ITS; break; }
+      HFX_HIP(hipMemcpyAsync(rh.p, r.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(64), 0, s, sc.p);
+      HFX_HIP(hipGetLastError());
+      dots(2, rh.p, r.p, r.p, r.p, 0);
+      ++info.restarts;
+      restart = false;
+      hipLaunchKernelGGL(bicg_p_kernel, dim3(nblk), dim3(256), 0, s, n, 1, sc.p, r.p, p.p, v.p);
+    } else {
+      hipLaunchKernelGGL(bicg_p_kernel, dim3(nblk), dim3(256), 0, s, n, 0, sc.p, r.p, p.p, v.p);
+    }
+    HFX_HIP(hipGetLastError());
+    apply_M(p.p, ph.p);
+    bsr_spmv(A, ph.p, v.p, nullptr, s);
+    dots(1, rh.p, v.p, nullptr, nullptr, 1);
+    hipLaunchKernelGGL(bicg_s_kernel, dim3(nblk), dim3(256), 0, s, n, sc.p, r.p, v.p, sv.p);
+    HFX_HIP(hipGetLastError());
+    apply_M(sv.p, sh.p);
+    bsr_spmv(A, sh.p, tv.p, nullptr, s);
+    dots(2, tv.p, sv.p, tv.p, tv.p, 2);
+    hipLaunchKernelGGL(bicg_xr_kernel, dim3(nblk), dim3(256), 0, s, n, sc.p, ph.p, sh.p, sv.p, tv.p, x, r.p);
+    HFX_HIP(hipGetLastError());
+    dots(2, rh.p, r.p, r.p, r.p, 0);
+    ++it;
+    if (it % check_every == 0 || it >= o.maxits) {
+      double hs[SC_N];
+      HFX_HIP(hipMemcpyAsync(hs, sc.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+      HFX_HIP(hipStreamSynchronize(s));
+      const double rr = std::sqrt(std::fabs(hs[SC_RR]));
+      if (!std::isfinite(hs[SC_RR])) { info.reason = DIVERGED_NANORINF; break; }
+      // converged recurrence, breakdown, or the iteration budget: restart from
+      // (and test) the true residual; maxits bounds the number of restarts
+      if (rr <= target || hs[SC_BRK] != 0.0 || it >= o.maxits) restart = true;
+    }
   }
   info.iters = it;
   HFX_HIP(hipStreamSynchronize(s));

@@ -188,6 +188,10 @@ class HDGProblem:
         check(lib.hfx_hdg_get_trace_csr(self._h, _ptr(rowptr), _ptr(cols), _ptr(vals), _ptr(rhs)))
         return rowptr, cols, vals, rhs
 
+    def set_krylov(self, method):
+        """0 restarted FGMRES (default), 1 BiCGStab."""
+        check(lib.hfx_hdg_set_krylov(self._h, method))
+
     def solve(self, pc=2, rtol=1e-10, maxits=10000, restart=30):
         info = np.zeros(2, dtype=np.int32)
         rel = np.zeros(1)

@@ -16,6 +16,7 @@ import numpy as np
 from ._lib import check, lib
 
 GMRES = 0
+BCGS = 1
 PC_NONE, PC_JACOBI = 0, 1
 
 

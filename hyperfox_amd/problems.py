@@ -137,3 +137,46 @@ def setup_gpu(w, prob):
     f = w.source_at(xip)
     prob.set_source(f, None)
     return f
+
+
+def grf_2d(n, ell=0.1, sigma=1.0, seed=0):
+    """Gaussian random field with squared-exponential covariance
+    sigma^2 exp(-|x - y|^2 / (2 ell^2)) on the (n+1) x (n+1) vertices of the
+    unit-square grid, sampled exactly by circulant embedding: the grid is
+    embedded in a 2n x 2n torus of side 2, the covariance's eigenvalues are
+    the FFT of its first row (negative round-off clipped), and
+    Re FFT(sqrt(lambda / m^2) xi) with complex normal xi is a sample."""
+    m = 2 * n
+    d = np.minimum(np.arange(m), m - np.arange(m)) / n
+    c = sigma * sigma * np.exp(-(d[:, None] ** 2 + d[None, :] ** 2) / (2.0 * ell * ell))
+    lam = np.maximum(np.fft.fft2(c).real, 0.0)
+    rng = np.random.default_rng(seed)
+    xi = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+    g = np.fft.fft2(np.sqrt(lam / (m * m)) * xi).real
+    return g[:n + 1, :n + 1]
+
+
+def bilinear(grid, x):
+    """Bilinear interpolation of vertex values grid[i, j] (at (i/n, j/n)) at points x."""
+    n = grid.shape[0] - 1
+    s = np.clip(x[..., 0] * n, 0.0, n - 1e-12)
+    t = np.clip(x[..., 1] * n, 0.0, n - 1e-12)
+    i, j = np.floor(s).astype(int), np.floor(t).astype(int)
+    fs, ft = s - i, t - j
+    return ((1 - fs) * (1 - ft) * grid[i, j] + fs * (1 - ft) * grid[i + 1, j]
+            + (1 - fs) * ft * grid[i, j + 1] + fs * ft * grid[i + 1, j + 1])
+
+
+def diffusion_grf_2d(n, p, seed, ell=0.1, sigma=1.0, jitter=0.1, mesh=None, mesh_seed=2024):
+    """The learned-preconditioner workload: unit square, n x n squares split
+    in two (jittered +-jitter h), diffusion coefficient k = exp(g) with g a
+    GRF (ell, sigma) drawn from `seed`, source f = 1, u = 0 on the boundary,
+    tau = 1.  k is a scalar node field (the reference's "DiffusionTensor"
+    with one entry per node) interpolated bilinearly from the grid."""
+    m = mesh if mesh is not None else Mesh.structured(2, p, n, jitter, mesh_seed)
+    g = grf_2d(n, ell, sigma, seed)
+    k = np.exp(bilinear(g, m.arrays()['coords']))
+    tau = np.ones((m.n_faces, m.sizes['nNf']))
+    return Workload('diffusion-grf-2d-n%d-p%d-s%d' % (n, p, seed), 2, p, m, 1, tau, 1, D=k, ndD=1,
+                    dirichlet=np.zeros((m.n_faces, m.sizes['nNf'])), f_fn=lambda xx: np.ones(xx.shape[:-1]),
+                    h=1.0 / n)

@@ -147,8 +147,14 @@ int hfx_hdg_trace_sizes(hfx_hdg* h, long long* out);
 /* Trace system as CSR (rows sorted by column) and its RHS (after assemble). */
 int hfx_hdg_get_trace_csr(hfx_hdg* h, long long* rowptr, int* cols, double* vals, double* rhs);
 
-/* Trace solve (LinAlgebraInterface::solve, HDGSolver.cpp:699-708): restarted
- * FGMRES in FP64 with preconditioner `pc`, zero initial guess, stopping when
+/* Krylov method of hfx_hdg_solve: restarted FGMRES (default; the reference's
+ * KSPGMRES) or BiCGStab with device-resident scalars (KSPBCGS). */
+enum { HFX_KRYLOV_FGMRES = 0, HFX_KRYLOV_BICGSTAB = 1 };
+int hfx_hdg_set_krylov(hfx_hdg* h, int method);
+
+/* Trace solve (LinAlgebraInterface::solve, HDGSolver.cpp:699-708): FP64
+ * Krylov method of hfx_hdg_set_krylov with preconditioner `pc` (restart
+ * applies to FGMRES), zero initial guess, stopping when
  * the true residual ||b - A lambda||_2 / ||b||_2 <= rtol.  info[0] =
  * iterations, info[1] = converged reason (2 rtol, 3 zero rhs, -3 max its,
  * -9 nan); relres[0] = final true relative residual. */
@@ -177,8 +183,9 @@ int hfx_element_lds_bytes(int dim, int order, long long* bytes);
  * hfx_la_solve touches the GPU. */
 typedef struct hfx_la hfx_la;
 
-/* Solver types of hfx_la_configure (PetscOpts::solverType, PetscOpts.h:14). */
-enum { HFX_LA_GMRES = 0 };
+/* Solver types of hfx_la_configure (PetscOpts::solverType, PetscOpts.h:14):
+ * KSPGMRES and KSPBCGS. */
+enum { HFX_LA_GMRES = 0, HFX_LA_BCGS = 1 };
 
 /* Construction (PetscInterface::PetscInterface, PetscInterface.cpp:5-12):
  * a system that will solve on GPU `device`. */
@@ -190,7 +197,7 @@ void hfx_la_free(hfx_la* h);
 int hfx_la_initialize(hfx_la* h);
 
 /* LinAlgebraInterface::configure with the PetscOpts fields (PetscInterface.cpp:60-86,
- * PetscOpts.h:9-31): solver_type HFX_LA_GMRES, pc HFX_PC_NONE or HFX_PC_JACOBI,
+ * PetscOpts.h:9-31): solver_type HFX_LA_GMRES or HFX_LA_BCGS, pc HFX_PC_NONE or HFX_PC_JACOBI,
  * rtol (on the true residual ||b - A x||_2 / ||b||_2), maxits, GMRES restart
  * (1..63; PETSc's default is 30).  Fails before initialize. */
 int hfx_la_configure(hfx_la* h, int solver_type, int pc, double rtol, int maxits, int restart);
@@ -249,6 +256,61 @@ int hfx_la_get_rhs(hfx_la* h, int nr, const int* rows, double* out);
  * pointer may be NULL. */
 int hfx_la_nnz(hfx_la* h, long long* nnz);
 int hfx_la_get_csr(hfx_la* h, long long* rowptr, int* cols, double* vals, double* rhs);
+
+/* ------------------------------------------------ learned preconditioner (lpc) */
+
+/* No counterpart in the reference.  A message-passing network over the
+ * face-block graph of the trace matrix (three layers, width 64) emits one
+ * correction block per matrix block, added to the face-block Jacobi inverse
+ * in the diagonally normalised system; HFX_PC_LEARNED in hfx_hdg_solve then
+ * applies M = S^-1/2 (inv(Ahat_FF) + C) S^-1/2 inside FGMRES, which still
+ * stops on the true FP64 residual.  Training runs on the GPU (FP32 forward,
+ * backward and Adam); inference runs on BF16 MFMA with FP32 accumulation. */
+typedef struct hfx_lpc hfx_lpc;
+
+/* A network for face blocks of `block_size` (= nNf) on GPU `device`,
+ * parameters all zero (= block Jacobi). */
+int hfx_lpc_create(int device, int block_size, hfx_lpc** out);
+void hfx_lpc_free(hfx_lpc* h);
+int hfx_lpc_num_params(hfx_lpc* h, long long* n);
+/* He-uniform weights from a splitmix64 stream, zero biases and decoders
+ * (the untrained network is exactly block Jacobi); resets Adam. */
+int hfx_lpc_init_params(hfx_lpc* h, unsigned long long seed);
+int hfx_lpc_set_params(hfx_lpc* h, const float* p, long long n);
+int hfx_lpc_get_params(hfx_lpc* h, float* p, long long n);
+
+/* Add an assembled and solved problem to the training set (its normalised
+ * trace matrix and solution are copied on the GPU; the problem may then be
+ * re-assembled with other fields).  Samples of one problem handle share a
+ * graph; problems on different meshes may be mixed. */
+int hfx_lpc_add_sample(hfx_lpc* h, hfx_hdg* prob);
+int hfx_lpc_num_samples(hfx_lpc* h, int* n);
+int hfx_lpc_clear_samples(hfx_lpc* h);
+/* Graph of one sample: out[0..3] = nodes (faces), matrix blocks, edges
+ * (off-diagonal blocks), block size. */
+int hfx_lpc_graph_sizes(hfx_lpc* h, int sample, long long* out);
+
+/* Loss and gradient (FP32 forward/backward on the GPU) of one sample with the
+ * normalised solution probe plus n_random Rademacher probes drawn from
+ * `seed`.  loss = sum over probes ||z - Mhat Ahat z||^2 / ||z||^2.  grad (may be
+ * NULL) has hfx_lpc_num_params entries, in the parameter layout. */
+int hfx_lpc_loss_grad(hfx_lpc* h, int sample, int n_random, unsigned long long seed, double* loss, float* grad);
+
+/* nsteps Adam steps (lr, betas 0.9/0.999, eps 1e-8), step i on sample
+ * order[i] with probe seed seed + i; losses[i] = loss before step i. */
+int hfx_lpc_train(hfx_lpc* h, const int* order, int nsteps, float lr, int n_random, unsigned long long seed,
+                  double* losses);
+
+/* Network outputs of one sample: cn nodes*b*b (diagonal corrections), ce
+ * edges*b*b (off-diagonal blocks in BSR order); bf16 != 0 uses the MFMA path. */
+int hfx_lpc_outputs(hfx_lpc* h, int sample, int bf16, float* cn, float* ce);
+
+/* Attach a network to a problem for HFX_PC_LEARNED (NULL detaches);
+ * bf16 != 0 runs inference on BF16 MFMA, else FP32. */
+int hfx_hdg_set_lpc(hfx_hdg* h, hfx_lpc* lpc, int bf16);
+/* Host wall time of the last learned-preconditioner set-up (normalise +
+ * network + M assembly), ms. */
+int hfx_hdg_pc_setup_ms(hfx_hdg* h, double* ms);
 
 #ifdef __cplusplus
 }

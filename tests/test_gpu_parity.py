@@ -156,6 +156,27 @@ def test_solution_matches_oracle(dim, p, kind, pc):
     assert rel(q, qo) < 1e-10
 
 
+@pytest.mark.parametrize('pc', [hf.PC_NONE, hf.PC_BJACOBI])
+@pytest.mark.parametrize('kind', [1, 2])
+@pytest.mark.parametrize('dim,p', [(2, 2), (2, 4), (3, 2), (3, 3)])
+def test_bicgstab_solution_matches_oracle(dim, p, kind, pc):
+    """The BiCGStab trace solve reaches the same solution (true residual
+    checked on the GPU, solution against the oracle's dense LU)."""
+    w = small_workload(dim, p, kind)
+    prob, f = run_gpu(w)
+    prob.assemble()
+    prob.set_krylov(hf.KRYLOV_BICGSTAB)
+    its, reason, relres = prob.solve(pc=pc, rtol=1e-13, maxits=20000)
+    assert reason == 2 and relres <= 1e-13
+    u, q, lam = prob.solution()
+    s = oracle_system(w, f)
+    lo, _, _, _ = s.solve(method=1)
+    uo, qo = s.reconstruct(lo)
+    assert rel(lam, lo) < 1e-10
+    assert rel(u, uo) < 1e-10
+    assert rel(q, qo) < 1e-10
+
+
 def test_constant_dirichlet_light_tri2():
     z = np.load(os.path.join(GOLDEN, 'meshes', 'lightTri2.npz'))
     m = hf.Mesh.from_cells(2, 2, z['nodes'], z['cells'])

@@ -44,8 +44,9 @@ def test_ordering_then_solve_and_clear():
     assert la.last_info[1] == 3
 
 
-def test_identity_solving():
-    la = HipLinAlgebra(HipOpts(rtol=1e-16))
+@pytest.mark.parametrize('solver', [0, 1])
+def test_identity_solving(solver):
+    la = HipLinAlgebra(HipOpts(solverType=solver, rtol=1e-16))
     for k in range(100):
         fresh(la, k + 1)
         for i in range(k + 1):
@@ -74,8 +75,9 @@ def test_triangular_matrix():
     print('triangular: worst error %.2e' % worst)
 
 
-def test_hinge_matrix():
-    la = HipLinAlgebra(HipOpts(rtol=1e-16))
+@pytest.mark.parametrize('solver', [0, 1])
+def test_hinge_matrix(solver):
+    la = HipLinAlgebra(HipOpts(solverType=solver, rtol=1e-16))
     for k in range(100):
         fresh(la, k + 1)
         for i in range(k + 1):
@@ -89,8 +91,9 @@ def test_hinge_matrix():
         assert np.allclose(sol, ana, rtol=0, atol=1e-12), (k, la.last_info)
 
 
+@pytest.mark.parametrize('solver', [0, 1])
 @pytest.mark.parametrize('dim,p,n', [(2, 3, 8), (3, 2, 3)])
-def test_hdg_trace_system_through_the_interface(dim, p, n):
+def test_hdg_trace_system_through_the_interface(dim, p, n, solver):
     m = hf.Mesh.structured(dim, p, n, jitter=0.1, seed=11)
     a = m.arrays()
     s = m.sizes
@@ -99,7 +102,7 @@ def test_hdg_trace_system_through_the_interface(dim, p, n):
     g = np.linspace(-1.0, 2.0, m.n_faces * nNf).reshape(m.n_faces, nNf)
     sysm = orc.System(dim, p, 0, a, tau, dirichlet=g)
     cond = sysm.condensed()
-    la = HipLinAlgebra(HipOpts(rtol=1e-13, maxits=20000))
+    la = HipLinAlgebra(HipOpts(solverType=solver, rtol=1e-13, maxits=20000))
     la.initialize()
     la.configure()
     la.allocate(m.n_trace)
