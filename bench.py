@@ -6,8 +6,10 @@ manufactured u = e^{x+y+z} sin(pi x) sin(pi y) sin(pi z), upwind
 double-valued tau.  One step = one pass of [element assembly + condensation +
 Dirichlet rows + BSR scatter] over all elements, inputs resident in HBM.
 value = u-DOFs (cells x nodes per cell) per second over the K timed steps.
-The trace solve (FGMRES + block Jacobi to a true FP64 relative residual of
-1e-10) is timed once after the steps and reported beside the main number.
+The trace solve (BiCGStab and FGMRES(30), block Jacobi, to a true FP64
+relative residual of 1e-10) is timed after the steps and reported beside the
+main number, with the learned preconditioner against block Jacobi on the
+2-D GRF-coefficient workload (256 x 256 x 2 triangles, p = 3).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 24] [--no-cpu]
        python bench.py --copy-calibration MiB   (a plain device copy of known
@@ -76,6 +78,120 @@ def cpu_baseline(n_cpu, p, threads):
                       % (n_cpu, w.mesh.n_cells, best)}
 
 
+def solver_bytes_per_iter(n, nb, bs, nnzb, method):
+    """Algorithmic HBM bytes of one Krylov iteration with block Jacobi:
+    BiCGStab = 2 BSR SpMV (values, column ids, x and y once) + 2 block-diagonal
+    applies + 21 vector passes; FGMRES at Krylov dimension ~15 = 1 SpMV + 1
+    apply + 2 x (2k + 3) vector passes (two Gram-Schmidt sweeps)."""
+    spmv = nnzb * bs * bs * 8 + nnzb * 4 + 2 * n * 8
+    bj = nb * bs * bs * 8 + 2 * n * 8
+    if method == 'bicgstab':
+        return 2 * spmv + 2 * bj + 21 * n * 8
+    return spmv + bj + 2 * (2 * 15 + 3) * n * 8
+
+
+def trace_solves(hf, pr, prob, w):
+    """Trace solve of the benchmark problem to a true residual of 1e-10 with
+    block Jacobi: BiCGStab (the default here) and the reference's GMRES(30)
+    (as FGMRES)."""
+    n_dofs, nnz, nnzb = prob.trace_sizes()
+    bs = prob.sizes['nNf']
+    nb = n_dofs // bs
+    out = {}
+    for name, kry in (('bicgstab', hf.KRYLOV_BICGSTAB), ('fgmres30', hf.KRYLOV_FGMRES)):
+        prob.set_krylov(kry)
+        t = time.perf_counter()
+        its, reason, relres = prob.solve(pc=hf.PC_BJACOBI, rtol=1e-10, maxits=20000, restart=30)
+        ms = (time.perf_counter() - t) * 1e3
+        by = solver_bytes_per_iter(n_dofs, nb, bs, nnzb, name) * its
+        out[name] = {'pc': 'block-jacobi', 'iters': its, 'reason': reason, 'true_relres': relres, 'ms': ms,
+                     'algorithmic_GB': by / 1e9, 'achieved_GBs': by / (ms * 1e-3) / 1e9}
+        print('# solve %s: %d its, relres %.2e, %.1f ms' % (name, its, relres, ms), file=sys.stderr, flush=True)
+    prob.set_krylov(hf.KRYLOV_BICGSTAB)
+    prob.solve(pc=hf.PC_BJACOBI, rtol=1e-10, maxits=20000)
+    u, q, lam = prob.solution()
+    a = w.mesh.arrays()
+    ue = pr.u_c3(a['coords'][a['cells']])
+    out['nodal_rel_err_vs_exact'] = float(np.linalg.norm(u - ue) / np.linalg.norm(ue))
+    return out
+
+
+def learned_vs_plain(hf, pr, n):
+    """Learned-preconditioner workload: 2-D HDG diffusion, n x n x 2 jittered
+    triangles, p = 3, k = exp(GRF(ell 0.1, sigma 1)), f = 1, held-out GRF
+    seeds; BiCGStab to a true residual of 1e-10 with block Jacobi and with the
+    trained network (BF16 MFMA inference; set-up time included)."""
+    path = os.path.join(ROOT, 'hyperfox_amd', 'data', 'lpc_grf2d_p3.npy')
+    if not os.path.exists(path):
+        return {'skipped': 'no trained network at hyperfox_amd/data/lpc_grf2d_p3.npy'}
+    from hyperfox_amd.lpc import LearnedPreconditioner, pc_setup_ms
+    net = LearnedPreconditioner(4)
+    net.load(path)
+    mesh = hf.Mesh.structured(2, 3, n, 0.1, 2024)
+    prob = hf.HDGProblem(2, 3)
+    prob.set_krylov(hf.KRYLOV_BICGSTAB)
+    net.attach(prob, bf16=True)
+    rows = []
+    for i, seed in enumerate((900000 + n * 1000 + 500, 900000 + n * 1000 + 501)):
+        w = pr.diffusion_grf_2d(n, 3, seed, mesh=mesh)
+        if i == 0:
+            pr.setup_gpu(w, prob)
+        else:
+            prob.set_fields(w.tau, w.tau_sides, w.D, w.ndD, w.b, w.dirichlet)
+        prob.assemble()
+        prob.solve(pc=hf.PC_BJACOBI, rtol=1e-10, maxits=50000)  # warm-up (allocations)
+        t = time.perf_counter()
+        ib, rb, relb = prob.solve(pc=hf.PC_BJACOBI, rtol=1e-10, maxits=50000)
+        tb = (time.perf_counter() - t) * 1e3
+        lam_b = prob.solution()[2].copy()
+        prob.solve(pc=hf.PC_LEARNED, rtol=1e-10, maxits=50000)
+        t = time.perf_counter()
+        il, rl, rell = prob.solve(pc=hf.PC_LEARNED, rtol=1e-10, maxits=50000)
+        tl = (time.perf_counter() - t) * 1e3
+        lam_l = prob.solution()[2]
+        rows.append({'seed': seed, 'plain_iters': ib, 'plain_ms': tb, 'plain_relres': relb, 'learned_iters': il,
+                     'learned_ms': tl, 'learned_setup_ms': pc_setup_ms(prob), 'learned_relres': rell,
+                     'reasons': [rb, rl],
+                     'solution_rel_diff': float(np.abs(lam_l - lam_b).max() / np.abs(lam_b).max())})
+        print('# learned vs plain n=%d seed %d: plain %d its %.1f ms, learned %d its %.1f ms' % (
+            n, seed, ib, tb, il, tl), file=sys.stderr, flush=True)
+    return {'workload': 'HDG 2D diffusion, %dx%dx2=%d tris, p=3, k=exp(GRF ell=0.1 sigma=1), f=1' % (n, n, 2 * n * n),
+            'trace_dofs': int(mesh.n_trace), 'method': 'BiCGStab FP64, true relres 1e-10', 'plain_pc': 'block-jacobi',
+            'learned_pc': 'block-jacobi + GNN correction (BF16 MFMA inference)', 'runs': rows,
+            'iters_ratio': float(np.mean([r['plain_iters'] / r['learned_iters'] for r in rows])),
+            'time_ratio': float(np.mean([r['plain_ms'] / r['learned_ms'] for r in rows]))}
+
+
+def batched(hf, pr, count=64, seed=7):
+    """Batched workload: `count` independent 2-D diffusion problems (cells
+    log-uniform in [1k, 40k], p uniform in 1..5, D log-uniform in
+    [1e-2, 1e2], 8-mode Fourier sources) assembled by one persistent launch
+    whose (waves, tile) the autotuner picks, eager and as a replayed HIP graph,
+    against the same problems launched one by one."""
+    specs = pr.batch_specs(count=count, seed=seed)
+    probs = []
+    for sp in specs:
+        w = pr.batch_workload(sp)
+        prob = hf.HDGProblem(2, sp['p'])
+        pr.setup_gpu(w, prob)
+        probs.append(prob)
+    b = hf.Batch()
+    for prob in probs:
+        b.add(prob)
+    dofs = sum(prob.mesh.n_cells * prob.sizes['nN'] for prob in probs)
+    cells = sum(prob.mesh.n_cells for prob in probs)
+    best = b.tune(reps=3)
+    cfg = b.config()
+    out = {'workload': '%d 2-D HDG diffusion problems, %d cells, orders %s' % (
+        count, cells, sorted({sp['p'] for sp in specs})), 'u_dofs': dofs, 'tuned': cfg, 'tune_best_kernel_ms': best}
+    for name, what in (('kernel_only', 0), ('eager', 1), ('graph', 2), ('one_by_one', 3)):
+        ms = float(np.median(b.time(what, warmup=1, reps=5)))
+        out[name] = {'ms': ms, 'dof_per_s': dofs / (ms * 1e-3)}
+        print('# batched %s: %.3f ms' % (name, ms), file=sys.stderr, flush=True)
+    out['graph_vs_one_by_one'] = out['one_by_one']['ms'] / out['graph']['ms']
+    return out
+
+
 def copy_calibration(mib):
     """Copy `mib` MiB device-to-device 5 times with torch (one plain copy
     kernel per call): the PMC reading of FETCH_SIZE / WRITE_SIZE for these
@@ -104,6 +220,9 @@ def main():
     ap.add_argument('--traffic', type=float, nargs=2, metavar=('FETCH_KIB', 'WRITE_KIB'), default=None,
                     help='HBM KiB per element-kernel dispatch from rocprofv3 --pmc runs, already scaled')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MiB')
+    ap.add_argument('--no-learned', action='store_true')
+    ap.add_argument('--no-batch', action='store_true')
+    ap.add_argument('--learned-n', type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on exactly one GPU (multi-GPU sharding is designed, not built); '
@@ -170,16 +289,11 @@ def main():
         'event_ms_per_step': float(np.median(steps_ms)),
     }
     if not args.no_solve:
-        t2 = time.perf_counter()
-        its, reason, relres = prob.solve(pc=hf.PC_BJACOBI, rtol=1e-10, maxits=20000, restart=30)
-        solve_s = time.perf_counter() - t2
-        res['trace_solve'] = {'pc': 'block-jacobi', 'method': 'FGMRES(30) FP64', 'iters': its, 'reason': reason,
-                              'true_relres': relres, 'ms': solve_s * 1e3}
-        u, q, lam = prob.solution()
-        a = w.mesh.arrays()
-        ue = pr.u_c3(a['coords'][a['cells']])
-        res['trace_solve']['nodal_rel_err_vs_exact'] = float(np.linalg.norm(u - ue) / np.linalg.norm(ue))
-        print('# solve: %d its, relres %.2e, %.1f ms' % (its, relres, solve_s * 1e3), file=sys.stderr, flush=True)
+        res['trace_solve'] = trace_solves(hf, pr, prob, w)
+        if not args.no_learned:
+            res['learned_vs_plain'] = learned_vs_plain(hf, pr, args.learned_n)
+    if not args.no_batch:
+        res['batched'] = batched(hf, pr)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         res['cpu_baseline'] = cpu_baseline(args.cpu_n, args.p, threads)

@@ -6,7 +6,7 @@ for the tests and the benchmark; it has no CPU fallback: importing
 ``hyperfox_amd.lib`` raises if the shared library has not been built.
 """
 from ._lib import lib, LIB_PATH, HfxError  # noqa: F401
-from .hdg import Mesh, HDGProblem, refel_sizes, refel_tables, cubature  # noqa: F401
+from .hdg import Mesh, HDGProblem, Batch, refel_sizes, refel_tables, cubature  # noqa: F401
 from .linalg import HipLinAlgebra, HipOpts  # noqa: F401
 
 MODEL_LAPLACE = 0

@@ -74,6 +74,15 @@ _SIGS = {
     'hfx_la_get_rhs': ([_P, _I, _P, _P], _I),
     'hfx_la_nnz': ([_P, _P], _I),
     'hfx_la_get_csr': ([_P] * 5, _I),
+    'hfx_batch_create': ([_I, _P], _I),
+    'hfx_batch_free': ([_P], None),
+    'hfx_batch_add': ([_P, _P], _I),
+    'hfx_batch_size': ([_P, _P, _P], _I),
+    'hfx_batch_set_config': ([_P, _I, _I], _I),
+    'hfx_batch_get_config': ([_P, _P], _I),
+    'hfx_batch_tune': ([_P, _I, _P], _I),
+    'hfx_batch_assemble': ([_P, _I], _I),
+    'hfx_batch_time': ([_P, _I, _I, _I, _P], _I),
     'hfx_lpc_create': ([_I, _I, _P], _I),
     'hfx_lpc_free': ([_P], None),
     'hfx_lpc_num_params': ([_P, _P], _I),

@@ -110,16 +110,18 @@ __device__ inline int wave_argmax(double v, bool valid) {
   return 63 - (int)(key & 63u);
 }
 
+// Cells e = e_first + wave, e_first + wave + e_stride, ... below e_end of one
+// problem, one wavefront per cell: loads the reference tables into LDS
+// (`sm`, s_pos, s_fn), then assembles and condenses each cell.  Shared by
+// the per-problem kernel and the batched kernel.
 template <int D, int P, bool TD, int W>
-__global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
+__device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int e_end, int e_stride, double* sm,
+                                             int* s_pos, int* s_fn) {
   using L = ALay<D, P, TD>;
   using RC = RefConst<D, P>;
   constexpr int NN = L::NN, NNF = L::NNF, NIP = L::NIP, NIPF = L::NIPF, NF = L::NF, NL = L::NL, NS = L::NS;
   constexpr int NCOL = L::NCOL, ND = L::ND;
   static_assert(!TD, "tensor diffusion runs on the general element kernel");
-  extern __shared__ double sm[];
-  __shared__ int s_pos[NF * NN];
-  __shared__ int s_fn[NF * NNF];
 
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -154,8 +156,7 @@ __global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
   double* FWT = ws + L::O_FWT;
   double* RB = ws + L::O_RB;
 
-  const int e_end = a.el_offset + a.nEl;
-  for (int e = a.el_offset + blockIdx.x * W + wv; e < e_end; e += gridDim.x * W) {
+  for (int e = e_first + wv; e < e_end; e += e_stride) {
     const int* cell = a.cells + (size_t)e * NN;
     // ------------------------------------------------------------ gather
     for (int t = lane; t < (D + 1) * D; t += 64) VX[t] = a.coords[(size_t)cell[t / D] * D + t % D];
@@ -541,6 +542,38 @@ __global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
 }
 
 template <int D, int P, bool TD, int W>
+__global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
+  using L = ALay<D, P, TD>;
+  extern __shared__ double sm[];
+  __shared__ int s_pos[L::NF * L::NN];
+  __shared__ int s_fn[L::NF * L::NNF];
+  affine_cells<D, P, TD, W>(a, a.el_offset + blockIdx.x * W, a.el_offset + a.nEl, gridDim.x * W, sm, s_pos, s_fn);
+}
+
+// One launch over many 2-D problems of mixed order: workgroup b takes the
+// items b, b + gridDim.x, ... of a cost-sorted list; an item is a range of
+// cells of one problem, all of one order.
+template <int W>
+__global__ __launch_bounds__(64 * W) void hdg_affine_batch_kernel(const ElemArgs* __restrict__ probs,
+                                                                  const BatchItem* __restrict__ items, int nitems) {
+  extern __shared__ double sm[];
+  __shared__ int s_pos[3 * 21];
+  __shared__ int s_fn[3 * 6];
+  for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const BatchItem w = items[it];
+    const ElemArgs a = probs[w.prob];
+    switch (w.order) {
+      case 1: affine_cells<2, 1, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
+      case 2: affine_cells<2, 2, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
+      case 3: affine_cells<2, 3, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
+      case 4: affine_cells<2, 4, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
+      default: affine_cells<2, 5, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
+    }
+    __syncthreads();  // the next item reloads the tables
+  }
+}
+
+template <int D, int P, bool TD, int W>
 void launch_aff(const ElemArgs& a, int grid_cap, hipStream_t s) {
   using L = ALay<D, P, TD>;
   const size_t lds = (size_t)(L::TAB + W * L::PER) * sizeof(double);
@@ -590,6 +623,50 @@ void launch_affine(int dim, int order, bool tensor_d, int waves, const ElemArgs&
   HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3)
 #undef HFX_CASE
   throw std::runtime_error("affine element kernel: (dim, order) not supported");
+}
+
+}  // namespace hfx
+
+namespace hfx {
+
+size_t affine_batch_lds_bytes(int waves) {
+  size_t m = 0;
+  m = std::max(m, aff_lds<2, 1>(false, waves));
+  m = std::max(m, aff_lds<2, 2>(false, waves));
+  m = std::max(m, aff_lds<2, 3>(false, waves));
+  m = std::max(m, aff_lds<2, 4>(false, waves));
+  m = std::max(m, aff_lds<2, 5>(false, waves));
+  return m;
+}
+
+void launch_affine_batch(int waves, const ElemArgs* probs, const BatchItem* items, int nitems, int grid,
+                         hipStream_t s, bool attribute_only) {
+  if (nitems <= 0) return;
+  const size_t lds = affine_batch_lds_bytes(waves);
+  if (lds > 160 * 1024) throw std::runtime_error("batched element kernel: LDS exceeds 160 KB for this wave count");
+  if (grid <= 0) throw std::runtime_error("batched element kernel: empty grid");
+  auto go = [&](auto kern) {
+    // the LDS limit is raised once, outside any stream capture (see prepare_affine_batch)
+    static bool attr_set = false;
+    if (!attr_set) {
+      HFX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      attr_set = true;
+    }
+    if (attribute_only) return;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * waves), lds, s, probs, items, nitems);
+    HFX_HIP(hipGetLastError());
+  };
+  switch (waves) {
+    case 1: go(hdg_affine_batch_kernel<1>); break;
+    case 2: go(hdg_affine_batch_kernel<2>); break;
+    case 4: go(hdg_affine_batch_kernel<4>); break;
+    default: throw std::runtime_error("batched element kernel: waves per workgroup must be 1, 2 or 4");
+  }
+}
+
+void prepare_affine_batch(int waves) {
+  const BatchItem* none = nullptr;
+  launch_affine_batch(waves, nullptr, none, 1, 1, nullptr, true);
 }
 
 }  // namespace hfx

@@ -1,0 +1,595 @@
+// Fused HDG assembly + static condensation for affine (straight-sided)
+// simplices, FP64, one wavefront per element.
+//
+// With a constant Jacobian J the mass and gradient blocks are reference
+// matrices scaled by det J and J^-1 (Operator.cpp:14-84 evaluated once per
+// element), so Sqq^-1 [Squ | Sql] needs no per-element factorisation:
+//   A_k = sum_a Jinv[k][a] Ahat_a,          Ahat_a = Mhat^-1 Chat_a
+//   B_k[:, (g,b)] = -(|f_g| / det J) n_gk G_g[:, b],  G_g = Mhat^-1[:, fn_g] Mfhat
+// (HDGBase.cpp:106-157 with Mass.cpp:5-38 on flat faces).  The remaining
+// element work is the variable-coefficient integrals at the cubature points
+// (HDGDiffusion.cpp:31-145, HDGConvection.cpp:31-104, Source.cpp:5-48), one
+// LU of K = Suu - sum_k Suq_k A_k, and the trace-column solves of
+// HDGSolver.cpp:331-348:
+//   U = -K^-1 (Sul - sum_k Suq_k B_k), U0 = K^-1 F_u, Q_k = -A_k U - B_k,
+//   S = Slu U + sum_k Slq_k Q_k + Sll, S0 = -Slu U0 - sum_k Slq_k Q0_k.
+//
+// Work mapping on the 64-wide wavefront: the element-integral phases spread
+// matrix entries over the lanes; the condensation gives each lane one trace
+// column c (plus one lane for the F_u column) and keeps that column of U and
+// Q in registers, reading the shared factors (LU of K, Ahat, face masses)
+// with uniform, broadcast LDS addresses.  Waves never wait on each other:
+// the only workgroup barrier is the one after the reference tables are staged.
+#include "hdg_element.h"
+#include "hfx_internal.h"
+
+namespace hfx {
+
+template <int D, int P>
+struct RefConst;
+#include "data/refel_const.inc"
+
+namespace {
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// value of v in lane l (l uniform across the wave)
+__device__ inline double readlane_d(double v, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+  return __hiloint2double(hi, lo);
+}
+
+template <int D, int P, bool TD>
+struct ALay {
+  using Z = ESz<D, P>;
+  static constexpr int NN = Z::NN, NNF = Z::NNF, NIP = Z::NIP, NIPF = Z::NIPF, NF = Z::NF, NL = Z::NL;
+  static constexpr int ND = TD ? D * D : 1;      // stored diffusion components per node
+  static constexpr int NS = 2 + (TD ? D : 1);    // face weights: tau, b.n, D (or (D n)_k)
+  static constexpr int NCOL = NL + 1;            // trace columns + the F_u column
+  // shared reference tables (packed by the host in this order)
+  static constexpr int T_PHI = 0;
+  static constexpr int T_DPHI = T_PHI + NIP * NN;
+  static constexpr int T_FPHI = T_DPHI + NIP * NN * D;
+  static constexpr int T_W = T_FPHI + NIPF * NNF;
+  static constexpr int T_FW = T_W + NIP;
+  static constexpr int T_AH = T_FW + NIPF;       // Ahat and G stay in global memory
+  static constexpr int T_G = T_AH + D * NN * NN;
+  static constexpr int TAB_ALL = T_G + NF * NN * NNF;
+  static constexpr int TAB = T_AH;                 // staged in LDS
+  // per-wave scratch
+  static constexpr int NSF = NS;                  // LDS face masses: tau, b.n, D
+  static constexpr int O_FM = 0;
+  static constexpr int O_SUQ = O_FM + NF * NSF * NNF * NNF;
+  static constexpr int O_KM = O_SUQ + D * NN * NN;  // Suu, then K^-1
+  // after the K^-1 solve, Suq and K are dead: the face traces n.q of the Q
+  // columns take their place
+  static constexpr int NQN = NF * NNF * NCOL;
+  static constexpr int END_SK = O_KM + NN * NN;
+  static constexpr int O_FU = (O_SUQ + NQN > END_SK) ? O_SUQ + NQN : END_SK;
+  static constexpr int O_FG = O_FU + NN;          // per face: n (D), |f|
+  static constexpr int O_VX = O_FG + NF * (D + 1);
+  static constexpr int O_FWT = O_VX + (D + 1) * D;  // face-IP weights (kept for the b.n columns)
+  static constexpr int O_RD = O_FWT + NF * NIPF * NS;  // 1 / pivot of each K^-1 row
+  static constexpr int O_OV = O_RD + NN;
+  // overlay, phase 1
+  static constexpr int O_DN = O_OV;
+  static constexpr int O_BN = O_DN + NN * ND;
+  static constexpr int O_TAU = O_BN + NN * D;
+  static constexpr int O_FQ = O_TAU + NF * NNF;
+  static constexpr int O_RQ = O_FQ + NIP;
+  static constexpr int O_EJ = O_RQ + NIP;
+  static constexpr int O_BJ = O_EJ + NIP * D * D;
+  static constexpr int O_WR = O_BJ + NIP * D;
+  static constexpr int O_WF = O_WR + NIP;
+  static constexpr int END1 = O_WF + NIP;
+  // overlay, phase 2: R columns, overwritten in place by the U columns
+  static constexpr int O_RB = O_OV;
+  static constexpr int END2 = O_RB + NN * NCOL;
+  static constexpr int PER = END1 > END2 ? END1 : END2;
+};
+
+// wave-wide argmax of a non-negative value; ties go to the lowest lane.  The
+// key keeps the top 26 bits of |v| as a float and the lane in the low 6 bits.
+__device__ inline int wave_argmax(double v, bool valid) {
+  unsigned key = 0u;
+  if (valid) {
+    float fv = (float)v;
+    if (!(fv >= 0.0f)) fv = 0.0f;  // NaN
+    key = (__float_as_uint(fv) & ~63u) | (63u - (unsigned)(threadIdx.x & 63));
+    key += 1u << 6;  // valid lanes beat invalid ones even when |v| = 0
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned o = (unsigned)__shfl_xor((int)key, off, 64);
+    key = o > key ? o : key;
+  }
+  return 63 - (int)(key & 63u);
+}
+
+template <int D, int P, bool TD, int W>
+__global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
+  using L = ALay<D, P, TD>;
+  using RC = RefConst<D, P>;
+  constexpr int NN = L::NN, NNF = L::NNF, NIP = L::NIP, NIPF = L::NIPF, NF = L::NF, NL = L::NL, NS = L::NS;
+  constexpr int NCOL = L::NCOL, ND = L::ND;
+  static_assert(!TD, "tensor diffusion runs on the general element kernel");
+  extern __shared__ double sm[];
+  __shared__ int s_pos[NF * NN];
+  __shared__ int s_fn[NF * NNF];
+
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  for (int t = threadIdx.x; t < L::TAB; t += 64 * W) sm[t] = a.tabs[t];
+  for (int t = threadIdx.x; t < NF * NN; t += 64 * W) s_pos[t] = RC::POS[t / NN][t % NN];
+  for (int t = threadIdx.x; t < NF * NNF; t += 64 * W) s_fn[t] = RC::FN[t / NNF][t % NNF];
+  __syncthreads();
+
+  const double* PHI = sm + L::T_PHI;
+  const double* DPHI = sm + L::T_DPHI;
+  const double* FPHI = sm + L::T_FPHI;
+  const double* WQ = sm + L::T_W;
+  const double* FWQ = sm + L::T_FW;
+  const double* __restrict__ AH = a.tabs + L::T_AH;  // global, read with uniform addresses
+  const double* __restrict__ GT = a.tabs + L::T_G;
+  double* ws = sm + L::TAB + wv * L::PER;
+  double* FM = ws + L::O_FM;
+  double* SUQ = ws + L::O_SUQ;
+  double* KM = ws + L::O_KM;
+  double* FU = ws + L::O_FU;
+  double* FG = ws + L::O_FG;
+  double* VX = ws + L::O_VX;
+  double* DN = ws + L::O_DN;
+  double* BN = ws + L::O_BN;
+  double* TAU = ws + L::O_TAU;
+  double* FQ = ws + L::O_FQ;
+  double* RQ = ws + L::O_RQ;
+  double* EJ = ws + L::O_EJ;
+  double* BJ = ws + L::O_BJ;
+  double* WR = ws + L::O_WR;
+  double* WF = ws + L::O_WF;
+  double* FWT = ws + L::O_FWT;
+  double* RB = ws + L::O_RB;
+
+  const int e_end = a.el_offset + a.nEl;
+  for (int e = a.el_offset + blockIdx.x * W + wv; e < e_end; e += gridDim.x * W) {
+    const int* cell = a.cells + (size_t)e * NN;
+    // ------------------------------------------------------------ gather
+    for (int t = lane; t < (D + 1) * D; t += 64) VX[t] = a.coords[(size_t)cell[t / D] * D + t % D];
+    for (int t = lane; t < NN * ND; t += 64) {
+      const int i = t / ND, c = t % ND;
+      double v;
+      (void)c;
+      v = a.ndD == 1 ? a.Dn[cell[i]] : (a.useDiff ? 1.0 : 0.0);
+      DN[t] = v;
+    }
+    for (int t = lane; t < NN * D; t += 64) BN[t] = a.hasB ? a.bn[(size_t)cell[t / D] * D + t % D] : 0.0;
+    for (int t = lane; t < NF * NNF; t += 64) {
+      const int f = t / NNF, j = t % NNF;
+      const int fc = a.cell2face[(size_t)e * NF + f];
+      const int side = (a.tau_sides == 2 && a.face2cell[2 * (size_t)fc] != e) ? 1 : 0;
+      const int p = a.face_perm[((size_t)e * NF + f) * NNF + j];
+      TAU[t] = a.tau[((size_t)fc * NNF + p) * a.tau_sides + side];
+    }
+    for (int t = lane; t < NIP; t += 64) {
+      FQ[t] = (a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + t] : 0.0;
+      RQ[t] = a.hasR ? a.r_ip[(size_t)e * NIP + t] : 0.0;
+    }
+    wave_sync();
+    // ------------------------------------------------ element geometry
+    // J[r][c] = dx_c/dxi_r (transposed convention of Operator.cpp:31-36)
+    double J[D * D], JI[D * D], det;
+    for (int r = 0; r < D; ++r)
+      for (int c = 0; c < D; ++c) J[r * D + c] = 0.5 * (VX[(r + 1) * D + c] - VX[c]);
+    if constexpr (D == 2) {
+      det = J[0] * J[3] - J[1] * J[2];
+      const double r = 1.0 / det;
+      JI[0] = J[3] * r; JI[1] = -J[1] * r; JI[2] = -J[2] * r; JI[3] = J[0] * r;
+    } else {
+      det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) + J[2] * (J[3] * J[7] - J[4] * J[6]);
+      const double r = 1.0 / det;
+      JI[0] = (J[4] * J[8] - J[5] * J[7]) * r; JI[1] = (J[2] * J[7] - J[1] * J[8]) * r; JI[2] = (J[1] * J[5] - J[2] * J[4]) * r;
+      JI[3] = (J[5] * J[6] - J[3] * J[8]) * r; JI[4] = (J[0] * J[8] - J[2] * J[6]) * r; JI[5] = (J[2] * J[3] - J[0] * J[5]) * r;
+      JI[6] = (J[3] * J[7] - J[4] * J[6]) * r; JI[7] = (J[1] * J[6] - J[0] * J[7]) * r; JI[8] = (J[0] * J[4] - J[1] * J[3]) * r;
+    }
+    if (lane < NF) {
+      // face measure and outward unit normal (HDGBase.cpp:34-65)
+      const int f = lane;
+      const double* x0 = VX + s_fn[f * NNF] * D;
+      double t0[D], t1[D], nv[D], meas;
+      for (int c = 0; c < D; ++c) t0[c] = 0.5 * (VX[s_fn[f * NNF + 1] * D + c] - x0[c]);
+      if constexpr (D == 2) {
+        meas = sqrt(t0[0] * t0[0] + t0[1] * t0[1]);
+        nv[0] = t0[1];
+        nv[1] = -t0[0];
+      } else {
+        for (int c = 0; c < D; ++c) t1[c] = 0.5 * (VX[s_fn[f * NNF + 2] * D + c] - x0[c]);
+        nv[0] = t0[1] * t1[2] - t0[2] * t1[1];
+        nv[1] = t0[2] * t1[0] - t0[0] * t1[2];
+        nv[2] = t0[0] * t1[1] - t0[1] * t1[0];
+        meas = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+      }
+      double nn = 0.0;
+      for (int c = 0; c < D; ++c) nn += nv[c] * nv[c];
+      nn = 1.0 / sqrt(nn);
+      int vo = 0;
+      for (int i = D; i >= 0; --i)
+        if (s_pos[f * NN + i] < 0) vo = i;
+      double prod = 0.0;
+      for (int c = 0; c < D; ++c) {
+        nv[c] *= nn;
+        prod += (VX[vo * D + c] - x0[c]) * nv[c];
+      }
+      const double sg = prod > 0 ? -1.0 : 1.0;
+      for (int c = 0; c < D; ++c) FG[f * (D + 1) + c] = sg * nv[c];
+      FG[f * (D + 1) + D] = meas;
+    }
+    // ------------------------------------------------ cubature-point data
+    for (int q = lane; q < NIP; q += 64) {
+      double dq[TD ? D * D : 1], bq[D];
+      for (int u = 0; u < ND; ++u) dq[u] = 0.0;
+      for (int c = 0; c < D; ++c) bq[c] = 0.0;
+      for (int i = 0; i < NN; ++i) {
+        const double ph = PHI[q * NN + i];
+        for (int u = 0; u < ND; ++u) dq[u] += ph * DN[i * ND + u];
+        for (int c = 0; c < D; ++c) bq[c] += ph * BN[i * D + c];
+      }
+      const double wd = WQ[q] * det;
+      for (int k = 0; k < D; ++k)
+        for (int r = 0; r < D; ++r) {
+          double s;
+          if constexpr (TD) {
+            s = 0.0;
+            for (int l = 0; l < D; ++l) s += dq[k * D + l] * JI[l * D + r];
+          } else {
+            s = dq[0] * JI[k * D + r];
+          }
+          EJ[(q * D + k) * D + r] = wd * s;
+        }
+      for (int r = 0; r < D; ++r) {
+        double s = 0.0;
+        for (int k = 0; k < D; ++k) s += bq[k] * JI[k * D + r];
+        BJ[q * D + r] = wd * s;
+      }
+      WR[q] = wd * RQ[q];
+      WF[q] = wd * FQ[q];
+    }
+    wave_sync();
+    for (int t = lane; t < NF * NIPF; t += 64) {
+      const int f = t / NIPF, q = t % NIPF;
+      double tq = 0.0, bq[D], dq[TD ? D * D : 1];
+      for (int c = 0; c < D; ++c) bq[c] = 0.0;
+      for (int u = 0; u < ND; ++u) dq[u] = 0.0;
+      for (int j = 0; j < NNF; ++j) {
+        const double ps = FPHI[q * NNF + j];
+        const int node = s_fn[f * NNF + j];
+        tq += ps * TAU[f * NNF + j];
+        for (int c = 0; c < D; ++c) bq[c] += ps * BN[node * D + c];
+        for (int u = 0; u < ND; ++u) dq[u] += ps * DN[node * ND + u];
+      }
+      const double* nf = FG + f * (D + 1);
+      const double ds = FWQ[q] * nf[D];
+      double bn = 0.0;
+      for (int c = 0; c < D; ++c) bn += bq[c] * nf[c];
+      double* w = FWT + t * NS;
+      w[0] = ds * tq;
+      w[1] = ds * bn;
+      if constexpr (TD) {
+        for (int k = 0; k < D; ++k) {
+          double s = 0.0;
+          for (int c = 0; c < D; ++c) s += dq[k * D + c] * nf[c];
+          w[2 + k] = ds * s;
+        }
+      } else {
+        w[2] = ds * dq[0];
+      }
+    }
+    wave_sync();
+    // weighted face masses FM[f][s][a][b] = sum_q w_s psi_a psi_b, s = 0: tau, 1: b.n, 2: D
+    constexpr int NSF = L::NSF;
+    for (int t = lane; t < NF * NSF * NNF * NNF; t += 64) {
+      const int b = t % NNF, aa = (t / NNF) % NNF, sf = (t / (NNF * NNF)) % NSF, f = t / (NSF * NNF * NNF);
+      double acc = 0.0;
+      for (int q = 0; q < NIPF; ++q) acc += FWT[(f * NIPF + q) * NS + sf] * FPHI[q * NNF + aa] * FPHI[q * NNF + b];
+      FM[t] = acc;
+    }
+    for (int i = lane; i < NN; i += 64) {
+      double s = 0.0;
+      for (int q = 0; q < NIP; ++q) s += WF[q] * PHI[q * NN + i];
+      FU[i] = s;
+    }
+    wave_sync();
+    // ----------------------------------------- Suq_k and Suu (entries)
+    {
+      constexpr int RV = (NN * NN + 63) / 64;
+      double aq[RV][D], au[RV];
+      for (int r = 0; r < RV; ++r) {
+        au[r] = 0.0;
+        for (int k = 0; k < D; ++k) aq[r][k] = 0.0;
+      }
+#pragma unroll 1
+      for (int q = 0; q < NIP; ++q) {
+        double ej[D * D], bj[D];
+        for (int u = 0; u < D * D; ++u) ej[u] = EJ[q * D * D + u];
+        for (int c = 0; c < D; ++c) bj[c] = BJ[q * D + c];
+        const double wr = WR[q];
+#pragma unroll
+        for (int r = 0; r < RV; ++r) {
+          const int t = lane + 64 * r;
+          if (t < NN * NN) {
+            const int i = t / NN, j = t % NN;
+            double g[D];
+            for (int c = 0; c < D; ++c) g[c] = DPHI[(q * NN + i) * D + c];
+            const double pj = PHI[q * NN + j];
+            double bg = 0.0;
+            for (int k = 0; k < D; ++k) {
+              double s = 0.0;
+              for (int c = 0; c < D; ++c) s += ej[k * D + c] * g[c];
+              aq[r][k] += s * pj;
+              bg += bj[k] * g[k];
+            }
+            au[r] += (wr * PHI[q * NN + i] - bg) * pj;
+          }
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < RV; ++r) {
+        const int t = lane + 64 * r;
+        if (t < NN * NN) {
+          const int i = t / NN, j = t % NN;
+          for (int f = 0; f < NF; ++f) {
+            const int ai = s_pos[f * NN + i], bj = s_pos[f * NN + j];
+            if (ai >= 0 && bj >= 0) {
+              const double* F0 = FM + (f * NSF) * NNF * NNF + ai * NNF + bj;
+              au[r] += F0[0];
+              for (int k = 0; k < D; ++k) aq[r][k] -= FG[f * (D + 1) + k] * F0[2 * NNF * NNF];
+            }
+          }
+          // store Suq transformed to the reference frame: SUQ_a = sum_k Jinv[k][a] Suq_k
+          for (int r2 = 0; r2 < D; ++r2) {
+            double s = 0.0;
+            for (int k = 0; k < D; ++k) s += JI[k * D + r2] * aq[r][k];
+            SUQ[(r2 * NN + i) * NN + j] = s;
+          }
+          KM[t] = au[r];
+        }
+      }
+    }
+    wave_sync();
+    // K = Suu - sum_k Suq_k A_k = Suu - sum_a SUQ_a Ahat_a, one row per lane
+    // in registers; then K^-1 by Gauss-Jordan with partial pivoting on the
+    // register rows (the pivot row is fetched with readlane, no LDS traffic)
+    {
+      double kr[NN], ir[NN];
+      const int rl = lane < NN ? lane : 0;
+#pragma unroll
+      for (int j = 0; j < NN; ++j) {
+        kr[j] = KM[rl * NN + j];
+        ir[j] = (j == lane) ? 1.0 : 0.0;
+      }
+      for (int r2 = 0; r2 < D; ++r2) {
+#pragma unroll 2
+        for (int l = 0; l < NN; ++l) {
+          const double sv = SUQ[(r2 * NN + rl) * NN + l];
+          const double* arow = AH + (r2 * NN + l) * NN;
+#pragma unroll
+          for (int j = 0; j < NN; ++j) kr[j] -= sv * arow[j];
+        }
+      }
+      unsigned long long used = 0ull;
+      double mypiv = 1.0;
+      int myk = 0;
+#pragma unroll
+      for (int k = 0; k < NN; ++k) {
+        const int p = __builtin_amdgcn_readfirstlane(
+            wave_argmax(fabs(kr[k]), lane < NN && !((used >> lane) & 1ull)));
+        used |= 1ull << p;
+        if (lane == p) { mypiv = kr[k]; myk = k; }
+        const double fac = kr[k] / readlane_d(kr[k], p);
+#pragma unroll
+        for (int j = k + 1; j < NN; ++j) {
+          const double pj = readlane_d(kr[j], p);
+          if (lane != p) kr[j] -= fac * pj;
+        }
+#pragma unroll
+        for (int j = 0; j < NN; ++j) {
+          const double pj = readlane_d(ir[j], p);
+          if (lane != p) ir[j] -= fac * pj;
+        }
+      }
+      // lane holding variable myk writes row myk of K^-1
+      if (lane < NN) {
+        const double r = 1.0 / mypiv;
+#pragma unroll
+        for (int j = 0; j < NN; ++j) KM[myk * NN + j] = ir[j] * r;
+      }
+    }
+    wave_sync();
+    // ---------------------------------------------- trace-column phase
+    // lane c < NL owns trace column c = (g, bp); lane NL owns the F_u column.
+    // Column vectors live in registers (compile-time row index) or in
+    // lane-contiguous LDS rows [row][NCOL] when read at face-node positions.
+    const int c = lane;
+    const bool tr = c < NL;
+    const int g = tr ? c / NNF : 0, bp = tr ? c % NNF : 0;
+    double ng[D];
+    for (int k = 0; k < D; ++k) ng[k] = FG[g * (D + 1) + k];
+    const double rho = FG[g * (D + 1) + D] / det;
+    double* UB = RB;               // U columns replace R columns lane by lane
+    double* QN = ws + L::O_SUQ;    // face traces of n.q, over the dead Suq and K
+    // R[:, c] = Sul[:, c] - sum_k Suq_k B_k[:, c] (or -F_u on lane NL), with
+    // B_k[:, (g,b)] = -rho_g n_gk G_g[:, b] and sum_k n_gk Suq_k = sum_a nt_a SUQ_a
+    if (c <= NL) {
+      for (int i = 0; i < NN; ++i) RB[i * NCOL + c] = tr ? 0.0 : -FU[i];
+      if (tr) {
+#pragma unroll
+        for (int a2 = 0; a2 < NNF; ++a2)
+          RB[s_fn[g * NNF + a2] * NCOL + c] =
+              -FM[((g * NSF + 0) * NNF + a2) * NNF + bp] + FM[((g * NSF + 1) * NNF + a2) * NNF + bp];
+        double nt[D];
+        for (int r2 = 0; r2 < D; ++r2) {
+          double s = 0.0;
+          for (int k = 0; k < D; ++k) s += J[r2 * D + k] * ng[k];
+          nt[r2] = rho * s;
+        }
+        double gcol[NN];
+#pragma unroll
+        for (int j = 0; j < NN; ++j) gcol[j] = GT[((size_t)g * NN + j) * NNF + bp];
+#pragma unroll 2
+        for (int i = 0; i < NN; ++i) {
+          double s = 0.0;
+          for (int r2 = 0; r2 < D; ++r2) {
+            const double* srow = SUQ + (r2 * NN + i) * NN;
+            double t2 = 0.0;
+#pragma unroll
+            for (int j = 0; j < NN; ++j) t2 += srow[j] * gcol[j];
+            s += nt[r2] * t2;
+          }
+          RB[i * NCOL + c] += s;
+        }
+      }
+    }
+    wave_sync();
+    if (c <= NL) {
+      // u = -K^-1 R[:, c]
+      double rr[NN], u[NN];  // R column read in full before U overwrites it
+#pragma unroll
+      for (int j = 0; j < NN; ++j) rr[j] = RB[j * NCOL + c];
+#pragma unroll
+      for (int i = 0; i < NN; ++i) {
+        const double* krow = KM + i * NN;
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NN; ++j) s += krow[j] * rr[j];
+        u[i] = -s;
+        UB[i * NCOL + c] = u[i];
+      }
+    }
+    wave_sync();  // every lane is done with Suq and K^-1 before QN overwrites them
+    if (c <= NL) {
+      double u[NN];
+#pragma unroll
+      for (int j = 0; j < NN; ++j) u[j] = UB[j * NCOL + c];
+      // Q_k = -A_k u - B_k[:, c]; write U, Q (or U0, Q0); keep n_f . q on faces
+      double* Qe = tr ? a.Q + (size_t)e * NN * D * NL : a.Q0 + (size_t)e * NN * D;
+      double* Ue = tr ? a.U + (size_t)e * NN * NL : a.U0 + (size_t)e * NN;
+#pragma unroll 2
+      for (int i = 0; i < NN; ++i) {
+        double y[D];
+        for (int r2 = 0; r2 < D; ++r2) {
+          const double* arow = AH + (r2 * NN + i) * NN;
+          double s = 0.0;
+#pragma unroll
+          for (int j = 0; j < NN; ++j) s += arow[j] * u[j];
+          y[r2] = s;
+        }
+        const double gb = tr ? rho * GT[((size_t)g * NN + i) * NNF + bp] : 0.0;
+        double qk[D];
+        for (int k = 0; k < D; ++k) {
+          double s = 0.0;
+          for (int r2 = 0; r2 < D; ++r2) s -= JI[k * D + r2] * y[r2];
+          qk[k] = s + ng[k] * gb;
+        }
+        if (tr) {
+          Ue[(size_t)i * NL + c] = UB[i * NCOL + c];
+          for (int k = 0; k < D; ++k) Qe[(size_t)(i * D + k) * NL + c] = qk[k];
+        } else {
+          Ue[i] = UB[i * NCOL + c];
+          for (int k = 0; k < D; ++k) Qe[i * D + k] = qk[k];
+        }
+        for (int f = 0; f < NF; ++f) {
+          const int pf = s_pos[f * NN + i];
+          if (pf >= 0) {
+            double s = 0.0;
+            for (int k = 0; k < D; ++k) s += FG[f * (D + 1) + k] * qk[k];
+            QN[(f * NNF + pf) * NCOL + c] = s;
+          }
+        }
+      }
+      // S[:, c] = Slu u + sum_k Slq_k q_k (+ Sll), or S0 = -(Slu u0 + Slq q0)
+      double* Se = tr ? a.S + (size_t)e * NL * NL : a.S0 + (size_t)e * NL;
+#pragma unroll 1
+      for (int f = 0; f < NF; ++f) {
+        double ut[NNF], qt[NNF];
+#pragma unroll
+        for (int b = 0; b < NNF; ++b) {
+          ut[b] = UB[s_fn[f * NNF + b] * NCOL + c];
+          qt[b] = QN[(f * NNF + b) * NCOL + c];
+        }
+        const double* fmt = FM + (f * NSF + 0) * NNF * NNF;
+        const double* fmb = FM + (f * NSF + 1) * NNF * NNF;
+        const double* fmd = FM + (f * NSF + 2) * NNF * NNF;
+#pragma unroll
+        for (int aa = 0; aa < NNF; ++aa) {
+          double s = 0.0;
+#pragma unroll
+          for (int b = 0; b < NNF; ++b) s += fmt[aa * NNF + b] * ut[b] - fmd[aa * NNF + b] * qt[b];
+          if (tr) {
+            if (f == g) s += -fmt[aa * NNF + bp] + fmb[aa * NNF + bp];
+            Se[(size_t)(f * NNF + aa) * NL + c] = s;
+          } else {
+            Se[f * NNF + aa] = -s;
+          }
+        }
+      }
+    }
+    wave_sync();
+  }
+}
+
+template <int D, int P, bool TD, int W>
+void launch_aff(const ElemArgs& a, int grid_cap, hipStream_t s) {
+  using L = ALay<D, P, TD>;
+  const size_t lds = (size_t)(L::TAB + W * L::PER) * sizeof(double);
+  auto kern = hdg_affine_kernel<D, P, TD, W>;
+  static bool attr_set = false;
+  if (!attr_set) {
+    HFX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    attr_set = true;
+  }
+  long long grid = ((long long)a.nEl + W - 1) / W;
+  if (grid_cap > 0 && grid > grid_cap) grid = grid_cap;
+  if (grid <= 0) return;
+  hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * W), lds, s, a);
+  HFX_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+template <int D, int P>
+static size_t aff_lds(bool td, int w) {
+  return td ? (size_t)(ALay<D, P, true>::TAB + w * ALay<D, P, true>::PER) * 8
+            : (size_t)(ALay<D, P, false>::TAB + w * ALay<D, P, false>::PER) * 8;
+}
+
+size_t affine_lds_bytes(int dim, int order, bool tensor_d, int waves) {
+#define HFX_CASE(D, P) \
+  if (dim == D && order == P) return aff_lds<D, P>(tensor_d, waves);
+  HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
+  HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3)
+#undef HFX_CASE
+  throw std::runtime_error("affine element kernel: (dim, order) not supported");
+}
+
+void launch_affine(int dim, int order, bool tensor_d, int waves, const ElemArgs& a, int grid_cap, hipStream_t s) {
+  if (tensor_d) throw std::runtime_error("affine element kernel: tensor diffusion uses the general kernel");
+#define HFX_CASE(D, P)                                                   \
+  if (dim == D && order == P) {                                          \
+    switch (waves) {                                                     \
+      case 1: launch_aff<D, P, false, 1>(a, grid_cap, s); return;       \
+      case 2: launch_aff<D, P, false, 2>(a, grid_cap, s); return;       \
+      case 3: launch_aff<D, P, false, 3>(a, grid_cap, s); return;       \
+      case 4: launch_aff<D, P, false, 4>(a, grid_cap, s); return;       \
+      default: throw std::runtime_error("affine element kernel: waves per workgroup must be 1 to 4"); \
+    }                                                                    \
+  }
+  HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
+  HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3)
+#undef HFX_CASE
+  throw std::runtime_error("affine element kernel: (dim, order) not supported");
+}
+
+}  // namespace hfx

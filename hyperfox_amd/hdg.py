@@ -210,3 +210,48 @@ class HDGProblem:
         ms = np.zeros(reps)
         check(lib.hfx_hdg_time(self._h, what, warmup, reps, _ptr(ms)))
         return ms
+
+
+class Batch:
+    """Many 2-D problems assembled by one persistent launch (hfx_batch_*)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib.hfx_batch_create(device, ctypes.byref(h)))
+        self._h = h
+        self.problems = []
+
+    def __del__(self):
+        if getattr(self, '_h', None):
+            lib.hfx_batch_free(self._h)
+            self._h = None
+
+    def add(self, prob):
+        check(lib.hfx_batch_add(self._h, prob._h))
+        self.problems.append(prob)  # the batch holds raw handles: keep them alive
+
+    def size(self):
+        n, c = ctypes.c_int(), ctypes.c_longlong()
+        check(lib.hfx_batch_size(self._h, ctypes.byref(n), ctypes.byref(c)))
+        return n.value, c.value
+
+    def set_config(self, waves, tile):
+        check(lib.hfx_batch_set_config(self._h, waves, tile))
+
+    def config(self):
+        out = np.zeros(4, dtype=np.int32)
+        check(lib.hfx_batch_get_config(self._h, _ptr(out)))
+        return dict(waves=int(out[0]), tile=int(out[1]), items=int(out[2]), workgroups=int(out[3]))
+
+    def tune(self, reps=3):
+        v = ctypes.c_double()
+        check(lib.hfx_batch_tune(self._h, reps, ctypes.byref(v)))
+        return v.value
+
+    def assemble(self, use_graph=False):
+        check(lib.hfx_batch_assemble(self._h, 1 if use_graph else 0))
+
+    def time(self, what, warmup=1, reps=3):
+        ms = np.zeros(reps)
+        check(lib.hfx_batch_time(self._h, what, warmup, reps, _ptr(ms)))
+        return ms

@@ -180,3 +180,38 @@ def diffusion_grf_2d(n, p, seed, ell=0.1, sigma=1.0, jitter=0.1, mesh=None, mesh
     return Workload('diffusion-grf-2d-n%d-p%d-s%d' % (n, p, seed), 2, p, m, 1, tau, 1, D=k, ndD=1,
                     dirichlet=np.zeros((m.n_faces, m.sizes['nNf'])), f_fn=lambda xx: np.ones(xx.shape[:-1]),
                     h=1.0 / n)
+
+
+def fourier_source(modes):
+    """f(x, y) = sum_m a_m sin(pi i_m x) sin(pi j_m y)."""
+    def f(x):
+        out = np.zeros(x.shape[:-1])
+        for a, i, j in modes:
+            out += a * np.sin(PI * i * x[..., 0]) * np.sin(PI * j * x[..., 1])
+        return out
+    return f
+
+
+def batch_specs(count=64, seed=7, cells=(1000, 40000), orders=(1, 5), diff=(1e-2, 1e2)):
+    """The batched workload: `count` independent 2-D diffusion problems with
+    cell counts log-uniform in `cells` (n x n x 2 meshes, n = round(sqrt(c/2))),
+    orders uniform in `orders`, scalar diffusivity log-uniform in `diff` and a
+    source of 8 Fourier modes with N(0, 1) amplitudes, all from `seed`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        c = math.exp(rng.uniform(math.log(cells[0]), math.log(cells[1])))
+        n = max(2, int(round(math.sqrt(c / 2.0))))
+        p = int(rng.integers(orders[0], orders[1] + 1))
+        D = math.exp(rng.uniform(math.log(diff[0]), math.log(diff[1])))
+        modes = [(float(rng.standard_normal()), int(rng.integers(1, 5)), int(rng.integers(1, 5))) for _ in range(8)]
+        out.append(dict(n=n, p=p, D=D, modes=modes, mesh_seed=seed * 1000 + k))
+    return out
+
+
+def batch_workload(spec, jitter=0.1):
+    m = Mesh.structured(2, spec['p'], spec['n'], jitter, spec['mesh_seed'])
+    nNf = m.sizes['nNf']
+    return Workload('batch-n%d-p%d' % (spec['n'], spec['p']), 2, spec['p'], m, 1, np.ones((m.n_faces, nNf)), 1,
+                    D=np.full(m.n_nodes, spec['D']), ndD=1, dirichlet=np.zeros((m.n_faces, nNf)),
+                    f_fn=fourier_source(spec['modes']), h=1.0 / spec['n'])

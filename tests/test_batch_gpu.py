@@ -1,0 +1,108 @@
+"""Batched assembly of many 2-D problems of mixed order in one persistent
+launch: every problem's condensed blocks and trace system equal, bit for bit,
+what its own hfx_hdg_assemble produces (the same device code per cell), for
+eager launches, for several replays of the captured HIP graph (with eager
+work in between) and after autotuning."""
+import numpy as np
+import pytest
+
+import hyperfox_amd as hf
+import hyperfox_amd.problems as pr
+
+pytestmark = pytest.mark.gpu
+
+
+def build(specs):
+    probs = []
+    for s in specs:
+        w = pr.batch_workload(s)
+        prob = hf.HDGProblem(2, s['p'])
+        pr.setup_gpu(w, prob)
+        probs.append(prob)
+    return probs
+
+
+def snapshot(probs):
+    out = []
+    for prob in probs:
+        c = prob.condensed()
+        rp, cols, vals, rhs = prob.trace_csr()
+        out.append((c['S'], c['S0'], c['U'], c['Q'], c['U0'], c['Q0'], vals, rhs))
+    return out
+
+
+def same(a, b):
+    return all(np.array_equal(x, y) for pa, pb in zip(a, b) for x, y in zip(pa, pb))
+
+
+@pytest.fixture(scope='module')
+def batch_case():
+    specs = pr.batch_specs(count=10, seed=3, cells=(20, 400))
+    assert {s['p'] for s in specs} >= {1, 2, 3}
+    probs = build(specs)
+    for prob in probs:
+        prob.assemble()
+    ref = snapshot(probs)
+    return specs, probs, ref
+
+
+@pytest.mark.parametrize('waves,tile', [(1, 8), (2, 16), (4, 64)])
+def test_batched_equals_per_problem(batch_case, waves, tile):
+    specs, probs, ref = batch_case
+    b = hf.Batch()
+    for prob in probs:
+        b.add(prob)
+    b.set_config(waves, tile)
+    # poison the outputs so stale values cannot pass
+    for prob in probs:
+        prob.set_kernel(1)
+        prob.assemble()
+        prob.set_kernel(0)
+    b.assemble(use_graph=False)
+    assert same(snapshot(probs), ref)
+
+
+def test_graph_replays_and_tuning(batch_case):
+    specs, probs, ref = batch_case
+    b = hf.Batch()
+    for prob in probs:
+        b.add(prob)
+    b.set_config(2, 32)
+    for rep in range(3):
+        for prob in probs:           # eager work between replays
+            prob.set_kernel(1)
+            prob.assemble()
+            prob.set_kernel(0)
+        b.assemble(use_graph=True)
+        assert same(snapshot(probs), ref), rep
+    best = b.tune(reps=2)
+    cfg = b.config()
+    print('tuned: %s, %.3f ms' % (cfg, best))
+    assert best > 0 and cfg['waves'] in (1, 2, 4)
+    b.assemble(use_graph=True)
+    assert same(snapshot(probs), ref)
+    t = b.time(3, warmup=1, reps=2)
+    assert np.all(t > 0)
+
+
+def test_batch_solves_match_single_problem(batch_case):
+    specs, probs, ref = batch_case
+    b = hf.Batch()
+    for prob in probs[:3]:
+        b.add(prob)
+    b.assemble(use_graph=False)
+    for prob in probs[:3]:
+        prob.set_krylov(hf.KRYLOV_BICGSTAB)
+        its, reason, rel = prob.solve(pc=hf.PC_BJACOBI, rtol=1e-12, maxits=20000)
+        assert reason == 2 and rel <= 1e-12
+
+
+def test_batch_rejects_unsupported_problems():
+    m = hf.Mesh.structured(3, 1, 2, 0.0, 1)
+    prob = hf.HDGProblem(3, 1)
+    prob.set_mesh(m)
+    prob.set_model(hf.MODEL_LAPLACE)
+    prob.set_fields(np.ones((m.n_faces, m.sizes['nNf'])), 1)
+    b = hf.Batch()
+    with pytest.raises(hf.HfxError, match='2-D'):
+        b.add(prob)
