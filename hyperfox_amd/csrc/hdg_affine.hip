@@ -57,10 +57,10 @@ struct ALay {
   static constexpr int T_FPHI = T_DPHI + NIP * NN * D;
   static constexpr int T_W = T_FPHI + NIPF * NNF;
   static constexpr int T_FW = T_W + NIP;
-  static constexpr int T_AH = T_FW + NIPF;       // Ahat and G stay in global memory
+  static constexpr int T_AH = T_FW + NIPF;
   static constexpr int T_G = T_AH + D * NN * NN;
   static constexpr int TAB_ALL = T_G + NF * NN * NNF;
-  static constexpr int TAB = T_AH;                 // staged in LDS
+  static constexpr int TAB = TAB_ALL;              // all staged in LDS, shared by the waves
   // per-wave scratch
   static constexpr int NSF = NS;                  // LDS face masses: tau, b.n, D
   static constexpr int O_FM = 0;
@@ -135,8 +135,8 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
   const double* FPHI = sm + L::T_FPHI;
   const double* WQ = sm + L::T_W;
   const double* FWQ = sm + L::T_FW;
-  const double* __restrict__ AH = a.tabs + L::T_AH;  // global, read with uniform addresses
-  const double* __restrict__ GT = a.tabs + L::T_G;
+  const double* AH = sm + L::T_AH;  // read with uniform (broadcast) addresses
+  const double* GT = sm + L::T_G;
   double* ws = sm + L::TAB + wv * L::PER;
   double* FM = ws + L::O_FM;
   double* SUQ = ws + L::O_SUQ;
