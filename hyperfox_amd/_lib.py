@@ -78,7 +78,7 @@ _SIGS = {
     'hfx_batch_free': ([_P], None),
     'hfx_batch_add': ([_P, _P], _I),
     'hfx_batch_size': ([_P, _P, _P], _I),
-    'hfx_batch_set_config': ([_P, _I, _I], _I),
+    'hfx_batch_set_config': ([_P, _I, _I, _I], _I),
     'hfx_batch_get_config': ([_P, _P], _I),
     'hfx_batch_tune': ([_P, _I, _P], _I),
     'hfx_batch_assemble': ([_P, _I], _I),

@@ -1,12 +1,16 @@
 // C ABI of batched assembly (hfx_batch_*), declared in include/hfx_hip.h.
 //
 // Many independent 2-D HDG problems of mixed order are assembled and
-// condensed by ONE persistent launch of the batched affine-cell kernel,
-// followed by each problem's trace scatter; the whole sequence can be
-// captured once as a HIP graph and replayed.  The work list is cut into
-// items of `tile` cells, sorted by decreasing estimated cost and dealt
-// round-robin to the workgroups; (waves per workgroup, tile) come from an
+// condensed by one persistent launch per order present (at most five),
+// running concurrently on side streams forked from and joined back to the
+// batch stream, followed by each problem's trace scatter; the whole sequence
+// is captured once as a HIP graph and replayed.  Each order's cells are cut
+// into items of `tile` cells, sorted largest first and dealt round-robin to
+// the workgroups; (waves per workgroup, tile) per order come from an
 // autotuner that times the candidate launches on the problems themselves.
+// (A single kernel switching on the order was measured 1.3x slower than
+// launching the problems one by one: it carries the register budget of the
+// highest order for every cell.)
 // The reference assembles one problem at a time on the host
 // (HDGSolver::assemble, src/solver/HDGSolver.cpp:166-675).
 #include <hip/hip_runtime.h>
@@ -24,18 +28,21 @@ void set_last_error(const std::string& msg);
 }
 
 struct hfx_batch {
+  static constexpr int NORD = 6;  // orders 1..5
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side[NORD] = {};       // one stream per order: the order kernels run concurrently
+  hipEvent_t fork = nullptr, join[NORD] = {};
   std::vector<hfx_hdg*> probs;
-  int waves = 2, tile = 64;
+  int waves[NORD] = {0, 2, 2, 2, 2, 2}, tile[NORD] = {0, 64, 64, 64, 64, 64};
   bool items_dirty = true;
-  std::vector<hfx::BatchItem> items;
+  std::vector<hfx::BatchItem> items[NORD];
   hfx::DevBuf<hfx::ElemArgs> dargs;
-  hfx::DevBuf<hfx::BatchItem> ditems;
-  int grid = 0;
+  hfx::DevBuf<hfx::BatchItem> ditems[NORD];
+  int grid[NORD] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  std::vector<double> tune_ms;  // per candidate, from the last tune
+  bool concurrent = true;  // order kernels on side streams (chosen by the tuner)
 
   void set_device() const { HFX_HIP(hipSetDevice(device)); }
 
@@ -46,55 +53,99 @@ struct hfx_batch {
     graph = nullptr;
   }
 
-  // cost of one cell of order p relative to p = 1: the condensation's dense
-  // work grows like the cube of the local system size
+  bool has(int p) const { return !items[p].empty(); }
+
+  // relative cost of one cell of order p: the condensation's dense work
+  // grows like the cube of the local system size
   static double cell_cost(int p) {
     const int nN = (p + 1) * (p + 2) / 2, n = 3 * nN + 3 * (p + 1);
     return (double)n * n * n;
   }
 
+  void build_order(int p) {
+    items[p].clear();
+    for (size_t k = 0; k < probs.size(); ++k) {
+      hfx_hdg* h = probs[k];
+      if (h->order != p) continue;
+      for (int e0 = 0; e0 < h->mesh.nCells; e0 += tile[p]) {
+        hfx::BatchItem it;
+        it.prob = (int)k;
+        it.order = p;
+        it.e0 = e0;
+        it.e1 = std::min(h->mesh.nCells, e0 + tile[p]);
+        items[p].push_back(it);
+      }
+    }
+    // largest items first, dealt round-robin to the workgroups
+    std::stable_sort(items[p].begin(), items[p].end(),
+                     [](const hfx::BatchItem& x, const hfx::BatchItem& y) { return (x.e1 - x.e0) > (y.e1 - y.e0); });
+    if (items[p].empty()) return;
+    hfx::prepare_affine_batch(p, waves[p]);
+    ditems[p].upload(items[p], stream);
+    // persistent grid: as many workgroups as fit on the 256 CUs at once
+    const size_t lds = hfx::affine_batch_lds_bytes(p, waves[p]);
+    const int per_cu = std::max(1, (int)((160 * 1024) / (lds + 1024)));
+    grid[p] = std::min((int)items[p].size(), 256 * std::min(per_cu, 8));
+  }
+
   void build_items() {
     if (!items_dirty) return;
     std::vector<hfx::ElemArgs> args;
-    items.clear();
-    for (size_t k = 0; k < probs.size(); ++k) {
-      hfx_hdg* h = probs[k];
+    for (hfx_hdg* h : probs) {
       hfx::ElemArgs a = h->elem_args();
       a.tabs = h->tabs_aff.p;
       args.push_back(a);
-      for (int e0 = 0; e0 < h->mesh.nCells; e0 += tile) {
-        hfx::BatchItem it;
-        it.prob = (int)k;
-        it.order = h->order;
-        it.e0 = e0;
-        it.e1 = std::min(h->mesh.nCells, e0 + tile);
-        items.push_back(it);
-      }
     }
-    std::stable_sort(items.begin(), items.end(), [](const hfx::BatchItem& x, const hfx::BatchItem& y) {
-      return (x.e1 - x.e0) * cell_cost(x.order) > (y.e1 - y.e0) * cell_cost(y.order);
-    });
-    hfx::prepare_affine_batch(waves);
     dargs.upload(args, stream);
-    ditems.upload(items, stream);
+    for (int p = 1; p < NORD; ++p) build_order(p);
     HFX_HIP(hipStreamSynchronize(stream));
-    // persistent grid: as many workgroups as fit on the 256 CUs at once
-    const size_t lds = hfx::affine_batch_lds_bytes(waves);
-    const int per_cu = std::max(1, (int)((160 * 1024) / (lds + 1024)));
-    grid = std::min((int)items.size(), 256 * per_cu);
     items_dirty = false;
     drop_graph();
   }
 
-  void run_kernel() {
+  void launch_order(int p, hipStream_t s) {
+    hfx::launch_affine_batch(p, waves[p], dargs.p, ditems[p].p, (int)items[p].size(), grid[p], s);
+  }
+
+  // every order's kernel, concurrently on the side streams (joined back) or
+  // one after another on the batch stream
+  void run_kernels() {
     build_items();
-    hfx::launch_affine_batch(waves, dargs.p, ditems.p, (int)items.size(), grid, stream);
+    if (!concurrent) {
+      for (int p = 1; p < NORD; ++p)
+        if (has(p)) launch_order(p, stream);
+      return;
+    }
+    HFX_HIP(hipEventRecord(fork, stream));
+    for (int p = 1; p < NORD; ++p) {
+      if (!has(p)) continue;
+      HFX_HIP(hipStreamWaitEvent(side[p], fork, 0));
+      launch_order(p, side[p]);
+      HFX_HIP(hipEventRecord(join[p], side[p]));
+      HFX_HIP(hipStreamWaitEvent(stream, join[p], 0));
+    }
   }
 
   void run_all() {
-    run_kernel();
+    run_kernels();
     for (hfx_hdg* h : probs)
       hfx::launch_scatter(h->A, h->mesh.nFpc, h->S.p, h->S0.p, h->invperm.p, h->is_dir.p, h->g.p, h->rhs.p, stream);
+  }
+
+  void capture() {
+    if (exec) return;
+    build_items();
+    HFX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    try {
+      run_all();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HFX_HIP(hipStreamEndCapture(stream, &graph));
+    HFX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
 
   void sync_problems() {
@@ -133,6 +184,11 @@ extern "C" int hfx_batch_create(int device, hfx_batch** out) {
   b->device = device;
   b->set_device();
   HFX_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  for (int p = 1; p < hfx_batch::NORD; ++p) {
+    HFX_HIP(hipStreamCreateWithFlags(&b->side[p], hipStreamNonBlocking));
+    HFX_HIP(hipEventCreateWithFlags(&b->join[p], hipEventDisableTiming));
+  }
+  HFX_HIP(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
   *out = b.release();
   B_END
 }
@@ -142,6 +198,14 @@ extern "C" void hfx_batch_free(hfx_batch* b) {
   (void)hipSetDevice(b->device);
   (void)hipStreamSynchronize(b->stream);
   b->drop_graph();
+  for (int p = 1; p < hfx_batch::NORD; ++p) {
+    if (b->side[p]) {
+      (void)hipStreamSynchronize(b->side[p]);
+      (void)hipStreamDestroy(b->side[p]);
+    }
+    if (b->join[p]) (void)hipEventDestroy(b->join[p]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -172,76 +236,99 @@ extern "C" int hfx_batch_size(hfx_batch* b, int* n, long long* cells) {
   B_END
 }
 
-extern "C" int hfx_batch_set_config(hfx_batch* b, int waves, int tile) {
+extern "C" int hfx_batch_set_config(hfx_batch* b, int order, int waves, int tile) {
   B_TRY
   need(b != nullptr, "null handle");
+  need(order >= 0 && order <= 5, "order must be 1..5, or 0 for all orders");
   need(waves == 1 || waves == 2 || waves == 4, "waves per workgroup must be 1, 2 or 4");
   need(tile >= 1 && tile <= (1 << 20), "tile must be in [1, 2^20] cells");
-  need(hfx::affine_batch_lds_bytes(waves) <= 160 * 1024, "this wave count needs more than 160 KB of LDS");
-  if (waves != b->waves || tile != b->tile) {
-    b->waves = waves;
-    b->tile = tile;
-    b->items_dirty = true;
-    b->drop_graph();
+  for (int p = 1; p < hfx_batch::NORD; ++p) {
+    if (order != 0 && p != order) continue;
+    need(hfx::affine_batch_lds_bytes(p, waves) <= 160 * 1024, "this wave count needs more than 160 KB of LDS");
+    b->waves[p] = waves;
+    b->tile[p] = tile;
   }
+  b->items_dirty = true;
+  b->drop_graph();
   B_END
 }
 
+// out[4 * (p - 1) + 0..3] = waves, tile, work items, workgroups of order p (1..5);
+// out[20] = 1 when the orders run concurrently
 extern "C" int hfx_batch_get_config(hfx_batch* b, int* out) {
   B_TRY
   need(b && out, "null argument");
   b->set_device();
   b->build_items();
-  out[0] = b->waves;
-  out[1] = b->tile;
-  out[2] = (int)b->items.size();
-  out[3] = b->grid;
+  for (int p = 1; p < hfx_batch::NORD; ++p) {
+    out[4 * (p - 1) + 0] = b->waves[p];
+    out[4 * (p - 1) + 1] = b->tile[p];
+    out[4 * (p - 1) + 2] = (int)b->items[p].size();
+    out[4 * (p - 1) + 3] = b->has(p) ? b->grid[p] : 0;
+  }
+  out[20] = b->concurrent ? 1 : 0;
   B_END
 }
 
-// Time every (waves, tile) candidate on the batch itself and keep the fastest.
+// Time every (waves, tile) candidate of each order's kernel on this batch
+// (alone on the batch stream) and keep the fastest per order, then choose
+// between running the orders concurrently or one after another; best_ms
+// returns the tuned launch of all orders.
 extern "C" int hfx_batch_tune(hfx_batch* b, int reps, double* best_ms) {
   B_TRY
   need(b != nullptr && !b->probs.empty(), "empty batch");
   need(reps >= 1 && reps <= 100, "reps must be in [1, 100]");
   b->set_device();
   b->sync_problems();
+  b->build_items();
   hipEvent_t e0, e1;
   HFX_HIP(hipEventCreate(&e0));
   HFX_HIP(hipEventCreate(&e1));
-  double best = 1e300;
-  int bw = b->waves, bt = b->tile;
-  b->tune_ms.clear();
-  for (int w : {1, 2, 4}) {
-    if (hfx::affine_batch_lds_bytes(w) > 160 * 1024) continue;
-    for (int t : {8, 16, 32, 64, 128, 256}) {
-      if (t < w) continue;
-      b->waves = w;
-      b->tile = t;
-      b->items_dirty = true;
-      b->run_kernel();  // warm-up (and item upload)
-      HFX_HIP(hipEventRecord(e0, b->stream));
-      for (int r = 0; r < reps; ++r) b->run_kernel();
-      HFX_HIP(hipEventRecord(e1, b->stream));
-      HFX_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      HFX_HIP(hipEventElapsedTime(&ms, e0, e1));
-      const double m = ms / reps;
-      b->tune_ms.push_back(m);
-      if (m < best) {
-        best = m;
-        bw = w;
-        bt = t;
+  auto timed = [&](auto&& fn) {
+    fn();  // warm-up
+    HFX_HIP(hipEventRecord(e0, b->stream));
+    for (int r = 0; r < reps; ++r) fn();
+    HFX_HIP(hipEventRecord(e1, b->stream));
+    HFX_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HFX_HIP(hipEventElapsedTime(&ms, e0, e1));
+    return (double)ms / reps;
+  };
+  for (int p = 1; p < hfx_batch::NORD; ++p) {
+    if (!b->has(p)) continue;
+    double best = 1e300;
+    int bw = b->waves[p], bt = b->tile[p];
+    for (int w : {1, 2, 4}) {
+      if (hfx::affine_batch_lds_bytes(p, w) > 160 * 1024) continue;
+      for (int t : {8, 16, 32, 64, 128, 256}) {
+        if (t < w) continue;
+        b->waves[p] = w;
+        b->tile[p] = t;
+        b->build_order(p);
+        HFX_HIP(hipStreamSynchronize(b->stream));
+        const double m = timed([&] { b->launch_order(p, b->stream); });
+        if (m < best) {
+          best = m;
+          bw = w;
+          bt = t;
+        }
       }
     }
+    b->waves[p] = bw;
+    b->tile[p] = bt;
+    b->build_order(p);
+    HFX_HIP(hipStreamSynchronize(b->stream));
   }
+  b->drop_graph();
+  b->concurrent = false;
+  const double serial = timed([&] { b->run_kernels(); });
+  b->concurrent = true;
+  const double conc = timed([&] { b->run_kernels(); });
+  b->concurrent = conc <= serial;
+  const double all = std::min(conc, serial);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b->waves = bw;
-  b->tile = bt;
-  b->items_dirty = true;
-  b->build_items();
-  if (best_ms) *best_ms = best;
+  if (best_ms) *best_ms = all;
   B_END
 }
 
@@ -252,19 +339,7 @@ extern "C" int hfx_batch_assemble(hfx_batch* b, int use_graph) {
   b->sync_problems();
   b->build_items();
   if (use_graph) {
-    if (!b->exec) {
-      HFX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-      try {
-        b->run_all();
-      } catch (...) {
-        hipGraph_t g = nullptr;
-        (void)hipStreamEndCapture(b->stream, &g);
-        if (g) (void)hipGraphDestroy(g);
-        throw;
-      }
-      HFX_HIP(hipStreamEndCapture(b->stream, &b->graph));
-      HFX_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-    }
+    b->capture();
     HFX_HIP(hipGraphLaunch(b->exec, b->stream));
   } else {
     b->run_all();
@@ -284,18 +359,13 @@ extern "C" int hfx_batch_time(hfx_batch* b, int what, int warmup, int reps, doub
   b->set_device();
   b->sync_problems();
   b->build_items();
-  if (what == 2 && !b->exec) {
-    HFX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed));
-    b->run_all();
-    HFX_HIP(hipStreamEndCapture(b->stream, &b->graph));
-    HFX_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-  }
+  if (what == 2) b->capture();
   hipEvent_t e0, e1;
   HFX_HIP(hipEventCreate(&e0));
   HFX_HIP(hipEventCreate(&e1));
   for (int i = 0; i < warmup + reps; ++i) {
     HFX_HIP(hipEventRecord(e0, b->stream));
-    if (what == 0) b->run_kernel();
+    if (what == 0) b->run_kernels();
     else if (what == 1) b->run_all();
     else if (what == 2) HFX_HIP(hipGraphLaunch(b->exec, b->stream));
     else {

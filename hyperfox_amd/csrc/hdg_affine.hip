@@ -31,6 +31,8 @@ struct RefConst;
 
 namespace {
 
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -110,25 +112,33 @@ __device__ inline int wave_argmax(double v, bool valid) {
   return 63 - (int)(key & 63u);
 }
 
+// Stage the reference tables of order (D, P) in LDS (`sm`, s_pos, s_fn);
+// they depend on the order only, not on the problem.
+template <int D, int P, bool TD, int W>
+__device__ __forceinline__ void affine_load_tables(const double* __restrict__ tabs, double* sm, int* s_pos,
+                                                   int* s_fn) {
+  using L = ALay<D, P, TD>;
+  using RC = RefConst<D, P>;
+  for (int t = threadIdx.x; t < L::TAB; t += 64 * W) sm[t] = tabs[t];
+  for (int t = threadIdx.x; t < L::NF * L::NN; t += 64 * W) s_pos[t] = RC::POS[t / L::NN][t % L::NN];
+  for (int t = threadIdx.x; t < L::NF * L::NNF; t += 64 * W) s_fn[t] = RC::FN[t / L::NNF][t % L::NNF];
+  __syncthreads();
+}
+
 // Cells e = e_first + wave, e_first + wave + e_stride, ... below e_end of one
-// problem, one wavefront per cell: loads the reference tables into LDS
-// (`sm`, s_pos, s_fn), then assembles and condenses each cell.  Shared by
-// the per-problem kernel and the batched kernel.
+// problem, one wavefront per cell, with the tables already staged: assembles
+// and condenses each cell.  Shared by the per-problem kernel and the batched
+// kernels.
 template <int D, int P, bool TD, int W>
 __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int e_end, int e_stride, double* sm,
                                              int* s_pos, int* s_fn) {
   using L = ALay<D, P, TD>;
-  using RC = RefConst<D, P>;
   constexpr int NN = L::NN, NNF = L::NNF, NIP = L::NIP, NIPF = L::NIPF, NF = L::NF, NL = L::NL, NS = L::NS;
   constexpr int NCOL = L::NCOL, ND = L::ND;
   static_assert(!TD, "tensor diffusion runs on the general element kernel");
 
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  for (int t = threadIdx.x; t < L::TAB; t += 64 * W) sm[t] = a.tabs[t];
-  for (int t = threadIdx.x; t < NF * NN; t += 64 * W) s_pos[t] = RC::POS[t / NN][t % NN];
-  for (int t = threadIdx.x; t < NF * NNF; t += 64 * W) s_fn[t] = RC::FN[t / NNF][t % NNF];
-  __syncthreads();
 
   const double* PHI = sm + L::T_PHI;
   const double* DPHI = sm + L::T_DPHI;
@@ -360,9 +370,42 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       }
     }
     wave_sync();
-    // K = Suu - sum_k Suq_k A_k = Suu - sum_a SUQ_a Ahat_a, one row per lane
-    // in registers; then K^-1 by Gauss-Jordan with partial pivoting on the
-    // register rows (the pivot row is fetched with readlane, no LDS traffic)
+    // K = Suu - sum_k Suq_k A_k = Suu - sum_a SUQ_a Ahat_a on FP64 MFMA, in
+    // place in KM: [SUQ_0 | .. | SUQ_{D-1}] (NN x D NN) times [Ahat_0; ..] (D NN x NN)
+    {
+      constexpr int MT = (NN + 15) / 16, KD = D * NN, KS = (KD + 3) / 4;
+      const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 1
+      for (int nt = 0; nt < MT; ++nt) {
+        const int jb = nt * 16 + li;
+        dvec4 acc[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[m] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 3
+        for (int ks = 0; ks < KS; ++ks) {
+          const int kk = ks * 4 + lk;
+          const bool kv = kk < KD;
+          const int a2 = kv ? kk / NN : 0, l = kv ? kk % NN : 0;
+          const double bv = (kv && jb < NN) ? AH[(a2 * NN + l) * NN + jb] : 0.0;
+#pragma unroll
+          for (int m = 0; m < MT; ++m) {
+            const int i = m * 16 + li;
+            const double av = (kv && i < NN) ? SUQ[(a2 * NN + i) * NN + l] : 0.0;
+            acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[m], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = m * 16 + 4 * r + lk;
+            if (i < NN && jb < NN) KM[i * NN + jb] -= acc[m][r];
+          }
+      }
+    }
+    wave_sync();
+    // K^-1 by Gauss-Jordan with partial pivoting, one row of K per lane in
+    // registers (the pivot row is fetched with readlane, no LDS traffic)
     {
       double kr[NN], ir[NN];
       const int rl = lane < NN ? lane : 0;
@@ -370,15 +413,6 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       for (int j = 0; j < NN; ++j) {
         kr[j] = KM[rl * NN + j];
         ir[j] = (j == lane) ? 1.0 : 0.0;
-      }
-      for (int r2 = 0; r2 < D; ++r2) {
-#pragma unroll 2
-        for (int l = 0; l < NN; ++l) {
-          const double sv = SUQ[(r2 * NN + rl) * NN + l];
-          const double* arow = AH + (r2 * NN + l) * NN;
-#pragma unroll
-          for (int j = 0; j < NN; ++j) kr[j] -= sv * arow[j];
-        }
       }
       unsigned long long used = 0ull;
       double mypiv = 1.0;
@@ -416,9 +450,6 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     const int c = lane;
     const bool tr = c < NL;
     const int g = tr ? c / NNF : 0, bp = tr ? c % NNF : 0;
-    double ng[D];
-    for (int k = 0; k < D; ++k) ng[k] = FG[g * (D + 1) + k];
-    const double rho = FG[g * (D + 1) + D] / det;
     double* UB = RB;               // U columns replace R columns lane by lane
     double* QN = ws + L::O_SUQ;    // face traces of n.q, over the dead Suq and K
     // R[:, c] = Sul[:, c] - sum_k Suq_k B_k[:, c] (or -F_u on lane NL), with
@@ -430,27 +461,53 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         for (int a2 = 0; a2 < NNF; ++a2)
           RB[s_fn[g * NNF + a2] * NCOL + c] =
               -FM[((g * NSF + 0) * NNF + a2) * NNF + bp] + FM[((g * NSF + 1) * NNF + a2) * NNF + bp];
-        double nt[D];
-        for (int r2 = 0; r2 < D; ++r2) {
+      }
+    }
+    wave_sync();
+    // ... + sum_a nt_a(c) SUQ_a G_g(c)[:, bp(c)] for the trace columns on FP64
+    // MFMA: X = [SUQ_0 | .. | SUQ_{D-1}] (NN x D NN) times Bt (D NN x NL),
+    // Bt[(a, j)][c] = nt_a(c) G_g(c)[j][bp(c)], nt_a = rho_g sum_k J[a][k] n_gk.
+    // Operand lanes: A[lane % 16][lane / 16], B[lane / 16][lane % 16]; result
+    // row 4 r + lane / 16, column lane % 16 sits in accumulator element r.
+    {
+      constexpr int MT = (NN + 15) / 16, NT = (NL + 15) / 16, KD = D * NN, KS = (KD + 3) / 4;
+      const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 1
+      for (int nt = 0; nt < NT; ++nt) {
+        const int cc = nt * 16 + li;
+        const bool cv = cc < NL;
+        const int gg = cv ? cc / NNF : 0, bb = cv ? cc % NNF : 0;
+        const double rg = FG[gg * (D + 1) + D] / det;
+        double ntc[3] = {0.0, 0.0, 0.0};
+        for (int a2 = 0; a2 < D; ++a2) {
           double s = 0.0;
-          for (int k = 0; k < D; ++k) s += J[r2 * D + k] * ng[k];
-          nt[r2] = rho * s;
+          for (int k = 0; k < D; ++k) s += J[a2 * D + k] * FG[gg * (D + 1) + k];
+          ntc[a2] = cv ? rg * s : 0.0;
         }
-        double gcol[NN];
+        dvec4 acc[MT];
 #pragma unroll
-        for (int j = 0; j < NN; ++j) gcol[j] = GT[((size_t)g * NN + j) * NNF + bp];
-#pragma unroll 2
-        for (int i = 0; i < NN; ++i) {
-          double s = 0.0;
-          for (int r2 = 0; r2 < D; ++r2) {
-            const double* srow = SUQ + (r2 * NN + i) * NN;
-            double t2 = 0.0;
+        for (int m = 0; m < MT; ++m) acc[m] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 3
+        for (int ks = 0; ks < KS; ++ks) {
+          const int kk = ks * 4 + lk;
+          const bool kv = kk < KD;
+          const int a2 = kv ? kk / NN : 0, j = kv ? kk % NN : 0;
+          const double ntv = a2 == 0 ? ntc[0] : (a2 == 1 ? ntc[1] : ntc[2]);
+          const double bv = kv ? ntv * GT[(gg * NN + j) * NNF + bb] : 0.0;
 #pragma unroll
-            for (int j = 0; j < NN; ++j) t2 += srow[j] * gcol[j];
-            s += nt[r2] * t2;
+          for (int m = 0; m < MT; ++m) {
+            const int i = m * 16 + li;
+            const double av = (kv && i < NN) ? SUQ[(a2 * NN + i) * NN + j] : 0.0;
+            acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[m], 0, 0, 0);
           }
-          RB[i * NCOL + c] += s;
         }
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = m * 16 + 4 * r + lk;
+            if (cv && i < NN) RB[i * NCOL + cc] += acc[m][r];
+          }
       }
     }
     wave_sync();
@@ -470,69 +527,120 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       }
     }
     wave_sync();  // every lane is done with Suq and K^-1 before QN overwrites them
-    if (c <= NL) {
-      double u[NN];
+    // Q_k = -A_k U - B_k on FP64 MFMA: Y_a = Ahat_a U (NN x NN times NN x NCOL),
+    // then per entry q_k = -sum_a Jinv[k][a] Y_a + n_gk rho_g G_g[i][bp];
+    // writes U, Q (U0, Q0 for column NL) and keeps n_f . q on the faces (QN)
+    {
+      constexpr int MT = (NN + 15) / 16, NT = (NCOL + 15) / 16, KS = (NN + 3) / 4;
+      const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 1
+      for (int nt = 0; nt < NT; ++nt) {
+        const int cc = nt * 16 + li;
+        dvec4 y[MT][D];
 #pragma unroll
-      for (int j = 0; j < NN; ++j) u[j] = UB[j * NCOL + c];
-      // Q_k = -A_k u - B_k[:, c]; write U, Q (or U0, Q0); keep n_f . q on faces
-      double* Qe = tr ? a.Q + (size_t)e * NN * D * NL : a.Q0 + (size_t)e * NN * D;
-      double* Ue = tr ? a.U + (size_t)e * NN * NL : a.U0 + (size_t)e * NN;
-#pragma unroll 2
-      for (int i = 0; i < NN; ++i) {
-        double y[D];
-        for (int r2 = 0; r2 < D; ++r2) {
-          const double* arow = AH + (r2 * NN + i) * NN;
-          double s = 0.0;
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
-          for (int j = 0; j < NN; ++j) s += arow[j] * u[j];
-          y[r2] = s;
-        }
-        const double gb = tr ? rho * GT[((size_t)g * NN + i) * NNF + bp] : 0.0;
-        double qk[D];
-        for (int k = 0; k < D; ++k) {
-          double s = 0.0;
-          for (int r2 = 0; r2 < D; ++r2) s -= JI[k * D + r2] * y[r2];
-          qk[k] = s + ng[k] * gb;
-        }
-        if (tr) {
-          Ue[(size_t)i * NL + c] = UB[i * NCOL + c];
-          for (int k = 0; k < D; ++k) Qe[(size_t)(i * D + k) * NL + c] = qk[k];
-        } else {
-          Ue[i] = UB[i * NCOL + c];
-          for (int k = 0; k < D; ++k) Qe[i * D + k] = qk[k];
-        }
-        for (int f = 0; f < NF; ++f) {
-          const int pf = s_pos[f * NN + i];
-          if (pf >= 0) {
-            double s = 0.0;
-            for (int k = 0; k < D; ++k) s += FG[f * (D + 1) + k] * qk[k];
-            QN[(f * NNF + pf) * NCOL + c] = s;
+          for (int a2 = 0; a2 < D; ++a2) y[m][a2] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const int j = ks * 4 + lk;
+          const bool kv = j < NN;
+          const double bv = (kv && cc < NCOL) ? UB[j * NCOL + cc] : 0.0;
+#pragma unroll
+          for (int m = 0; m < MT; ++m) {
+            const int i = m * 16 + li;
+#pragma unroll
+            for (int a2 = 0; a2 < D; ++a2) {
+              const double av = (kv && i < NN) ? AH[(a2 * NN + i) * NN + j] : 0.0;
+              y[m][a2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, y[m][a2], 0, 0, 0);
+            }
           }
         }
+        if (cc <= NL) {
+          const bool trc = cc < NL;
+          const int gg = trc ? cc / NNF : 0, bb = trc ? cc % NNF : 0;
+          const double rg = FG[gg * (D + 1) + D] / det;
+          double* Qe = trc ? a.Q + (size_t)e * NN * D * NL : a.Q0 + (size_t)e * NN * D;
+          double* Ue = trc ? a.U + (size_t)e * NN * NL : a.U0 + (size_t)e * NN;
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const int i = m * 16 + 4 * r + lk;
+              if (i >= NN) continue;
+              const double gb = trc ? rg * GT[(gg * NN + i) * NNF + bb] : 0.0;
+              double qk[D];
+              for (int k = 0; k < D; ++k) {
+                double s = 0.0;
+                for (int a2 = 0; a2 < D; ++a2) s -= JI[k * D + a2] * y[m][a2][r];
+                qk[k] = s + FG[gg * (D + 1) + k] * gb;
+              }
+              if (trc) {
+                Ue[(size_t)i * NL + cc] = UB[i * NCOL + cc];
+                for (int k = 0; k < D; ++k) Qe[(size_t)(i * D + k) * NL + cc] = qk[k];
+              } else {
+                Ue[i] = UB[i * NCOL + cc];
+                for (int k = 0; k < D; ++k) Qe[i * D + k] = qk[k];
+              }
+              for (int f = 0; f < NF; ++f) {
+                const int pf = s_pos[f * NN + i];
+                if (pf >= 0) {
+                  double sq = 0.0;
+                  for (int k = 0; k < D; ++k) sq += FG[f * (D + 1) + k] * qk[k];
+                  QN[(f * NNF + pf) * NCOL + cc] = sq;
+                }
+              }
+            }
+        }
       }
-      // S[:, c] = Slu u + sum_k Slq_k q_k (+ Sll), or S0 = -(Slu u0 + Slq q0)
-      double* Se = tr ? a.S + (size_t)e * NL * NL : a.S0 + (size_t)e * NL;
+    }
+    wave_sync();  // QN complete before the S rows read it
+    // S rows of face f on FP64 MFMA: [Slu_f | Slq_f] (NNF x 2 NNF) times
+    // [U at the face nodes; n.q at the face nodes] (2 NNF x NCOL), i.e.
+    // S[(f, aa)][c] = sum_b FMt[aa][b] U[fn_b][c] - FMd[aa][b] QN[(f, b)][c],
+    // plus -FMt[aa][bp] + FMb[aa][bp] on the trace columns of face f (Sll);
+    // the F_u column gives S0 = -(...)
+    {
+      constexpr int NT = (NCOL + 15) / 16, KF = 2 * NNF, KS = (KF + 3) / 4;
+      const int li = lane & 15, lk = lane >> 4;
+      static_assert(NNF <= 16, "one row tile per face");
 #pragma unroll 1
       for (int f = 0; f < NF; ++f) {
-        double ut[NNF], qt[NNF];
-#pragma unroll
-        for (int b = 0; b < NNF; ++b) {
-          ut[b] = UB[s_fn[f * NNF + b] * NCOL + c];
-          qt[b] = QN[(f * NNF + b) * NCOL + c];
-        }
         const double* fmt = FM + (f * NSF + 0) * NNF * NNF;
         const double* fmb = FM + (f * NSF + 1) * NNF * NNF;
         const double* fmd = FM + (f * NSF + 2) * NNF * NNF;
+#pragma unroll 1
+        for (int nt = 0; nt < NT; ++nt) {
+          const int cc = nt * 16 + li;
+          dvec4 acc = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int aa = 0; aa < NNF; ++aa) {
-          double s = 0.0;
+          for (int ks = 0; ks < KS; ++ks) {
+            const int kk = ks * 4 + lk;
+            const bool kv = kk < KF;
+            const int b = kv ? (kk < NNF ? kk : kk - NNF) : 0;
+            double bv = 0.0;
+            if (kv && cc < NCOL)
+              bv = kk < NNF ? UB[s_fn[f * NNF + b] * NCOL + cc] : QN[(f * NNF + b) * NCOL + cc];
+            double av = 0.0;
+            if (kv && li < NNF) av = kk < NNF ? fmt[li * NNF + b] : -fmd[li * NNF + b];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+          }
+          if (cc <= NL) {
+            const bool trc = cc < NL;
+            const int gg = trc ? cc / NNF : 0, bb = trc ? cc % NNF : 0;
+            double* Se = trc ? a.S + (size_t)e * NL * NL : a.S0 + (size_t)e * NL;
 #pragma unroll
-          for (int b = 0; b < NNF; ++b) s += fmt[aa * NNF + b] * ut[b] - fmd[aa * NNF + b] * qt[b];
-          if (tr) {
-            if (f == g) s += -fmt[aa * NNF + bp] + fmb[aa * NNF + bp];
-            Se[(size_t)(f * NNF + aa) * NL + c] = s;
-          } else {
-            Se[f * NNF + aa] = -s;
+            for (int r = 0; r < 4; ++r) {
+              const int aa = 4 * r + lk;
+              if (aa >= NNF) continue;
+              double sv = acc[r];
+              if (trc) {
+                if (f == gg) sv += -fmt[aa * NNF + bb] + fmb[aa * NNF + bb];
+                Se[(size_t)(f * NNF + aa) * NL + cc] = sv;
+              } else {
+                Se[f * NNF + aa] = -sv;
+              }
+            }
           }
         }
       }
@@ -547,29 +655,29 @@ __global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
   extern __shared__ double sm[];
   __shared__ int s_pos[L::NF * L::NN];
   __shared__ int s_fn[L::NF * L::NNF];
+  affine_load_tables<D, P, TD, W>(a.tabs, sm, s_pos, s_fn);
   affine_cells<D, P, TD, W>(a, a.el_offset + blockIdx.x * W, a.el_offset + a.nEl, gridDim.x * W, sm, s_pos, s_fn);
 }
 
-// One launch over many 2-D problems of mixed order: workgroup b takes the
-// items b, b + gridDim.x, ... of a cost-sorted list; an item is a range of
-// cells of one problem, all of one order.
-template <int W>
+// Persistent kernel over the cells of many 2-D problems of one order P:
+// workgroup b stages the order's reference tables once, then takes the items
+// b, b + gridDim.x, ... of a cost-sorted list; an item is a range of cells of
+// one problem.  Each order has its own kernel so that its register and LDS
+// budget is that of the order alone.
+template <int P, int W>
 __global__ __launch_bounds__(64 * W) void hdg_affine_batch_kernel(const ElemArgs* __restrict__ probs,
                                                                   const BatchItem* __restrict__ items, int nitems) {
+  using L = ALay<2, P, false>;
   extern __shared__ double sm[];
-  __shared__ int s_pos[3 * 21];
-  __shared__ int s_fn[3 * 6];
+  __shared__ int s_pos[L::NF * L::NN];
+  __shared__ int s_fn[L::NF * L::NNF];
+  if (blockIdx.x >= nitems) return;
+  // every problem of one order shares the reference tables: stage them once
+  affine_load_tables<2, P, false, W>(probs[items[blockIdx.x].prob].tabs, sm, s_pos, s_fn);
   for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
     const BatchItem w = items[it];
     const ElemArgs a = probs[w.prob];
-    switch (w.order) {
-      case 1: affine_cells<2, 1, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
-      case 2: affine_cells<2, 2, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
-      case 3: affine_cells<2, 3, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
-      case 4: affine_cells<2, 4, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
-      default: affine_cells<2, 5, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn); break;
-    }
-    __syncthreads();  // the next item reloads the tables
+    affine_cells<2, P, false, W>(a, w.e0, w.e1, W, sm, s_pos, s_fn);
   }
 }
 
@@ -629,24 +737,25 @@ void launch_affine(int dim, int order, bool tensor_d, int waves, const ElemArgs&
 
 namespace hfx {
 
-size_t affine_batch_lds_bytes(int waves) {
-  size_t m = 0;
-  m = std::max(m, aff_lds<2, 1>(false, waves));
-  m = std::max(m, aff_lds<2, 2>(false, waves));
-  m = std::max(m, aff_lds<2, 3>(false, waves));
-  m = std::max(m, aff_lds<2, 4>(false, waves));
-  m = std::max(m, aff_lds<2, 5>(false, waves));
-  return m;
+size_t affine_batch_lds_bytes(int order, int waves) {
+  switch (order) {
+    case 1: return aff_lds<2, 1>(false, waves);
+    case 2: return aff_lds<2, 2>(false, waves);
+    case 3: return aff_lds<2, 3>(false, waves);
+    case 4: return aff_lds<2, 4>(false, waves);
+    case 5: return aff_lds<2, 5>(false, waves);
+    default: throw std::runtime_error("batched element kernel: order must be 1..5");
+  }
 }
 
-void launch_affine_batch(int waves, const ElemArgs* probs, const BatchItem* items, int nitems, int grid,
+void launch_affine_batch(int order, int waves, const ElemArgs* probs, const BatchItem* items, int nitems, int grid,
                          hipStream_t s, bool attribute_only) {
   if (nitems <= 0) return;
-  const size_t lds = affine_batch_lds_bytes(waves);
+  const size_t lds = affine_batch_lds_bytes(order, waves);
   if (lds > 160 * 1024) throw std::runtime_error("batched element kernel: LDS exceeds 160 KB for this wave count");
   if (grid <= 0) throw std::runtime_error("batched element kernel: empty grid");
   auto go = [&](auto kern) {
-    // the LDS limit is raised once, outside any stream capture (see prepare_affine_batch)
+    // the LDS limit is raised once, outside any stream capture (prepare_affine_batch)
     static bool attr_set = false;
     if (!attr_set) {
       HFX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -656,17 +765,25 @@ void launch_affine_batch(int waves, const ElemArgs* probs, const BatchItem* item
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * waves), lds, s, probs, items, nitems);
     HFX_HIP(hipGetLastError());
   };
-  switch (waves) {
-    case 1: go(hdg_affine_batch_kernel<1>); break;
-    case 2: go(hdg_affine_batch_kernel<2>); break;
-    case 4: go(hdg_affine_batch_kernel<4>); break;
-    default: throw std::runtime_error("batched element kernel: waves per workgroup must be 1, 2 or 4");
+#define HFX_BW(P)                                                                     \
+  case P:                                                                             \
+    switch (waves) {                                                                  \
+      case 1: go(hdg_affine_batch_kernel<P, 1>); break;                              \
+      case 2: go(hdg_affine_batch_kernel<P, 2>); break;                              \
+      case 4: go(hdg_affine_batch_kernel<P, 4>); break;                              \
+      default: throw std::runtime_error("batched element kernel: waves must be 1, 2 or 4"); \
+    }                                                                                 \
+    break;
+  switch (order) {
+    HFX_BW(1) HFX_BW(2) HFX_BW(3) HFX_BW(4) HFX_BW(5)
+    default: throw std::runtime_error("batched element kernel: order must be 1..5");
   }
+#undef HFX_BW
 }
 
-void prepare_affine_batch(int waves) {
+void prepare_affine_batch(int order, int waves) {
   const BatchItem* none = nullptr;
-  launch_affine_batch(waves, nullptr, none, 1, 1, nullptr, true);
+  launch_affine_batch(order, waves, nullptr, none, 1, 1, nullptr, true);
 }
 
 }  // namespace hfx

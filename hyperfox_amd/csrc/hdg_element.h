@@ -69,19 +69,20 @@ size_t element_lds_bytes(int dim, int order);
 void launch_affine(int dim, int order, bool tensor_d, int waves, const ElemArgs& a, int grid_cap, hipStream_t s);
 size_t affine_lds_bytes(int dim, int order, bool tensor_d, int waves);
 
-// One work item of the batched affine kernel: cells [e0, e1) of problem
-// `prob` (index into the ElemArgs table), all of order `order` (2-D).
+// One work item of the batched affine kernels: cells [e0, e1) of problem
+// `prob` (index into the ElemArgs table) of order `order` (2-D).
 struct BatchItem {
   int prob = 0, order = 0, e0 = 0, e1 = 0;
 };
 
-// Batched affine-cell kernel over 2-D problems of orders 1..5; `probs` and
-// `items` are device arrays; grid = number of workgroups (persistent).
-void launch_affine_batch(int waves, const ElemArgs* probs, const BatchItem* items, int nitems, int grid,
+// Batched affine-cell kernel over the cells of many 2-D problems of one
+// order (1..5); `probs` and `items` are device arrays; grid = number of
+// workgroups (persistent).
+void launch_affine_batch(int order, int waves, const ElemArgs* probs, const BatchItem* items, int nitems, int grid,
                          hipStream_t s, bool attribute_only = false);
-size_t affine_batch_lds_bytes(int waves);
+size_t affine_batch_lds_bytes(int order, int waves);
 // Raise the batched kernel's dynamic-LDS limit (done outside graph capture).
-void prepare_affine_batch(int waves);
+void prepare_affine_batch(int order, int waves);
 
 // Physical coordinates of the element IPs: out[(e*NIP + q)*D + k].
 void launch_ip_coords(int dim, int order, int nEl, const double* tabs, const double* coords, const int* cells,

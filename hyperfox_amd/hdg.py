@@ -235,13 +235,17 @@ class Batch:
         check(lib.hfx_batch_size(self._h, ctypes.byref(n), ctypes.byref(c)))
         return n.value, c.value
 
-    def set_config(self, waves, tile):
-        check(lib.hfx_batch_set_config(self._h, waves, tile))
+    def set_config(self, waves, tile, order=0):
+        """Launch shape of one order (1..5) or of all (0)."""
+        check(lib.hfx_batch_set_config(self._h, order, waves, tile))
 
     def config(self):
-        out = np.zeros(4, dtype=np.int32)
+        """{order: dict(waves, tile, items, workgroups)} for the orders present."""
+        out = np.zeros(21, dtype=np.int32)
         check(lib.hfx_batch_get_config(self._h, _ptr(out)))
-        return dict(waves=int(out[0]), tile=int(out[1]), items=int(out[2]), workgroups=int(out[3]))
+        self.concurrent = bool(out[20])
+        return {p: dict(waves=int(out[4 * (p - 1)]), tile=int(out[4 * (p - 1) + 1]), items=int(out[4 * (p - 1) + 2]),
+                        workgroups=int(out[4 * (p - 1) + 3])) for p in range(1, 6) if out[4 * (p - 1) + 2] > 0}
 
     def tune(self, reps=3):
         v = ctypes.c_double()
