@@ -314,58 +314,68 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     }
     wave_sync();
     // ----------------------------------------- Suq_k and Suu (entries)
+    // On FP64 MFMA over the cubature points: Suq_k = G_k PHI^T and
+    // Suu = G_u PHI^T with G_k[i][q] = sum_c EJ[q][k][c] dphi_c(i, q) and
+    // G_u[i][q] = WR[q] phi_i(q) - sum_k BJ[q][k] dphi_k(i, q), K = NIP;
+    // then the face terms, and Suq moved to the reference frame
+    // SUQ_a = sum_k Jinv[k][a] Suq_k.
     {
-      constexpr int RV = (NN * NN + 63) / 64;
-      double aq[RV][D], au[RV];
-      for (int r = 0; r < RV; ++r) {
-        au[r] = 0.0;
-        for (int k = 0; k < D; ++k) aq[r][k] = 0.0;
-      }
+      constexpr int MT = (NN + 15) / 16, KS = (NIP + 3) / 4, NX = D + 1;
+      const int li = lane & 15, lk = lane >> 4;
 #pragma unroll 1
-      for (int q = 0; q < NIP; ++q) {
-        double ej[D * D], bj[D];
-        for (int u = 0; u < D * D; ++u) ej[u] = EJ[q * D * D + u];
-        for (int c = 0; c < D; ++c) bj[c] = BJ[q * D + c];
-        const double wr = WR[q];
+      for (int m = 0; m < MT; ++m) {
+        const int ia = m * 16 + li;  // A row this lane supplies
+        double av[KS][NX];
 #pragma unroll
-        for (int r = 0; r < RV; ++r) {
-          const int t = lane + 64 * r;
-          if (t < NN * NN) {
-            const int i = t / NN, j = t % NN;
-            double g[D];
-            for (int c = 0; c < D; ++c) g[c] = DPHI[(q * NN + i) * D + c];
-            const double pj = PHI[q * NN + j];
-            double bg = 0.0;
-            for (int k = 0; k < D; ++k) {
-              double s = 0.0;
-              for (int c = 0; c < D; ++c) s += ej[k * D + c] * g[c];
-              aq[r][k] += s * pj;
-              bg += bj[k] * g[k];
-            }
-            au[r] += (wr * PHI[q * NN + i] - bg) * pj;
+        for (int ks = 0; ks < KS; ++ks) {
+          const int q = ks * 4 + lk;
+          const bool v = q < NIP && ia < NN;
+          double g[D];
+          for (int c2 = 0; c2 < D; ++c2) g[c2] = v ? DPHI[(q * NN + ia) * D + c2] : 0.0;
+          const int qq = v ? q : 0;
+          double bgs = 0.0;
+          for (int k = 0; k < D; ++k) {
+            double s2 = 0.0;
+            for (int c2 = 0; c2 < D; ++c2) s2 += EJ[qq * D * D + k * D + c2] * g[c2];
+            av[ks][k] = v ? s2 : 0.0;
+            bgs += BJ[qq * D + k] * g[k];
           }
+          av[ks][D] = v ? WR[qq] * PHI[qq * NN + ia] - bgs : 0.0;
         }
-      }
+#pragma unroll 1
+        for (int nt = 0; nt < MT; ++nt) {
+          const int jb = nt * 16 + li;
+          dvec4 acc[NX];
 #pragma unroll
-      for (int r = 0; r < RV; ++r) {
-        const int t = lane + 64 * r;
-        if (t < NN * NN) {
-          const int i = t / NN, j = t % NN;
-          for (int f = 0; f < NF; ++f) {
-            const int ai = s_pos[f * NN + i], bj = s_pos[f * NN + j];
-            if (ai >= 0 && bj >= 0) {
-              const double* F0 = FM + (f * NSF) * NNF * NNF + ai * NNF + bj;
-              au[r] += F0[0];
-              for (int k = 0; k < D; ++k) aq[r][k] -= FG[f * (D + 1) + k] * F0[2 * NNF * NNF];
+          for (int x = 0; x < NX; ++x) acc[x] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+          for (int ks = 0; ks < KS; ++ks) {
+            const int q = ks * 4 + lk;
+            const double bv = (q < NIP && jb < NN) ? PHI[q * NN + jb] : 0.0;
+#pragma unroll
+            for (int x = 0; x < NX; ++x) acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks][x], bv, acc[x], 0, 0, 0);
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = m * 16 + 4 * r + lk, j = jb;
+            if (i >= NN || j >= NN) continue;
+            double aq[D], au = acc[D][r];
+            for (int k = 0; k < D; ++k) aq[k] = acc[k][r];
+            for (int f = 0; f < NF; ++f) {
+              const int ai = s_pos[f * NN + i], bj = s_pos[f * NN + j];
+              if (ai >= 0 && bj >= 0) {
+                const double* F0 = FM + (f * NSF) * NNF * NNF + ai * NNF + bj;
+                au += F0[0];
+                for (int k = 0; k < D; ++k) aq[k] -= FG[f * (D + 1) + k] * F0[2 * NNF * NNF];
+              }
             }
+            for (int r2 = 0; r2 < D; ++r2) {
+              double s2 = 0.0;
+              for (int k = 0; k < D; ++k) s2 += JI[k * D + r2] * aq[k];
+              SUQ[(r2 * NN + i) * NN + j] = s2;
+            }
+            KM[i * NN + j] = au;
           }
-          // store Suq transformed to the reference frame: SUQ_a = sum_k Jinv[k][a] Suq_k
-          for (int r2 = 0; r2 < D; ++r2) {
-            double s = 0.0;
-            for (int k = 0; k < D; ++k) s += JI[k * D + r2] * aq[r][k];
-            SUQ[(r2 * NN + i) * NN + j] = s;
-          }
-          KM[t] = au[r];
         }
       }
     }
@@ -511,20 +521,44 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       }
     }
     wave_sync();
-    if (c <= NL) {
+    // U = -K^-1 R on FP64 MFMA, in place: every tile is accumulated before
+    // the first U value overwrites R
+    {
       // u = -K^-1 R[:, c]
-      double rr[NN], u[NN];  // R column read in full before U overwrites it
+      constexpr int MT = (NN + 15) / 16, NT = (NCOL + 15) / 16, KS = (NN + 3) / 4;
+      const int li = lane & 15, lk = lane >> 4;
+      dvec4 acc[MT][NT];
 #pragma unroll
-      for (int j = 0; j < NN; ++j) rr[j] = RB[j * NCOL + c];
+      for (int m = 0; m < MT; ++m)
 #pragma unroll
-      for (int i = 0; i < NN; ++i) {
-        const double* krow = KM + i * NN;
-        double s = 0.0;
+        for (int nt = 0; nt < NT; ++nt) acc[m][nt] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int j = 0; j < NN; ++j) s += krow[j] * rr[j];
-        u[i] = -s;
-        UB[i * NCOL + c] = u[i];
+      for (int ks = 0; ks < KS; ++ks) {
+        const int j = ks * 4 + lk;
+        const bool kv = j < NN;
+        double bv[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const int cb = nt * 16 + li;
+          bv[nt] = (kv && cb < NCOL) ? RB[j * NCOL + cb] : 0.0;
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const int i = m * 16 + li;
+          const double av = (kv && i < NN) ? KM[i * NN + j] : 0.0;
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) acc[m][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[nt], acc[m][nt], 0, 0, 0);
+        }
       }
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = m * 16 + 4 * r + lk, cc = nt * 16 + li;
+            if (i < NN && cc < NCOL) UB[i * NCOL + cc] = -acc[m][nt][r];
+          }
     }
     wave_sync();  // every lane is done with Suq and K^-1 before QN overwrites them
     // Q_k = -A_k U - B_k on FP64 MFMA: Y_a = Ahat_a U (NN x NN times NN x NCOL),
