@@ -95,25 +95,33 @@ struct ALay {
   static constexpr int PER = END1 > END2 ? END1 : END2;
 };
 
-// wave-wide argmax of a non-negative value; ties go to the lowest lane.  The
-// key keeps the top 26 bits of |v| as a float and the lane in the low 6 bits.
-__device__ inline int wave_argmax(double v, bool valid) {
+// Wave-wide argmax of a non-negative value over lanes < N; ties go to the
+// lowest lane.  The key keeps the top 26 bits of |v| as a float and the lane
+// in the low 6 bits; every lane posts its key to LDS, reads the N candidate
+// keys back (broadcast reads) and takes the maximum.  `keys` is per-wave
+// scratch of N words.
+template <int N>
+__device__ inline int lds_argmax(unsigned* keys, double v, bool valid) {
   unsigned key = 0u;
   if (valid) {
     float fv = (float)v;
     if (!(fv >= 0.0f)) fv = 0.0f;  // NaN
     key = (__float_as_uint(fv) & ~63u) | (63u - (unsigned)(threadIdx.x & 63));
-    key += 1u << 6;  // valid lanes beat invalid ones even when |v| = 0
+    key += 1u << 6;
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned o = (unsigned)__shfl_xor((int)key, off, 64);
-    key = o > key ? o : key;
+  const int lane = threadIdx.x & 63;
+  if (lane < N) keys[lane] = key;
+  wave_sync();
+  unsigned best = 0u;
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    const unsigned k2 = keys[l];
+    best = k2 > best ? k2 : best;
   }
-  return 63 - (int)(key & 63u);
+  wave_sync();  // every lane has read the keys before they are posted again
+  return 63 - (int)(best & 63u);
 }
 
-// Stage the reference tables of order (D, P) in LDS (`sm`, s_pos, s_fn);
-// they depend on the order only, not on the problem.
 template <int D, int P, bool TD, int W>
 __device__ __forceinline__ void affine_load_tables(const double* __restrict__ tabs, double* sm, int* s_pos,
                                                    int* s_fn) {
@@ -300,12 +308,37 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     }
     wave_sync();
     // weighted face masses FM[f][s][a][b] = sum_q w_s psi_a psi_b, s = 0: tau, 1: b.n, 2: D
+    // on FP64 MFMA: FM[f][s] = (w_s psi)^T psi over the face points (K = NIPF);
+    // lane (a, q) supplies psi_a(q) to both operands
     constexpr int NSF = L::NSF;
-    for (int t = lane; t < NF * NSF * NNF * NNF; t += 64) {
-      const int b = t % NNF, aa = (t / NNF) % NNF, sf = (t / (NNF * NNF)) % NSF, f = t / (NSF * NNF * NNF);
-      double acc = 0.0;
-      for (int q = 0; q < NIPF; ++q) acc += FWT[(f * NIPF + q) * NS + sf] * FPHI[q * NNF + aa] * FPHI[q * NNF + b];
-      FM[t] = acc;
+    {
+      constexpr int KSF = (NIPF + 3) / 4;
+      static_assert(NNF <= 16, "one MFMA tile per face mass");
+      const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 1
+      for (int f = 0; f < NF; ++f) {
+        dvec4 acc[NSF];
+#pragma unroll
+        for (int sf = 0; sf < NSF; ++sf) acc[sf] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KSF; ++ks) {
+          const int q = ks * 4 + lk;
+          const bool v = q < NIPF && li < NNF;
+          const double ps = v ? FPHI[q * NNF + li] : 0.0;
+#pragma unroll
+          for (int sf = 0; sf < NSF; ++sf) {
+            const double w = v ? FWT[(f * NIPF + q) * NS + sf] : 0.0;
+            acc[sf] = __builtin_amdgcn_mfma_f64_16x16x4f64(ps * w, ps, acc[sf], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int sf = 0; sf < NSF; ++sf)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int aa = 4 * r + lk;
+            if (aa < NNF && li < NNF) FM[((f * NSF + sf) * NNF + aa) * NNF + li] = acc[sf][r];
+          }
+      }
     }
     for (int i = lane; i < NN; i += 64) {
       double s = 0.0;
@@ -424,13 +457,14 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         kr[j] = KM[rl * NN + j];
         ir[j] = (j == lane) ? 1.0 : 0.0;
       }
+      unsigned* KEYS = reinterpret_cast<unsigned*>(ws + L::O_OV);  // dead overlay between the phases
       unsigned long long used = 0ull;
       double mypiv = 1.0;
       int myk = 0;
 #pragma unroll
       for (int k = 0; k < NN; ++k) {
         const int p = __builtin_amdgcn_readfirstlane(
-            wave_argmax(fabs(kr[k]), lane < NN && !((used >> lane) & 1ull)));
+            lds_argmax<NN>(KEYS, fabs(kr[k]), lane < NN && !((used >> lane) & 1ull)));
         used |= 1ull << p;
         if (lane == p) { mypiv = kr[k]; myk = k; }
         const double fac = kr[k] / readlane_d(kr[k], p);
