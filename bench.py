@@ -32,7 +32,7 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM KiB per element-kernel dispatch, (read, write), from rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE runs of this script, scaled by a 1 GiB copy kernel
 # (FETCH_SIZE x2.00, WRITE_SIZE x1.00); the summary is kept under profiles/.
-MEASURED_TRAFFIC_KIB = {(24, 3): (2 * 105748.1, 3191374.5)}
+MEASURED_TRAFFIC_KIB = {(24, 3): (2 * 100475.6, 3514147.3)}
 
 
 def elem_flops(dim, p, nN, nIP, nNf, nIPf, conv):
@@ -166,8 +166,9 @@ def batched(hf, pr, count=64, seed=7):
     """Batched workload: `count` independent 2-D diffusion problems (cells
     log-uniform in [1k, 40k], p uniform in 1..5, D log-uniform in
     [1e-2, 1e2], 8-mode Fourier sources) assembled by one persistent launch
-    whose (waves, tile) the autotuner picks, eager and as a replayed HIP graph,
-    against the same problems launched one by one."""
+    per order (running concurrently) whose (waves, tile) the autotuner picks,
+    eager and as a replayed HIP graph, against the same problems launched one
+    by one."""
     specs = pr.batch_specs(count=count, seed=seed)
     probs = []
     for sp in specs:
@@ -181,7 +182,8 @@ def batched(hf, pr, count=64, seed=7):
     dofs = sum(prob.mesh.n_cells * prob.sizes['nN'] for prob in probs)
     cells = sum(prob.mesh.n_cells for prob in probs)
     best = b.tune(reps=3)
-    cfg = b.config()
+    cfg = {str(k): v for k, v in b.config().items()}
+    cfg['concurrent_orders'] = b.concurrent
     out = {'workload': '%d 2-D HDG diffusion problems, %d cells, orders %s' % (
         count, cells, sorted({sp['p'] for sp in specs})), 'u_dofs': dofs, 'tuned': cfg, 'tune_best_kernel_ms': best}
     for name, what in (('kernel_only', 0), ('eager', 1), ('graph', 2), ('one_by_one', 3)):

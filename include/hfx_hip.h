@@ -260,12 +260,12 @@ int hfx_la_get_csr(hfx_la* h, long long* rowptr, int* cols, double* vals, double
 /* ------------------------------------------------------- batched assembly */
 
 /* Many independent 2-D problems (straight-sided cells, scalar diffusion,
- * orders 1..5, any mix) assembled and condensed by one persistent launch of
- * the batched element kernel, then each problem's scatter.  No counterpart in
- * the reference, which assembles one problem at a time on the host
- * (HDGSolver::assemble, HDGSolver.cpp:166-675); each problem's results are
- * exactly those of hfx_hdg_assemble.  The problems stay owned by the caller
- * and must outlive the batch. */
+ * orders 1..5, any mix) assembled and condensed by one persistent launch per
+ * order present, the orders running concurrently, then each problem's
+ * scatter.  No counterpart in the reference, which assembles one problem at a
+ * time on the host (HDGSolver::assemble, HDGSolver.cpp:166-675); each
+ * problem's results are exactly those of hfx_hdg_assemble.  The problems stay
+ * owned by the caller and must outlive the batch. */
 typedef struct hfx_batch hfx_batch;
 
 int hfx_batch_create(int device, hfx_batch** out);
@@ -274,18 +274,23 @@ void hfx_batch_free(hfx_batch* b);
 int hfx_batch_add(hfx_batch* b, hfx_hdg* h);
 /* Number of problems and total cells. */
 int hfx_batch_size(hfx_batch* b, int* n, long long* cells);
-/* Launch shape: wavefronts per workgroup (1, 2 or 4) and cells per work item. */
-int hfx_batch_set_config(hfx_batch* b, int waves, int tile);
-/* out[0..3] = waves, tile, work items, workgroups. */
+/* Launch shape of order `order` (1..5, or 0 for every order): wavefronts per
+ * workgroup (1, 2 or 4) and cells per work item. */
+int hfx_batch_set_config(hfx_batch* b, int order, int waves, int tile);
+/* out[4*(p-1) + 0..3] = waves, tile, work items, workgroups of order p = 1..5
+ * (zero items for an order not in the batch); out[20] = 1 when the orders run
+ * concurrently on side streams (21 ints). */
 int hfx_batch_get_config(hfx_batch* b, int* out);
-/* Autotune (waves, tile) by timing every candidate launch `reps` times on
- * this batch; keeps the fastest, whose mean time is returned in best_ms. */
+/* Autotune (waves, tile) of every order by timing each candidate launch
+ * `reps` times on this batch, keeping the fastest per order, then choose
+ * concurrent or one-after-another order launches; best_ms returns the time
+ * of the tuned launch of all orders. */
 int hfx_batch_tune(hfx_batch* b, int reps, double* best_ms);
-/* Assemble every problem: one batched launch + the scatters; use_graph != 0
- * captures the sequence as a HIP graph on first use and replays it. */
+/* Assemble every problem: the per-order launches + the scatters; use_graph
+ * != 0 captures the sequence as a HIP graph on first use and replays it. */
 int hfx_batch_assemble(hfx_batch* b, int use_graph);
-/* HIP-event times (ms, `reps` values): what = 0 batched kernel only,
- * 1 batched kernel + scatters, 2 the same replayed as a graph, 3 the
+/* HIP-event times (ms, `reps` values): what = 0 batched kernels only,
+ * 1 batched kernels + scatters, 2 the same replayed as a graph, 3 the
  * problems one by one with their own kernels. */
 int hfx_batch_time(hfx_batch* b, int what, int warmup, int reps, double* ms);
 

@@ -1,5 +1,6 @@
-"""Batched assembly of many 2-D problems of mixed order in one persistent
-launch: every problem's condensed blocks and trace system equal, bit for bit,
+"""Batched assembly of many 2-D problems of mixed order (one persistent
+launch per order, run concurrently): every problem's condensed blocks and
+trace system equal, bit for bit,
 what its own hfx_hdg_assemble produces (the same device code per cell), for
 eager launches, for several replays of the captured HIP graph (with eager
 work in between) and after autotuning."""
@@ -78,7 +79,8 @@ def test_graph_replays_and_tuning(batch_case):
     best = b.tune(reps=2)
     cfg = b.config()
     print('tuned: %s, %.3f ms' % (cfg, best))
-    assert best > 0 and cfg['waves'] in (1, 2, 4)
+    assert best > 0 and all(c['waves'] in (1, 2, 4) and c['items'] > 0 for c in cfg.values())
+    assert set(cfg) == {s['p'] for s in specs}
     b.assemble(use_graph=True)
     assert same(snapshot(probs), ref)
     t = b.time(3, warmup=1, reps=2)
