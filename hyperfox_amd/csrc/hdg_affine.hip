@@ -128,7 +128,16 @@ __device__ __forceinline__ void affine_load_tables(const double* __restrict__ ta
   using L = ALay<D, P, TD>;
   using RC = RefConst<D, P>;
   for (int t = threadIdx.x; t < L::TAB; t += 64 * W) sm[t] = tabs[t];
-  for (int t = threadIdx.x; t < L::NF * L::NN; t += 64 * W) s_pos[t] = RC::POS[t / L::NN][t % L::NN];
+  // s_pos[i]: byte f = position of node i on face f, 0xFF when i is not on f
+  static_assert(L::NF <= 4 && L::NNF < 255, "face positions packed in one word");
+  for (int t = threadIdx.x; t < L::NN; t += 64 * W) {
+    unsigned w = 0u;
+    for (int f = 0; f < L::NF; ++f) {
+      const int ps = RC::POS[f][t];
+      w |= (unsigned)(ps >= 0 ? ps : 0xFF) << (8 * f);
+    }
+    s_pos[t] = (int)w;
+  }
   for (int t = threadIdx.x; t < L::NF * L::NNF; t += 64 * W) s_fn[t] = RC::FN[t / L::NNF][t % L::NNF];
   __syncthreads();
 }
@@ -236,7 +245,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       nn = 1.0 / sqrt(nn);
       int vo = 0;
       for (int i = D; i >= 0; --i)
-        if (s_pos[f * NN + i] < 0) vo = i;
+        if (((unsigned)s_pos[i] >> (8 * f) & 0xFFu) == 0xFFu) vo = i;
       double prod = 0.0;
       for (int c = 0; c < D; ++c) {
         nv[c] *= nn;
@@ -394,9 +403,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
             if (i >= NN || j >= NN) continue;
             double aq[D], au = acc[D][r];
             for (int k = 0; k < D; ++k) aq[k] = acc[k][r];
+            const unsigned pi = (unsigned)s_pos[i], pj = (unsigned)s_pos[j];
             for (int f = 0; f < NF; ++f) {
-              const int ai = s_pos[f * NN + i], bj = s_pos[f * NN + j];
-              if (ai >= 0 && bj >= 0) {
+              const int ai = (int)(pi >> (8 * f) & 0xFFu), bj = (int)(pj >> (8 * f) & 0xFFu);
+              if (ai != 0xFF && bj != 0xFF) {
                 const double* F0 = FM + (f * NSF) * NNF * NNF + ai * NNF + bj;
                 au += F0[0];
                 for (int k = 0; k < D; ++k) aq[k] -= FG[f * (D + 1) + k] * F0[2 * NNF * NNF];
@@ -650,9 +660,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
                 Ue[i] = UB[i * NCOL + cc];
                 for (int k = 0; k < D; ++k) Qe[i * D + k] = qk[k];
               }
+              const unsigned pw = (unsigned)s_pos[i];
               for (int f = 0; f < NF; ++f) {
-                const int pf = s_pos[f * NN + i];
-                if (pf >= 0) {
+                const int pf = (int)(pw >> (8 * f) & 0xFFu);
+                if (pf != 0xFF) {
                   double sq = 0.0;
                   for (int k = 0; k < D; ++k) sq += FG[f * (D + 1) + k] * qk[k];
                   QN[(f * NNF + pf) * NCOL + cc] = sq;
@@ -721,7 +732,7 @@ template <int D, int P, bool TD, int W>
 __global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
   using L = ALay<D, P, TD>;
   extern __shared__ double sm[];
-  __shared__ int s_pos[L::NF * L::NN];
+  __shared__ int s_pos[L::NN];
   __shared__ int s_fn[L::NF * L::NNF];
   affine_load_tables<D, P, TD, W>(a.tabs, sm, s_pos, s_fn);
   affine_cells<D, P, TD, W>(a, a.el_offset + blockIdx.x * W, a.el_offset + a.nEl, gridDim.x * W, sm, s_pos, s_fn);
@@ -737,7 +748,7 @@ __global__ __launch_bounds__(64 * W) void hdg_affine_batch_kernel(const ElemArgs
                                                                   const BatchItem* __restrict__ items, int nitems) {
   using L = ALay<2, P, false>;
   extern __shared__ double sm[];
-  __shared__ int s_pos[L::NF * L::NN];
+  __shared__ int s_pos[L::NN];
   __shared__ int s_fn[L::NF * L::NNF];
   if (blockIdx.x >= nitems) return;
   // every problem of one order shares the reference tables: stage them once
