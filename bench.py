@@ -6,10 +6,11 @@ manufactured u = e^{x+y+z} sin(pi x) sin(pi y) sin(pi z), upwind
 double-valued tau.  One step = one pass of [element assembly + condensation +
 Dirichlet rows + BSR scatter] over all elements, inputs resident in HBM.
 value = u-DOFs (cells x nodes per cell) per second over the K timed steps.
-The trace solve (BiCGStab and FGMRES(30), block Jacobi, to a true FP64
-relative residual of 1e-10) is timed after the steps and reported beside the
-main number, with the learned preconditioner against block Jacobi on the
-2-D GRF-coefficient workload (256 x 256 x 2 triangles, p = 3).
+The trace solve (BiCGStab and FGMRES(30) with point Jacobi, block Jacobi and
+block ILU(0), to a true FP64 relative residual of 1e-10) is timed after the
+steps and reported beside the main number, with the learned preconditioner
+against block Jacobi and ILU(0) on the 2-D GRF-coefficient workload
+(256 x 256 x 2 triangles, p = 3).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 24] [--no-cpu]
        python bench.py --copy-calibration MiB   (a plain device copy of known
@@ -78,35 +79,57 @@ def cpu_baseline(n_cpu, p, threads):
                       % (n_cpu, w.mesh.n_cells, best)}
 
 
-def solver_bytes_per_iter(n, nb, bs, nnzb, method):
-    """Algorithmic HBM bytes of one Krylov iteration with block Jacobi:
-    BiCGStab = 2 BSR SpMV (values, column ids, x and y once) + 2 block-diagonal
-    applies + 21 vector passes; FGMRES at Krylov dimension ~15 = 1 SpMV + 1
-    apply + 2 x (2k + 3) vector passes (two Gram-Schmidt sweeps)."""
+PC_NAMES = {'jacobi': 1, 'block-jacobi': 2, 'learned': 3, 'ilu0': 4, 'ilu0-natural': 5}
+
+
+def solver_bytes_per_iter(n, nb, bs, nnzb, method, pc='block-jacobi'):
+    """Algorithmic HBM bytes of one Krylov iteration: BiCGStab = 2 BSR SpMV
+    (values, column ids, x and y once) + 2 preconditioner applies + 21 vector
+    passes; FGMRES at Krylov dimension ~15 = 1 SpMV + 1 apply + 2 x (2k + 3)
+    vector passes (two Gram-Schmidt sweeps).  Applies: point Jacobi 3 vector
+    passes; block Jacobi the inverse diagonal blocks + 2 passes; ILU(0) every
+    factor block (FP64) with its column id + the inverse pivot blocks + 4
+    passes; learned the FP32 BSR matrix + column ids + 2 passes."""
     spmv = nnzb * bs * bs * 8 + nnzb * 4 + 2 * n * 8
-    bj = nb * bs * bs * 8 + 2 * n * 8
+    apply = {'jacobi': 3 * n * 8, 'block-jacobi': nb * bs * bs * 8 + 2 * n * 8,
+             'ilu0': nnzb * bs * bs * 8 + nnzb * 8 + nb * bs * bs * 8 + 4 * n * 8,
+             'learned': nnzb * bs * bs * 4 + nnzb * 4 + 2 * n * 8}[pc.replace('-natural', '')]
     if method == 'bicgstab':
-        return 2 * spmv + 2 * bj + 21 * n * 8
-    return spmv + bj + 2 * (2 * 15 + 3) * n * 8
+        return 2 * spmv + 2 * apply + 21 * n * 8
+    return spmv + apply + 2 * (2 * 15 + 3) * n * 8
+
+
+def timed_solve(hf, prob, method, pc, maxits, warm=True):
+    """One solve to a true residual of 1e-10 (after an untimed warm-up solve
+    that allocates the Krylov work space), wall time in ms."""
+    from hyperfox_amd.lpc import pc_setup_ms
+    prob.set_krylov(hf.KRYLOV_BICGSTAB if method == 'bicgstab' else hf.KRYLOV_FGMRES)
+    if warm:
+        prob.solve(pc=PC_NAMES[pc], rtol=1e-10, maxits=min(maxits, 50), restart=30)
+    t = time.perf_counter()
+    its, reason, relres = prob.solve(pc=PC_NAMES[pc], rtol=1e-10, maxits=maxits, restart=30)
+    ms = (time.perf_counter() - t) * 1e3
+    return {'method': method, 'pc': pc, 'iters': its, 'reason': reason, 'converged': reason == 2,
+            'true_relres': relres, 'ms': ms, 'pc_setup_ms': pc_setup_ms(prob)}
 
 
 def trace_solves(hf, pr, prob, w):
-    """Trace solve of the benchmark problem to a true residual of 1e-10 with
-    block Jacobi: BiCGStab (the default here) and the reference's GMRES(30)
-    (as FGMRES)."""
+    """Trace solve of the benchmark problem to a true FP64 relative residual
+    of 1e-10: BiCGStab with block Jacobi and block ILU(0) (multicolour order),
+    and the reference's GMRES(30) (as FGMRES) with point Jacobi (PETSc's
+    PCJACOBI), block Jacobi and ILU(0)."""
     n_dofs, nnz, nnzb = prob.trace_sizes()
     bs = prob.sizes['nNf']
     nb = n_dofs // bs
     out = {}
-    for name, kry in (('bicgstab', hf.KRYLOV_BICGSTAB), ('fgmres30', hf.KRYLOV_FGMRES)):
-        prob.set_krylov(kry)
-        t = time.perf_counter()
-        its, reason, relres = prob.solve(pc=hf.PC_BJACOBI, rtol=1e-10, maxits=20000, restart=30)
-        ms = (time.perf_counter() - t) * 1e3
-        by = solver_bytes_per_iter(n_dofs, nb, bs, nnzb, name) * its
-        out[name] = {'pc': 'block-jacobi', 'iters': its, 'reason': reason, 'true_relres': relres, 'ms': ms,
-                     'algorithmic_GB': by / 1e9, 'achieved_GBs': by / (ms * 1e-3) / 1e9}
-        print('# solve %s: %d its, relres %.2e, %.1f ms' % (name, its, relres, ms), file=sys.stderr, flush=True)
+    for method, pc in (('bicgstab', 'block-jacobi'), ('bicgstab', 'ilu0'), ('fgmres30', 'jacobi'),
+                       ('fgmres30', 'block-jacobi'), ('fgmres30', 'ilu0')):
+        r = timed_solve(hf, prob, method, pc, 20000)
+        by = solver_bytes_per_iter(n_dofs, nb, bs, nnzb, method, pc) * r['iters']
+        r.update(algorithmic_GB=by / 1e9, achieved_GBs=by / (r['ms'] * 1e-3) / 1e9)
+        out['%s+%s' % (method, pc)] = r
+        print('# solve %s + %s: %d its, relres %.2e, %.1f ms' % (method, pc, r['iters'], r['true_relres'], r['ms']),
+              file=sys.stderr, flush=True)
     prob.set_krylov(hf.KRYLOV_BICGSTAB)
     prob.solve(pc=hf.PC_BJACOBI, rtol=1e-10, maxits=20000)
     u, q, lam = prob.solution()
@@ -116,22 +139,24 @@ def trace_solves(hf, pr, prob, w):
     return out
 
 
-def learned_vs_plain(hf, pr, n):
+def learned_vs_plain(hf, pr, n, fgmres_maxits=2000):
     """Learned-preconditioner workload: 2-D HDG diffusion, n x n x 2 jittered
     triangles, p = 3, k = exp(GRF(ell 0.1, sigma 1)), f = 1, held-out GRF
-    seeds; BiCGStab to a true residual of 1e-10 with block Jacobi and with the
-    trained network (BF16 MFMA inference; set-up time included)."""
+    seeds.  BiCGStab to a true residual of 1e-10 with block Jacobi, with the
+    trained network's correction of block Jacobi (BF16 MFMA inference; set-up
+    time included) and with block ILU(0); FGMRES(30) with the same three
+    preconditioners, capped at `fgmres_maxits` iterations (on these
+    coefficient contrasts restarted GMRES stagnates far above 1e-10)."""
     path = os.path.join(ROOT, 'hyperfox_amd', 'data', 'lpc_grf2d_p3.npy')
     if not os.path.exists(path):
         return {'skipped': 'no trained network at hyperfox_amd/data/lpc_grf2d_p3.npy'}
-    from hyperfox_amd.lpc import LearnedPreconditioner, pc_setup_ms
+    from hyperfox_amd.lpc import LearnedPreconditioner
     net = LearnedPreconditioner(4)
     net.load(path)
     mesh = hf.Mesh.structured(2, 3, n, 0.1, 2024)
     prob = hf.HDGProblem(2, 3)
-    prob.set_krylov(hf.KRYLOV_BICGSTAB)
     net.attach(prob, bf16=True)
-    rows = []
+    rows, fg = [], []
     for i, seed in enumerate((900000 + n * 1000 + 500, 900000 + n * 1000 + 501)):
         w = pr.diffusion_grf_2d(n, 3, seed, mesh=mesh)
         if i == 0:
@@ -139,27 +164,40 @@ def learned_vs_plain(hf, pr, n):
         else:
             prob.set_fields(w.tau, w.tau_sides, w.D, w.ndD, w.b, w.dirichlet)
         prob.assemble()
-        prob.solve(pc=hf.PC_BJACOBI, rtol=1e-10, maxits=50000)  # warm-up (allocations)
-        t = time.perf_counter()
-        ib, rb, relb = prob.solve(pc=hf.PC_BJACOBI, rtol=1e-10, maxits=50000)
-        tb = (time.perf_counter() - t) * 1e3
-        lam_b = prob.solution()[2].copy()
-        prob.solve(pc=hf.PC_LEARNED, rtol=1e-10, maxits=50000)
-        t = time.perf_counter()
-        il, rl, rell = prob.solve(pc=hf.PC_LEARNED, rtol=1e-10, maxits=50000)
-        tl = (time.perf_counter() - t) * 1e3
-        lam_l = prob.solution()[2]
-        rows.append({'seed': seed, 'plain_iters': ib, 'plain_ms': tb, 'plain_relres': relb, 'learned_iters': il,
-                     'learned_ms': tl, 'learned_setup_ms': pc_setup_ms(prob), 'learned_relres': rell,
-                     'reasons': [rb, rl],
-                     'solution_rel_diff': float(np.abs(lam_l - lam_b).max() / np.abs(lam_b).max())})
-        print('# learned vs plain n=%d seed %d: plain %d its %.1f ms, learned %d its %.1f ms' % (
-            n, seed, ib, tb, il, tl), file=sys.stderr, flush=True)
+        runs = {}
+        for pc in ('block-jacobi', 'learned', 'ilu0'):
+            runs[pc] = timed_solve(hf, prob, 'bicgstab', pc, 50000)
+            runs[pc]['lambda'] = prob.solution()[2].copy()
+        lam_b = runs['block-jacobi'].pop('lambda')
+        row = {'seed': seed}
+        for pc in ('learned', 'ilu0'):
+            lam = runs[pc].pop('lambda')
+            runs[pc]['solution_rel_diff_vs_block_jacobi'] = float(np.abs(lam - lam_b).max() / np.abs(lam_b).max())
+        row.update(runs)
+        rows.append(row)
+        print('# learned vs plain n=%d seed %d: BiCGStab block Jacobi %d its %.1f ms, learned %d its %.1f ms, '
+              'ILU(0) %d its %.1f ms' % (n, seed, runs['block-jacobi']['iters'], runs['block-jacobi']['ms'],
+                                         runs['learned']['iters'], runs['learned']['ms'], runs['ilu0']['iters'],
+                                         runs['ilu0']['ms']), file=sys.stderr, flush=True)
+        if i == 0:
+            for pc in ('block-jacobi', 'learned', 'ilu0'):
+                r = timed_solve(hf, prob, 'fgmres30', pc, fgmres_maxits, warm=False)
+                r['seed'] = seed
+                fg.append(r)
+                print('# FGMRES(30) + %s: %d its, relres %.2e, %.1f ms' % (pc, r['iters'], r['true_relres'], r['ms']),
+                      file=sys.stderr, flush=True)
+
+    def ratio(a, b, key):
+        return float(np.mean([r[a][key] / r[b][key] for r in rows]))
+
     return {'workload': 'HDG 2D diffusion, %dx%dx2=%d tris, p=3, k=exp(GRF ell=0.1 sigma=1), f=1' % (n, n, 2 * n * n),
             'trace_dofs': int(mesh.n_trace), 'method': 'BiCGStab FP64, true relres 1e-10', 'plain_pc': 'block-jacobi',
             'learned_pc': 'block-jacobi + GNN correction (BF16 MFMA inference)', 'runs': rows,
-            'iters_ratio': float(np.mean([r['plain_iters'] / r['learned_iters'] for r in rows])),
-            'time_ratio': float(np.mean([r['plain_ms'] / r['learned_ms'] for r in rows]))}
+            'iters_ratio': ratio('block-jacobi', 'learned', 'iters'),
+            'time_ratio': ratio('block-jacobi', 'learned', 'ms'),
+            'ilu0_iters_ratio': ratio('block-jacobi', 'ilu0', 'iters'),
+            'ilu0_time_ratio': ratio('block-jacobi', 'ilu0', 'ms'),
+            'fgmres30_capped': fg}
 
 
 def batched(hf, pr, count=64, seed=7):

@@ -6,11 +6,12 @@ for the tests and the benchmark; it has no CPU fallback: importing
 ``hyperfox_amd.lib`` raises if the shared library has not been built.
 """
 from ._lib import lib, LIB_PATH, HfxError  # noqa: F401
-from .hdg import Mesh, HDGProblem, Batch, refel_sizes, refel_tables, cubature  # noqa: F401
+from .hdg import Mesh, HDGProblem, Batch, refel_sizes, refel_tables, cubature, ilu_analyse  # noqa: F401
 from .linalg import HipLinAlgebra, HipOpts  # noqa: F401
 
 MODEL_LAPLACE = 0
 MODEL_DIFFUSION_SOURCE = 1
 MODEL_CDR_SOURCE = 2
-PC_NONE, PC_JACOBI, PC_BJACOBI, PC_LEARNED = 0, 1, 2, 3
+PC_NONE, PC_JACOBI, PC_BJACOBI, PC_LEARNED, PC_ILU0, PC_ILU0_NATURAL = 0, 1, 2, 3, 4, 5
+ILU_MULTICOLOUR, ILU_NATURAL = 0, 1
 KRYLOV_FGMRES, KRYLOV_BICGSTAB = 0, 1

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -280,6 +281,7 @@ extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
   static std::atomic<unsigned long long> serial{0};
   h->mesh_serial = ++serial;
   h->lpc_state.reset();
+  h->ilu.reset();
   h->mesh = mm->m;
   h->mesh.order = h->order;
   h->affine = mesh_is_affine(*h->re, h->mesh);
@@ -484,10 +486,13 @@ extern "C" int hfx_hdg_get_trace_csr(hfx_hdg* h, long long* rowptr, int* cols, d
 
 namespace {
 
-void do_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, hfx::KrylovInfo& info) {
+// Set up preconditioner `pc` on the assembled trace matrix; returns its apply
+// (empty for none).
+std::function<void(const double*, double*)> make_pc(hfx_hdg* h, int pc) {
   const long long n = h->A.nrows();
   hipStream_t s = h->stream;
   std::function<void(const double*, double*)> M;
+  const auto t0 = std::chrono::steady_clock::now();
   if (pc == HFX_PC_JACOBI) {
     h->dinv.alloc(n);
     hfx::bsr_inv_point_diag(h->A, h->dinv.p, s);
@@ -500,9 +505,29 @@ void do_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, hfx::Kry
     hfx::lpc_prepare(h);
     const hfx::LpcInfer* st = h->lpc_state.get();
     M = [st, s](const double* in, double* out) { hfx::bsr_spmv_f32vals(st->g, st->mvals.p, in, out, s); };
+  } else if (pc == HFX_PC_ILU0 || pc == HFX_PC_ILU0_NATURAL) {
+    const int ord = pc == HFX_PC_ILU0 ? hfx::ILU_MULTICOLOUR : hfx::ILU_NATURAL;
+    if (!h->ilu || h->ilu->ordering != ord) {
+      h->ilu.reset(new hfx::Ilu());
+      h->ilu->setup(h->A.nb, h->A.bs, h->pat.rowptr.data(), h->pat.col.data(), ord, s);
+    }
+    h->ilu->factor(h->A, s);
+    hfx::Ilu* f = h->ilu.get();
+    M = [f, s](const double* in, double* out) { f->apply(in, out, s); };
   } else if (pc != HFX_PC_NONE) {
     throw std::runtime_error("hfx_hdg_solve: unknown preconditioner");
   }
+  if (pc != HFX_PC_LEARNED) {  // the learned set-up times itself
+    HFX_HIP(hipStreamSynchronize(s));
+    h->last_pc_setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return M;
+}
+
+void do_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, hfx::KrylovInfo& info) {
+  const long long n = h->A.nrows();
+  hipStream_t s = h->stream;
+  std::function<void(const double*, double*)> M = make_pc(h, pc);
   HFX_HIP(hipMemsetAsync(h->lambda.p, 0, n * sizeof(double), s));
   hfx::KrylovOpts o;
   o.rtol = rtol;
@@ -543,6 +568,62 @@ extern "C" int hfx_hdg_solve(hfx_hdg* h, int pc, double rtol, int maxits, int re
   h->last_maxits = maxits;
   h->last_restart = restart;
   h->solved = true;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_pc_apply(hfx_hdg* h, int pc, const double* x, double* y) {
+  HFX_TRY
+  require(h && h->assembled, "hfx_hdg_pc_apply: system must be assembled first");
+  require(x && y, "hfx_hdg_pc_apply: null vector");
+  h->set_device();
+  const long long n = h->A.nrows();
+  hipStream_t s = h->stream;
+  std::function<void(const double*, double*)> M = make_pc(h, pc);
+  hfx::DevBuf<double> dx, dy;
+  dx.upload(x, n, s);
+  dy.alloc(n);
+  if (M) M(dx.p, dy.p);
+  else HFX_HIP(hipMemcpyAsync(dy.p, dx.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  dy.download(y, n, s);
+  HFX_HIP(hipStreamSynchronize(s));
+  HFX_END
+}
+
+extern "C" int hfx_hdg_ilu_factors(hfx_hdg* h, int ordering, double* lu, double* dinv, int* info) {
+  HFX_TRY
+  require(h && h->assembled, "hfx_hdg_ilu_factors: system must be assembled first");
+  require(ordering == hfx::ILU_MULTICOLOUR || ordering == hfx::ILU_NATURAL, "hfx_hdg_ilu_factors: unknown ordering");
+  h->set_device();
+  make_pc(h, ordering == hfx::ILU_MULTICOLOUR ? HFX_PC_ILU0 : HFX_PC_ILU0_NATURAL);
+  const hfx::Ilu& f = *h->ilu;
+  if (lu) f.lu.download(lu, f.lu.n, h->stream);
+  if (dinv) f.dinv.download(dinv, f.dinv.n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  if (info) {
+    info[0] = f.plan.ncolours;
+    info[1] = f.plan.nlev_f;
+    info[2] = f.plan.nlev_b;
+  }
+  HFX_END
+}
+
+extern "C" int hfx_ilu_analyse(int nb, const int* rowptr, const int* col, int ordering, int* rank, int* levf,
+                               int* levb, int* colour, long long* info) {
+  HFX_TRY
+  require(nb > 0 && rowptr && col, "hfx_ilu_analyse: empty pattern");
+  const hfx::IluPlan P = hfx::ilu_analyse(nb, rowptr, col, ordering);
+  for (int i = 0; i < nb; ++i) {
+    if (rank) rank[i] = P.rank[i];
+    if (levf) levf[i] = P.levf[i];
+    if (levb) levb[i] = P.levb[i];
+    if (colour) colour[i] = P.colour[i];
+  }
+  if (info) {
+    info[0] = P.ncolours;
+    info[1] = P.nlev_f;
+    info[2] = P.nlev_b;
+    info[3] = (long long)P.pslot.size();
+  }
   HFX_END
 }
 

@@ -11,6 +11,7 @@
 #include "../../include/hfx_hip.h"
 #include "hdg_element.h"
 #include "hfx_internal.h"
+#include "ilu.h"
 #include "krylov.h"
 #include "lpc.h"
 #include "mesh.h"
@@ -66,6 +67,7 @@ struct hfx_hdg {
   int lpc_bf16 = 1;
   std::unique_ptr<hfx::LpcInfer> lpc_state;
   double last_pc_setup_ms = 0.0;
+  std::unique_ptr<hfx::Ilu> ilu;  // ILU(0) schedule + factors for the current pattern
   int krylov = HFX_KRYLOV_FGMRES;
   // last solve options
   int last_pc = HFX_PC_BJACOBI, last_maxits = 10000, last_restart = 30;

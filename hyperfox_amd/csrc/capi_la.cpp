@@ -22,6 +22,7 @@
 
 #include "../../include/hfx_hip.h"
 #include "hfx_internal.h"
+#include "ilu.h"
 #include "krylov.h"
 
 namespace hfx {
@@ -63,6 +64,8 @@ struct hfx_la {
   hfx::BsrDev A;
   hfx::DevBuf<double> db, dx, dinv;
   bool dev_dirty = true;
+  std::unique_ptr<hfx::Ilu> ilu;  // point ILU(0) of the uploaded matrix
+  bool ilu_stale = true;
   int sol_low = 0, sol_high = 0;
 
   // Reduce the stash into the CSR: ADD sums every contribution, INSERT keeps
@@ -177,6 +180,7 @@ struct hfx_la {
     dx.alloc(n);
     HFX_HIP(hipStreamSynchronize(stream));
     dev_dirty = false;
+    ilu_stale = true;
   }
 };
 
@@ -225,7 +229,8 @@ extern "C" int hfx_la_configure(hfx_la* h, int solver_type, int pc, double rtol,
   if (!h->initialized) throw LaError("configure", "the object was not initialized before being configured.");
   if (solver_type != HFX_LA_GMRES && solver_type != HFX_LA_BCGS)
     throw LaError("configure", "the solver type must be GMRES or BiCGStab");
-  if (pc != HFX_PC_NONE && pc != HFX_PC_JACOBI) throw LaError("configure", "the preconditioner must be none or Jacobi");
+  if (pc != HFX_PC_NONE && pc != HFX_PC_JACOBI && pc != HFX_PC_ILU0 && pc != HFX_PC_ILU0_NATURAL)
+    throw LaError("configure", "the preconditioner must be none, Jacobi or ILU(0)");
   if (!(rtol > 0.0) || maxits <= 0 || restart < 1 || restart > 63)
     throw LaError("configure", "rtol must be > 0, maxits > 0 and restart in [1, 63]");
   h->solver = solver_type;
@@ -342,6 +347,21 @@ extern "C" int hfx_la_solve(hfx_la* h, double* x, int n, int* info, double* relr
       const double* dinv = h->dinv.p;
       const long long nn = h->n;
       M = [dinv, nn, s](const double* in, double* out) { hfx::apply_point_diag(nn, dinv, in, out, s); };
+    } else if (h->pc == HFX_PC_ILU0 || h->pc == HFX_PC_ILU0_NATURAL) {
+      const int ord = h->pc == HFX_PC_ILU0 ? hfx::ILU_MULTICOLOUR : hfx::ILU_NATURAL;
+      if (h->ilu_stale || !h->ilu || h->ilu->ordering != ord) {
+        h->ilu.reset(new hfx::Ilu());
+        try {
+          h->ilu->setup(h->n, 1, h->rowptr.data(), h->col.data(), ord, s);
+          h->ilu->factor(h->A, s);
+        } catch (const std::exception& e) {
+          h->ilu.reset();
+          throw LaError("solve", e.what());
+        }
+        h->ilu_stale = false;
+      }
+      hfx::Ilu* f = h->ilu.get();
+      M = [f, s](const double* in, double* out) { f->apply(in, out, s); };
     }
     hfx::KrylovOpts o;
     o.rtol = h->rtol;

@@ -204,13 +204,23 @@ __global__ void dots_kernel(long long n, int nd, const double* __restrict__ a0, 
 //   op 0: rho_prev = rho; rho = d0; rr = d1          (after the r update)
 //   op 1: rv = d0; alpha = rho / rv                   (after v = A p^)
 //   op 2: ts = d0; tt = d1; omega = ts / tt           (after t = A s^)
-__global__ void scalars_kernel(int G, int nd, int op, const double* __restrict__ partial, double* __restrict__ sc) {
-  __shared__ double red[RT / 64];
+// One workgroup of ST threads; the partials are read ST at a time with
+// several loads in flight per thread.
+constexpr int ST = 1024;
+__global__ __launch_bounds__(ST) void scalars_kernel(int G, int nd, int op, const double* __restrict__ partial,
+                                                     double* __restrict__ sc) {
+  __shared__ double red[ST / 64];
   double d[2] = {0.0, 0.0};
   for (int j = 0; j < nd; ++j) {
-    double acc = 0.0;
-    for (int b = threadIdx.x; b < G; b += RT) acc += partial[(size_t)j * G + b];
-    d[j] = block_sum(acc, red);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    const double* pj = partial + (size_t)j * G;
+    int b = threadIdx.x;
+    for (; b + 3 * ST < G; b += 4 * ST) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] += pj[b + u * ST];
+    }
+    for (; b < G; b += ST) acc[0] += pj[b];
+    d[j] = block_sum((acc[0] + acc[1]) + (acc[2] + acc[3]), red);
     __syncthreads();
   }
   if (threadIdx.x != 0) return;
@@ -260,6 +270,65 @@ __global__ void bicg_xr_kernel(long long n, const double* __restrict__ sc, const
   const double al = sc[SC_ALPHA], om = sc[SC_OMEGA];
   x[t] += al * ph[t] + om * sh[t];
   r[t] = s[t] - om * tv[t];
+}
+
+// y = A x (thread per row) with the block partials of w1 . y (and y . y when
+// `both`) in partial[blk] (and partial[gridDim.x + blk]): the SpMV of a
+// BiCGStab step and its dot products in one pass.
+template <int BS>
+__global__ __launch_bounds__(RT) void spmv_dot_kernel(int nb, const int* __restrict__ rowptr,
+                                                      const int* __restrict__ col, const double* __restrict__ vals,
+                                                      const double* __restrict__ x, double* __restrict__ y,
+                                                      const double* __restrict__ w1, int both,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[RT / 64];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a1 = 0.0, a2 = 0.0;
+  if (t < (long long)nb * BS) {
+    const int F = (int)(t / BS), r = (int)(t % BS);
+    double acc = 0.0;
+    for (int k = rowptr[F]; k < rowptr[F + 1]; ++k) {
+      const double* a = vals + ((size_t)k * BS + r) * BS;
+      const double* xv = x + (size_t)col[k] * BS;
+#pragma unroll
+      for (int c = 0; c < BS; ++c) acc += a[c] * xv[c];
+    }
+    y[t] = acc;
+    a1 = w1[t] * acc;
+    a2 = acc * acc;
+  }
+  const double s1 = block_sum(a1, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s1;
+  if (both) {
+    __syncthreads();
+    const double s2 = block_sum(a2, red);
+    if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = s2;
+  }
+}
+
+// x += alpha p^ + omega s^ ; r = s - omega t, with the block partials of
+// rhat . r and r . r
+__global__ __launch_bounds__(RT) void bicg_xr_dot_kernel(long long n, const double* __restrict__ sc,
+                                                         const double* __restrict__ ph, const double* __restrict__ sh,
+                                                         const double* __restrict__ s, const double* __restrict__ tv,
+                                                         const double* __restrict__ rh, double* __restrict__ x,
+                                                         double* __restrict__ r, double* __restrict__ partial) {
+  __shared__ double red[RT / 64];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double a1 = 0.0, a2 = 0.0;
+  if (t < n) {
+    const double al = sc[SC_ALPHA], om = sc[SC_OMEGA];
+    x[t] += al * ph[t] + om * sh[t];
+    const double rv = s[t] - om * tv[t];
+    r[t] = rv;
+    a1 = rh[t] * rv;
+    a2 = rv * rv;
+  }
+  const double s1 = block_sum(a1, red);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s1;
+  __syncthreads();
+  const double s2 = block_sum(a2, red);
+  if (threadIdx.x == 0) partial[gridDim.x + blockIdx.x] = s2;
 }
 
 __global__ void set_scalars_kernel(double* __restrict__ sc) {
@@ -490,7 +559,28 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
   auto dots = [&](int nd, const double* a0, const double* b0, const double* a1, const double* b1, int op) {
     hipLaunchKernelGGL(dots_kernel, dim3(G), dim3(RT), 0, s, n, nd, a0, b0, a1, b1, part.p);
     HFX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(RT), 0, s, G, nd, op, part.p, sc.p);
+    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(ST), 0, s, G, nd, op, part.p, sc.p);
+    HFX_HIP(hipGetLastError());
+  };
+  // fused path: SpMV / update kernels leave one partial per 256-row block
+  const int nblk_rows = ceil_div(n, RT);
+  DevBuf<double> fpart;
+  fpart.alloc((size_t)2 * nblk_rows);
+  auto spmv_dot = [&](const double* xin, double* yout, const double* w1, bool both) {
+#define HFX_BS(B)                                                                                          \
+  case B:                                                                                                  \
+    hipLaunchKernelGGL((spmv_dot_kernel<B>), dim3(nblk_rows), dim3(RT), 0, s, A.nb, A.rowptr.p, A.col.p,   \
+                       A.vals.p, xin, yout, w1, both ? 1 : 0, fpart.p);                                    \
+    break;
+    switch (A.bs) {
+      HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(10)
+      default: throw std::runtime_error("bicgstab: unsupported block size");
+    }
+#undef HFX_BS
+    HFX_HIP(hipGetLastError());
+  };
+  auto scalars = [&](int nd, int op) {
+    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(ST), 0, s, nblk_rows, nd, op, fpart.p, sc.p);
     HFX_HIP(hipGetLastError());
   };
   const int check_every = 8;
@@ -517,16 +607,17 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
     }
     HFX_HIP(hipGetLastError());
     apply_M(p.p, ph.p);
-    bsr_spmv(A, ph.p, v.p, nullptr, s);
-    dots(1, rh.p, v.p, nullptr, nullptr, 1);
+    spmv_dot(ph.p, v.p, rh.p, false);            // v = A p^, (rhat, v)
+    scalars(1, 1);
     hipLaunchKernelGGL(bicg_s_kernel, dim3(nblk), dim3(256), 0, s, n, sc.p, r.p, v.p, sv.p);
     HFX_HIP(hipGetLastError());
     apply_M(sv.p, sh.p);
-    bsr_spmv(A, sh.p, tv.p, nullptr, s);
-    dots(2, tv.p, sv.p, tv.p, tv.p, 2);
-    hipLaunchKernelGGL(bicg_xr_kernel, dim3(nblk), dim3(256), 0, s, n, sc.p, ph.p, sh.p, sv.p, tv.p, x, r.p);
+    spmv_dot(sh.p, tv.p, sv.p, true);            // t = A s^, (s, t), (t, t)
+    scalars(2, 2);
+    hipLaunchKernelGGL(bicg_xr_dot_kernel, dim3(nblk), dim3(RT), 0, s, n, sc.p, ph.p, sh.p, sv.p, tv.p, rh.p, x, r.p,
+                       fpart.p);
     HFX_HIP(hipGetLastError());
-    dots(2, rh.p, r.p, r.p, r.p, 0);
+    scalars(2, 0);                               // (rhat, r), (r, r)
     ++it;
     if (it % check_every == 0 || it >= o.maxits) {
       double hs[SC_N];

@@ -198,6 +198,25 @@ class HDGProblem:
         check(lib.hfx_hdg_solve(self._h, pc, rtol, maxits, restart, _ptr(info), _ptr(rel)))
         return int(info[0]), int(info[1]), float(rel[0])
 
+    def pc_apply(self, pc, x):
+        """y = M^-1 x for preconditioner `pc` set up on the assembled system."""
+        x = _f64(x)
+        y = np.zeros_like(x)
+        check(lib.hfx_hdg_pc_apply(self._h, pc, _ptr(x), _ptr(y)))
+        return y
+
+    def ilu_factors(self, ordering=0):
+        """Block ILU(0) factors (ordering 0 multicolour, 1 natural): lu
+        (nnzb, b, b) in trace-CSR block order, dinv (nFaces, b, b), and
+        (colours, forward levels, backward levels)."""
+        n, nnz, _ = self.trace_sizes()
+        b = self.sizes['nNf']
+        lu = np.zeros((nnz // (b * b), b, b))
+        dinv = np.zeros((n // b, b, b))
+        info = np.zeros(3, dtype=np.int32)
+        check(lib.hfx_hdg_ilu_factors(self._h, ordering, _ptr(lu), _ptr(dinv), _ptr(info)))
+        return lu, dinv, tuple(int(v) for v in info)
+
     def solution(self):
         s, ne, d = self.sizes, self.mesh.n_cells, self.dim
         u = np.zeros((ne, s['nN']))
@@ -210,6 +229,20 @@ class HDGProblem:
         ms = np.zeros(reps)
         check(lib.hfx_hdg_time(self._h, what, warmup, reps, _ptr(ms)))
         return ms
+
+
+def ilu_analyse(rowptr, col, ordering):
+    """Host analysis of a block pattern for ILU(0) (no GPU): dict of rank,
+    levf, levb, colour arrays and the counts (colours, forward levels,
+    backward levels, update pairs)."""
+    rowptr, col = _i32(rowptr), _i32(col)
+    nb = rowptr.size - 1
+    out = {k: np.zeros(nb, dtype=np.int32) for k in ('rank', 'levf', 'levb', 'colour')}
+    info = np.zeros(4, dtype=np.int64)
+    check(lib.hfx_ilu_analyse(nb, _ptr(rowptr), _ptr(col), ordering, _ptr(out['rank']), _ptr(out['levf']),
+                              _ptr(out['levb']), _ptr(out['colour']), _ptr(info)))
+    out.update(ncolours=int(info[0]), nlev_f=int(info[1]), nlev_b=int(info[2]), npairs=int(info[3]))
+    return out
 
 
 class Batch:

@@ -17,7 +17,7 @@ from ._lib import check, lib
 
 GMRES = 0
 BCGS = 1
-PC_NONE, PC_JACOBI = 0, 1
+PC_NONE, PC_JACOBI, PC_ILU0, PC_ILU0_NATURAL = 0, 1, 4, 5
 
 
 @dataclass

@@ -83,8 +83,20 @@ typedef struct hfx_hdg hfx_hdg;
  * HDGConvectionDiffusionReactionSource (src/model/HDG*.cpp). */
 enum { HFX_MODEL_LAPLACE = 0, HFX_MODEL_DIFFUSION_SOURCE = 1, HFX_MODEL_CDR_SOURCE = 2 };
 
-/* Trace preconditioners. */
-enum { HFX_PC_NONE = 0, HFX_PC_JACOBI = 1, HFX_PC_BJACOBI = 2, HFX_PC_LEARNED = 3 };
+/* Trace preconditioners: none, point Jacobi (PCJACOBI, the reference's
+ * default, PetscOpts.h:16), face-block Jacobi, the learned correction of
+ * face-block Jacobi, and block ILU(0) (PCILU, zero fill, on the face-block
+ * pattern) in a multicolour elimination order (rows of one colour share no
+ * block: as many sweep levels as colours) or in the natural face order
+ * (PETSc's default ordering; one sweep level per dependency level). */
+enum {
+  HFX_PC_NONE = 0,
+  HFX_PC_JACOBI = 1,
+  HFX_PC_BJACOBI = 2,
+  HFX_PC_LEARNED = 3,
+  HFX_PC_ILU0 = 4,
+  HFX_PC_ILU0_NATURAL = 5
+};
 
 /* Create a problem on GPU `device` for simplices of `dim` and `order`
  * (HDGSolver construction + initialize, Solver.cpp:5-12). */
@@ -160,6 +172,25 @@ int hfx_hdg_set_krylov(hfx_hdg* h, int method);
  * -9 nan); relres[0] = final true relative residual. */
 int hfx_hdg_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, int* info, double* relres);
 
+/* Set up preconditioner `pc` for the assembled trace matrix and apply it
+ * once: y = M^-1 x (host vectors of trace DOFs).  For tests and callers that
+ * drive their own iteration. */
+int hfx_hdg_pc_apply(hfx_hdg* h, int pc, const double* x, double* y);
+
+/* Block ILU(0) factors of the assembled trace matrix in `ordering` (0
+ * multicolour, 1 natural): lu (nnzb x nNf^2, the trace CSR's block order:
+ * earlier-column blocks L_ik, the diagonal A'_ii, later-column blocks
+ * Dinv_i U_ij), dinv (nFaces x nNf^2, inverse pivot blocks); either may be
+ * NULL.  info[0..2] = colours, forward levels, backward levels. */
+int hfx_hdg_ilu_factors(hfx_hdg* h, int ordering, double* lu, double* dinv, int* info);
+
+/* Host analysis of a block pattern for ILU(0) (no GPU): rank[nb] (position
+ * in the elimination order), levf[nb], levb[nb] (sweep levels) and colour[nb]
+ * (any may be NULL); info[0..3] = colours, forward levels, backward levels,
+ * update pairs of the factorisation. */
+int hfx_ilu_analyse(int nb, const int* rowptr, const int* col, int ordering, int* rank, int* levf, int* levb,
+                    int* colour, long long* info);
+
 /* Solution fields after solve (HDGSolver.cpp:741-775): u nCells*nN ("Solution"),
  * q nCells*nN*dim ("Flux"), lambda nFaces*nNf ("Trace").  Any may be NULL. */
 int hfx_hdg_get_solution(hfx_hdg* h, double* u, double* q, double* lambda);
@@ -197,7 +228,10 @@ void hfx_la_free(hfx_la* h);
 int hfx_la_initialize(hfx_la* h);
 
 /* LinAlgebraInterface::configure with the PetscOpts fields (PetscInterface.cpp:60-86,
- * PetscOpts.h:9-31): solver_type HFX_LA_GMRES or HFX_LA_BCGS, pc HFX_PC_NONE or HFX_PC_JACOBI,
+ * PetscOpts.h:9-31): solver_type HFX_LA_GMRES or HFX_LA_BCGS, pc HFX_PC_NONE, HFX_PC_JACOBI,
+ * HFX_PC_ILU0_NATURAL (PCILU: point ILU(0) in the natural row order) or
+ * HFX_PC_ILU0 (point ILU(0) in a multicolour order; a row without a diagonal
+ * entry or a zero pivot fails the solve),
  * rtol (on the true residual ||b - A x||_2 / ||b||_2), maxits, GMRES restart
  * (1..63; PETSc's default is 30).  Fails before initialize. */
 int hfx_la_configure(hfx_la* h, int solver_type, int pc, double rtol, int maxits, int restart);
@@ -345,8 +379,9 @@ int hfx_lpc_outputs(hfx_lpc* h, int sample, int bf16, float* cn, float* ce);
 /* Attach a network to a problem for HFX_PC_LEARNED (NULL detaches);
  * bf16 != 0 runs inference on BF16 MFMA, else FP32. */
 int hfx_hdg_set_lpc(hfx_hdg* h, hfx_lpc* lpc, int bf16);
-/* Host wall time of the last learned-preconditioner set-up (normalise +
- * network + M assembly), ms. */
+/* Host wall time of the last preconditioner set-up of hfx_hdg_solve, ms
+ * (learned: normalise + network + M assembly; ILU(0): analysis on a new
+ * pattern + factorisation; Jacobi: inverse diagonal). */
 int hfx_hdg_pc_setup_ms(hfx_hdg* h, double* ms);
 
 #ifdef __cplusplus
