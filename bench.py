@@ -139,6 +139,36 @@ def trace_solves(hf, pr, prob, w):
     return out
 
 
+def transient(hf, pr, prob, w, steps=3, dt=1e-2):
+    """Implicit Euler on the benchmark problem (the reference's Euler with
+    isExplicit = false on the u rows): from u(0) = the manufactured solution,
+    each step assembles with the previous step's solution field and solves
+    (BiCGStab + ILU(0), true residual 1e-10).  Reports per-step times and the
+    assembly rate with the scheme on (HIP events, element kernel + scatter)."""
+    a = w.mesh.arrays()
+    u0 = pr.u_c3(a['coords'][a['cells']])
+    prob.set_solution(u0)
+    prob.set_time_scheme(hf.TS_BEULER, dt)
+    prob.set_krylov(hf.KRYLOV_BICGSTAB)
+    rows = []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        prob.assemble()
+        t1 = time.perf_counter()
+        its, reason, relres = prob.solve(pc=hf.PC_ILU0, rtol=1e-10, maxits=20000)
+        t2 = time.perf_counter()
+        u = prob.solution()[0]
+        rows.append({'step': k + 1, 'assemble_ms': (t1 - t0) * 1e3, 'solve_ms': (t2 - t1) * 1e3, 'iters': its,
+                     'reason': reason, 'true_relres': relres, 'max_u': float(np.abs(u).max())})
+        print('# implicit Euler step %d: assemble %.2f ms, solve %d its %.1f ms' % (
+            k + 1, rows[-1]['assemble_ms'], its, rows[-1]['solve_ms']), file=sys.stderr, flush=True)
+    ms = float(np.median(prob.time(1, warmup=1, reps=5)))
+    dofs = w.mesh.n_cells * prob.sizes['nN']
+    prob.set_time_scheme(hf.TS_NONE)
+    return {'scheme': 'implicit Euler (Euler, isExplicit=false)', 'dt': dt, 'steps': rows,
+            'assembly_ms_with_scheme': ms, 'assembly_dof_per_s_with_scheme': dofs / (ms * 1e-3)}
+
+
 def learned_vs_plain(hf, pr, n, fgmres_maxits=2000):
     """Learned-preconditioner workload: 2-D HDG diffusion, n x n x 2 jittered
     triangles, p = 3, k = exp(GRF(ell 0.1, sigma 1)), f = 1, held-out GRF
@@ -330,6 +360,7 @@ def main():
     }
     if not args.no_solve:
         res['trace_solve'] = trace_solves(hf, pr, prob, w)
+        res['transient'] = transient(hf, pr, prob, w)
         if not args.no_learned:
             res['learned_vs_plain'] = learned_vs_plain(hf, pr, args.learned_n)
     if not args.no_batch:

@@ -48,6 +48,9 @@ _SIGS = {
     'hfx_hdg_get_trace_csr': ([_P] * 5, _I),
     'hfx_hdg_solve': ([_P, _I, _D, _I, _I, _P, _P], _I),
     'hfx_hdg_pc_apply': ([_P, _I, _P, _P], _I),
+    'hfx_hdg_set_fused_scatter': ([_P, _I], _I),
+    'hfx_hdg_set_time_scheme': ([_P, _I, _D], _I),
+    'hfx_hdg_set_solution': ([_P, _P], _I),
     'hfx_hdg_ilu_factors': ([_P, _I, _P, _P, _P], _I),
     'hfx_ilu_analyse': ([_I, _P, _P, _I, _P, _P, _P, _P, _P], _I),
     'hfx_hdg_get_solution': ([_P] * 4, _I),

@@ -306,6 +306,9 @@ extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
   h->A.diag.upload(h->pat.diag, s);
   h->A.contrib.upload(h->pat.contrib, s);
   h->A.face_cells.upload(h->pat.face_cells, s);
+  h->bpos.upload(hfx::cell_block_positions(h->pat, m.nCells, m.nFpc, m.cell2face.data()), s);
+  h->dside.alloc((size_t)2 * h->pat.nb * h->pat.bs * h->pat.bs);
+  h->last_fused = false;
   h->A.vals.alloc((size_t)h->A.nnzb * m.nNf * m.nNf);
   std::vector<uint8_t> dir(m.nFaces, 0);
   for (int f : m.boundary) dir[f] = 1;
@@ -320,12 +323,14 @@ extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
   h->rhs.alloc((size_t)m.nFaces * m.nNf);
   h->lambda.alloc((size_t)m.nFaces * m.nNf);
   h->u.alloc((size_t)m.nCells * nN);
+  HFX_HIP(hipMemsetAsync(h->u.p, 0, h->u.n * sizeof(double), s));  // "Solution" starts at 0
   h->q.alloc((size_t)m.nCells * nQ);
   h->g.alloc((size_t)m.nFaces * m.nNf);
   HFX_HIP(hipMemsetAsync(h->g.p, 0, h->g.n * sizeof(double), s));
   HFX_HIP(hipStreamSynchronize(s));
   h->has_mesh = true;
   h->has_fields = h->assembled = h->solved = false;
+  ++h->args_version;
   HFX_END
 }
 
@@ -335,6 +340,7 @@ extern "C" int hfx_hdg_set_model(hfx_hdg* h, int kind) {
   require(kind >= 0 && kind <= 2, "hfx_hdg_set_model: unknown model kind");
   h->kind = kind;
   h->assembled = h->solved = false;
+  ++h->args_version;
   HFX_END
 }
 
@@ -361,6 +367,7 @@ extern "C" int hfx_hdg_set_fields(hfx_hdg* h, const double* tau, int tau_sides, 
   HFX_HIP(hipStreamSynchronize(s));
   h->has_fields = true;
   h->assembled = h->solved = false;
+  ++h->args_version;
   HFX_END
 }
 
@@ -388,6 +395,7 @@ extern "C" int hfx_hdg_set_source(hfx_hdg* h, const double* f_ip, const double* 
   if (r_ip) h->r_ip.upload(r_ip, n, h->stream);
   HFX_HIP(hipStreamSynchronize(h->stream));
   h->assembled = h->solved = false;
+  ++h->args_version;
   HFX_END
 }
 
@@ -440,6 +448,9 @@ extern "C" int hfx_hdg_get_condensed(hfx_hdg* h, double* S, double* S0, double* 
   HFX_TRY
   require(h && h->assembled, "hfx_hdg_get_condensed: assemble first");
   h->set_device();
+  // the fused scatter does not store S: run the element kernel once more in
+  // its stored-S form (same device code up to the stores)
+  if (S && h->last_fused) h->run_element(true);
   if (S) h->S.download(S, h->S.n, h->stream);
   if (S0) h->S0.download(S0, h->S0.n, h->stream);
   if (U) h->U.download(U, h->U.n, h->stream);
@@ -538,6 +549,39 @@ void do_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, hfx::Kry
 }
 
 }  // namespace
+
+extern "C" int hfx_hdg_set_time_scheme(hfx_hdg* h, int scheme, double dt) {
+  HFX_TRY
+  require(h != nullptr, "hfx_hdg_set_time_scheme: null handle");
+  require(scheme == HFX_TS_NONE || scheme == HFX_TS_BEULER || scheme == HFX_TS_FEULER,
+          "hfx_hdg_set_time_scheme: unknown scheme");
+  require(scheme != HFX_TS_FEULER, "hfx_hdg_set_time_scheme: explicit Euler is not supported on the GPU path");
+  require(scheme == HFX_TS_NONE || (dt > 0.0 && std::isfinite(dt)),
+          "hfx_hdg_set_time_scheme: the time step needs to be set and it should not be 0");
+  h->ts_scheme = scheme;
+  h->ts_dt = scheme == HFX_TS_NONE ? 0.0 : dt;
+  h->assembled = h->solved = false;
+  ++h->args_version;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_solution(hfx_hdg* h, const double* u) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_set_solution: set the mesh first");
+  require(u != nullptr, "hfx_hdg_set_solution: null field");
+  h->set_device();
+  h->u.upload(u, h->u.n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_fused_scatter(hfx_hdg* h, int on) {
+  HFX_TRY
+  require(h != nullptr, "hfx_hdg_set_fused_scatter: null handle");
+  h->fuse_scatter = on != 0;
+  ++h->args_version;
+  HFX_END
+}
 
 extern "C" int hfx_hdg_set_krylov(hfx_hdg* h, int method) {
   HFX_TRY

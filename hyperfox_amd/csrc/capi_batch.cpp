@@ -36,6 +36,8 @@ struct hfx_batch {
   std::vector<hfx_hdg*> probs;
   int waves[NORD] = {0, 2, 2, 2, 2, 2}, tile[NORD] = {0, 64, 64, 64, 64, 64};
   bool items_dirty = true;
+  std::vector<bool> fused;  // per problem: fused scatter in the argument table
+  std::vector<unsigned long long> versions;  // per problem: args_version of the table
   std::vector<hfx::BatchItem> items[NORD];
   hfx::DevBuf<hfx::ElemArgs> dargs;
   hfx::DevBuf<hfx::BatchItem> ditems[NORD];
@@ -89,12 +91,16 @@ struct hfx_batch {
   }
 
   void build_items() {
+    for (size_t i = 0; i < probs.size() && !items_dirty; ++i)
+      if (i >= versions.size() || versions[i] != probs[i]->args_version) items_dirty = true;
     if (!items_dirty) return;
     std::vector<hfx::ElemArgs> args;
+    fused.clear();
+    versions.clear();
     for (hfx_hdg* h : probs) {
-      hfx::ElemArgs a = h->elem_args();
-      a.tabs = h->tabs_aff.p;
-      args.push_back(a);
+      args.push_back(h->affine_args(false));
+      fused.push_back(h->fuse_scatter);
+      versions.push_back(h->args_version);
     }
     dargs.upload(args, stream);
     for (int p = 1; p < NORD; ++p) build_order(p);
@@ -128,8 +134,7 @@ struct hfx_batch {
 
   void run_all() {
     run_kernels();
-    for (hfx_hdg* h : probs)
-      hfx::launch_scatter(h->A, h->mesh.nFpc, h->S.p, h->S0.p, h->invperm.p, h->is_dir.p, h->g.p, h->rhs.p, stream);
+    for (size_t i = 0; i < probs.size(); ++i) probs[i]->scatter_on(stream, fused[i]);
   }
 
   void capture() {
@@ -156,6 +161,7 @@ struct hfx_batch {
     for (hfx_hdg* h : probs) {
       h->assembled = true;
       h->solved = false;
+      h->last_fused = h->fuse_scatter;  // build_items ran with the current flags
     }
   }
 };
@@ -370,11 +376,8 @@ extern "C" int hfx_batch_time(hfx_batch* b, int what, int warmup, int reps, doub
     else if (what == 2) HFX_HIP(hipGraphLaunch(b->exec, b->stream));
     else {
       for (hfx_hdg* h : b->probs) {
-        hfx::ElemArgs a = h->elem_args();
-        a.tabs = h->tabs_aff.p;
-        hfx::launch_affine(2, h->order, false, h->affine_waves(), a, h->grid_cap, b->stream);
-        hfx::launch_scatter(h->A, h->mesh.nFpc, h->S.p, h->S0.p, h->invperm.p, h->is_dir.p, h->g.p, h->rhs.p,
-                            b->stream);
+        hfx::launch_affine(2, h->order, false, h->affine_waves(), h->affine_args(false), h->grid_cap, b->stream);
+        h->scatter_on(b->stream, h->fuse_scatter);
       }
     }
     HFX_HIP(hipEventRecord(e1, b->stream));

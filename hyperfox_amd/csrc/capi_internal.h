@@ -60,6 +60,19 @@ struct hfx_hdg {
   hfx::DevBuf<uint8_t> is_dir;
   hfx::DevBuf<double> S, S0, U, Q, U0, Q0, rhs, lambda, dinv, u, q;
   hfx::BsrPattern pat;
+  // fused trace scatter (affine kernel): per-cell BSR block index of each face
+  // pair, and the two cell-side contributions to each diagonal face block
+  hfx::DevBuf<int> bpos;
+  hfx::DevBuf<double> dside;
+  bool fuse_scatter = true;  // hfx_hdg_set_fused_scatter
+  // time scheme on the u rows (hfx_hdg_set_time_scheme); u_old is the
+  // problem's current "Solution" field u
+  int ts_scheme = HFX_TS_NONE;
+  double ts_dt = 0.0;
+  // bumped by every call that changes the element kernel's arguments, so
+  // that a batch holding a copy of them rebuilds it
+  unsigned long long args_version = 0;
+  bool last_fused = false;   // the last element launch wrote the BSR, not S
   hfx::BsrDev A;
   // learned preconditioner (attached network, per-problem inference state)
   unsigned long long mesh_serial = 0;  // distinct for every set_mesh in the process
@@ -97,6 +110,11 @@ struct hfx_hdg {
     a.f_ip = f_ip.p;
     a.r_ip = r_ip.p;
     a.S = S.p; a.S0 = S0.p; a.U = U.p; a.Q = Q.p; a.U0 = U0.p; a.Q0 = Q0.p;
+    if (ts_scheme == HFX_TS_BEULER) {
+      a.ts_dt = ts_dt;
+      a.ts_idt = 1.0 / ts_dt;
+      a.u_old = u.p;
+    }
     return a;
   }
   int threads() const {
@@ -115,17 +133,36 @@ struct hfx_hdg {
       if (hfx::affine_lds_bytes(dim, order, false, w) + 4096 <= 160 * 1024) return w;
     return 1;
   }
-  void run_element() {
+  // arguments of the affine kernel; with the fused scatter unless S must be stored
+  hfx::ElemArgs affine_args(bool store_S) {
     hfx::ElemArgs a = elem_args();
+    a.tabs = tabs_aff.p;
+    if (fuse_scatter && !store_S) {
+      a.bvals = A.vals.p;
+      a.dside = dside.p;
+      a.bpos = bpos.p;
+      a.is_dir = is_dir.p;
+    }
+    return a;
+  }
+  // element kernel; store_S forces the stored-S path (exports)
+  void run_element(bool store_S = false) {
     if (use_affine()) {
-      a.tabs = tabs_aff.p;
+      const hfx::ElemArgs a = affine_args(store_S);
       hfx::launch_affine(dim, order, false, affine_waves(), a, grid_cap, stream);
+      last_fused = a.bvals != nullptr;
     } else {
-      hfx::launch_element(dim, order, a, threads(), grid_cap, stream);
+      hfx::launch_element(dim, order, elem_args(), threads(), grid_cap, stream);
+      last_fused = false;
     }
   }
-  void run_scatter() {
-    hfx::launch_scatter(A, mesh.nFpc, S.p, S0.p, invperm.p, is_dir.p, g.p, rhs.p, stream);
+  // the trace-system pass that completes the last element launch
+  void scatter_on(hipStream_t s, bool fused) {
+    if (fused)
+      hfx::launch_scatter_diag(A, mesh.nFpc, dside.p, S0.p, invperm.p, is_dir.p, g.p, rhs.p, s);
+    else
+      hfx::launch_scatter(A, mesh.nFpc, S.p, S0.p, invperm.p, is_dir.p, g.p, rhs.p, s);
   }
+  void run_scatter() { scatter_on(stream, last_fused); }
 };
 

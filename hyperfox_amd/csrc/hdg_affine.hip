@@ -77,7 +77,16 @@ struct ALay {
   static constexpr int O_VX = O_FG + NF * (D + 1);
   static constexpr int O_FWT = O_VX + (D + 1) * D;  // face-IP weights (kept for the b.n columns)
   static constexpr int O_RD = O_FWT + NF * NIPF * NS;  // 1 / pivot of each K^-1 row
-  static constexpr int O_OV = O_RD + NN;
+  // per-cell indices (ints) for the fused trace scatter, gathered with the
+  // cell: face-node positions, BSR block of each face pair, face ids, side |
+  // Dirichlet flags
+  static constexpr int IX_PERM = 0;
+  static constexpr int IX_BPOS = IX_PERM + NF * NNF;
+  static constexpr int IX_FACE = IX_BPOS + NF * NF;
+  static constexpr int IX_FLAG = IX_FACE + NF;
+  static constexpr int NIX = IX_FLAG + NF;
+  static constexpr int O_IX = O_RD + NN;
+  static constexpr int O_OV = O_IX + (NIX + 1) / 2;
   // overlay, phase 1
   static constexpr int O_DN = O_OV;
   static constexpr int O_BN = O_DN + NN * ND;
@@ -182,6 +191,8 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
   double* WF = ws + L::O_WF;
   double* FWT = ws + L::O_FWT;
   double* RB = ws + L::O_RB;
+  int* IX = reinterpret_cast<int*>(ws + L::O_IX);
+  const bool fused = a.bvals != nullptr;
 
   for (int e = e_first + wv; e < e_end; e += e_stride) {
     const int* cell = a.cells + (size_t)e * NN;
@@ -201,10 +212,25 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       const int side = (a.tau_sides == 2 && a.face2cell[2 * (size_t)fc] != e) ? 1 : 0;
       const int p = a.face_perm[((size_t)e * NF + f) * NNF + j];
       TAU[t] = a.tau[((size_t)fc * NNF + p) * a.tau_sides + side];
+      IX[L::IX_PERM + t] = p;
+    }
+    if (fused) {
+      for (int t = lane; t < NF * NF; t += 64) IX[L::IX_BPOS + t] = a.bpos[(size_t)e * NF * NF + t];
+      for (int t = lane; t < NF; t += 64) {
+        const int fc = a.cell2face[(size_t)e * NF + t];
+        IX[L::IX_FACE + t] = fc;
+        IX[L::IX_FLAG + t] = (a.face2cell[2 * (size_t)fc] == e ? 0 : 1) | (a.is_dir && a.is_dir[fc] ? 2 : 0);
+      }
     }
     for (int t = lane; t < NIP; t += 64) {
-      FQ[t] = (a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + t] : 0.0;
-      RQ[t] = a.hasR ? a.r_ip[(size_t)e * NIP + t] : 0.0;
+      double fq = (a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + t] : 0.0;
+      if (a.u_old) {  // implicit Euler: + u_old(x_q) / dt
+        double uo = 0.0;
+        for (int j = 0; j < NN; ++j) uo += PHI[t * NN + j] * a.u_old[(size_t)e * NN + j];
+        fq += a.ts_idt * uo;
+      }
+      FQ[t] = fq;
+      RQ[t] = (a.hasR ? a.r_ip[(size_t)e * NIP + t] : 0.0) + a.ts_idt;
     }
     wave_sync();
     // ------------------------------------------------ element geometry
@@ -683,11 +709,20 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       constexpr int NT = (NCOL + 15) / 16, KF = 2 * NNF, KS = (KF + 3) / 4;
       const int li = lane & 15, lk = lane >> 4;
       static_assert(NNF <= 16, "one row tile per face");
+      // fused trace scatter: no S; each face block goes to its place in the
+      // BSR values in face-node order (off-diagonal blocks have this cell as
+      // their only contributor; Dirichlet rows get zeros), the diagonal block
+      // to this cell's side of dside, summed by the diagonal pass
 #pragma unroll 1
       for (int f = 0; f < NF; ++f) {
         const double* fmt = FM + (f * NSF + 0) * NNF * NNF;
         const double* fmb = FM + (f * NSF + 1) * NNF * NNF;
         const double* fmd = FM + (f * NSF + 2) * NNF * NNF;
+        const int Ff = fused ? IX[L::IX_FACE + f] : 0;
+        const int flg = fused ? IX[L::IX_FLAG + f] : 0;
+        const bool dirf = (flg & 2) != 0;
+        const int sidef = flg & 1;
+        const int* permf = IX + L::IX_PERM + f * NNF;
 #pragma unroll 1
         for (int nt = 0; nt < NT; ++nt) {
           const int cc = nt * 16 + li;
@@ -708,6 +743,12 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
             const bool trc = cc < NL;
             const int gg = trc ? cc / NNF : 0, bb = trc ? cc % NNF : 0;
             double* Se = trc ? a.S + (size_t)e * NL * NL : a.S0 + (size_t)e * NL;
+            double* blk = nullptr;  // fused: this lane's block, column already placed
+            if (fused && trc) {
+              const int pc = IX[L::IX_PERM + gg * NNF + bb];
+              blk = (gg == f) ? a.dside + (size_t)(2 * Ff + sidef) * NNF * NNF + pc
+                              : a.bvals + (size_t)IX[L::IX_BPOS + f * NF + gg] * NNF * NNF + pc;
+            }
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int aa = 4 * r + lk;
@@ -715,7 +756,8 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
               double sv = acc[r];
               if (trc) {
                 if (f == gg) sv += -fmt[aa * NNF + bb] + fmb[aa * NNF + bb];
-                Se[(size_t)(f * NNF + aa) * NL + cc] = sv;
+                if (fused) blk[(size_t)permf[aa] * NNF] = (dirf && gg != f) ? 0.0 : sv;
+                else Se[(size_t)(f * NNF + aa) * NL + cc] = sv;
               } else {
                 Se[f * NNF + aa] = -sv;
               }

@@ -55,6 +55,23 @@ struct ElemArgs {
   double *S = nullptr, *S0 = nullptr, *U = nullptr, *Q = nullptr, *U0 = nullptr, *Q0 = nullptr;
   // optional: full local matrices (NLOC x NLOC) and RHS instead of condensation
   double *Kloc = nullptr, *Floc = nullptr;
+  // optional (affine kernel): fused trace scatter instead of storing S.  The
+  // off-diagonal face blocks go straight into the BSR values bvals at block
+  // bpos[(e*NF + f)*NF + g] in face-node order (zero rows on Dirichlet faces
+  // is_dir); the diagonal block of face F from its side s (0 when
+  // face2cell[2F] == e) goes to dside[(2F + s)*NNF*NNF ...], summed by
+  // launch_scatter_diag.
+  double* bvals = nullptr;
+  double* dside = nullptr;
+  const int* bpos = nullptr;
+  const uint8_t* is_dir = nullptr;
+  // optional: implicit Euler on the u rows (Euler::apply, Euler.cpp:18-37):
+  // with ts_dt > 0 the element matrix of the model plus M / dt in Suu and
+  // M u_old / dt in F_u, u_old (nEl x nN, the "Solution" field) -- the
+  // reference's dt-scaled rows divided by dt, which condenses to the same
+  // blocks.  The local-matrix export scales the u rows back by dt.
+  double ts_dt = 0.0, ts_idt = 0.0;
+  const double* u_old = nullptr;
 };
 
 // Launch the element kernel for (dim, order) on `count` elements starting at

@@ -210,8 +210,14 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
       TAU[t] = a.tau[((size_t)fc * NNF + p) * a.tau_sides + side];
     }
     for (int t = tid; t < NIP; t += NT) {
-      FQ[t] = (a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + t] : 0.0;
-      RQ[t] = a.hasR ? a.r_ip[(size_t)e * NIP + t] : 0.0;
+      double fq = (a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + t] : 0.0;
+      if (a.u_old) {  // implicit Euler: + u_old(x_q) / dt
+        double uo = 0.0;
+        for (int j = 0; j < NN; ++j) uo += PHI[t * NN + j] * a.u_old[(size_t)e * NN + j];
+        fq += a.ts_idt * uo;
+      }
+      FQ[t] = fq;
+      RQ[t] = (a.hasR ? a.r_ip[(size_t)e * NIP + t] : 0.0) + a.ts_idt;
     }
     __syncthreads();
     // ---------------------------------------------- IP geometry (element)
@@ -395,10 +401,11 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
             if (g == f) v = -FM[((f * NS + 0) * NNF + aa) * NNF + b] + FM[((f * NS + 1) * NNF + aa) * NNF + b];
           }
         }
-        K[t] = v;
+        // implicit Euler: the reference's u rows are dt x (these rows)
+        K[t] = (a.ts_dt > 0.0 && r < OQ) ? a.ts_dt * v : v;
       }
       double* F = a.Floc + (size_t)(e - a.el_offset) * NLOC;
-      for (int t = tid; t < NLOC; t += NT) F[t] = t < NN ? FU[t] : 0.0;
+      for (int t = tid; t < NLOC; t += NT) F[t] = t < NN ? (a.ts_dt > 0.0 ? a.ts_dt * FU[t] : FU[t]) : 0.0;
       __syncthreads();
       continue;
     }

@@ -3,6 +3,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "bsr_row.h"
 #include "ilu.h"
 
 namespace hfx {
@@ -142,12 +143,8 @@ __global__ __launch_bounds__(AT) void ilu_fwd_kernel(int nrows, const int* __res
   if (t >= nrows * BS) return;
   const int i = rows[t / BS], r = t % BS;
   double acc = x[(size_t)i * BS + r];
-  for (int l = lptr[i]; l < lptr[i + 1]; ++l) {
-    const double* a = lu + ((size_t)lpos[l] * BS + r) * BS;
-    const double* yk = y + (size_t)lcol[l] * BS;
-#pragma unroll
-    for (int c = 0; c < BS; ++c) acc -= a[c] * yk[c];
-  }
+  for (int l = lptr[i]; l < lptr[i + 1]; ++l)
+    acc = bsr_row_sub<BS>(acc, lu + ((size_t)lpos[l] * BS + r) * BS, y + (size_t)lcol[l] * BS);
   y[(size_t)i * BS + r] = acc;
 }
 
@@ -161,17 +158,9 @@ __global__ __launch_bounds__(AT) void ilu_bwd_kernel(int nrows, const int* __res
   const int t = blockIdx.x * AT + threadIdx.x;
   if (t >= nrows * BS) return;
   const int i = rows[t / BS], r = t % BS;
-  const double* d = dinv + ((size_t)i * BS + r) * BS;
-  const double* yi = y + (size_t)i * BS;
-  double acc = 0.0;
-#pragma unroll
-  for (int c = 0; c < BS; ++c) acc += d[c] * yi[c];
-  for (int u = uptr[i]; u < uptr[i + 1]; ++u) {
-    const double* a = lu + ((size_t)upos[u] * BS + r) * BS;
-    const double* zj = z + (size_t)ucol[u] * BS;
-#pragma unroll
-    for (int c = 0; c < BS; ++c) acc -= a[c] * zj[c];
-  }
+  double acc = bsr_row_acc<BS>(0.0, dinv + ((size_t)i * BS + r) * BS, y + (size_t)i * BS);
+  for (int u = uptr[i]; u < uptr[i + 1]; ++u)
+    acc = bsr_row_sub<BS>(acc, lu + ((size_t)upos[u] * BS + r) * BS, z + (size_t)ucol[u] * BS);
   z[(size_t)i * BS + r] = acc;
 }
 

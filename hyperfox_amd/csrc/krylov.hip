@@ -2,6 +2,7 @@
 #include <cmath>
 #include <vector>
 
+#include "bsr_row.h"
 #include "krylov.h"
 
 namespace hfx {
@@ -37,10 +38,7 @@ __global__ void spmv_kernel(int nb, const int* __restrict__ rowptr, const int* _
   const int F = (int)(t / BS), r = (int)(t % BS);
   double acc = 0.0;
   for (int k = rowptr[F]; k < rowptr[F + 1]; ++k) {
-    const double* a = vals + ((size_t)k * BS + r) * BS;
-    const double* xv = x + (size_t)col[k] * BS;
-#pragma unroll
-    for (int c = 0; c < BS; ++c) acc += a[c] * xv[c];
+    acc = bsr_row_acc<BS>(acc, vals + ((size_t)k * BS + r) * BS, x + (size_t)col[k] * BS);
   }
   y[t] = b ? b[t] - acc : acc;
 }
@@ -92,12 +90,7 @@ __global__ void block_diag_apply_kernel(int nb, const double* __restrict__ dinv,
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nb * BS) return;
   const int F = (int)(t / BS), r = (int)(t % BS);
-  const double* a = dinv + ((size_t)F * BS + r) * BS;
-  const double* xv = x + (size_t)F * BS;
-  double acc = 0.0;
-#pragma unroll
-  for (int c = 0; c < BS; ++c) acc += a[c] * xv[c];
-  y[t] = acc;
+  y[t] = bsr_row_acc<BS>(0.0, dinv + ((size_t)F * BS + r) * BS, x + (size_t)F * BS);
 }
 
 __global__ void point_diag_apply_kernel(long long n, const double* __restrict__ d, const double* __restrict__ x,
@@ -287,12 +280,8 @@ __global__ __launch_bounds__(RT) void spmv_dot_kernel(int nb, const int* __restr
   if (t < (long long)nb * BS) {
     const int F = (int)(t / BS), r = (int)(t % BS);
     double acc = 0.0;
-    for (int k = rowptr[F]; k < rowptr[F + 1]; ++k) {
-      const double* a = vals + ((size_t)k * BS + r) * BS;
-      const double* xv = x + (size_t)col[k] * BS;
-#pragma unroll
-      for (int c = 0; c < BS; ++c) acc += a[c] * xv[c];
-    }
+    for (int k = rowptr[F]; k < rowptr[F + 1]; ++k)
+      acc = bsr_row_acc<BS>(acc, vals + ((size_t)k * BS + r) * BS, x + (size_t)col[k] * BS);
     y[t] = acc;
     a1 = w1[t] * acc;
     a2 = acc * acc;

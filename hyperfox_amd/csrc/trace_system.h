@@ -43,6 +43,17 @@ struct BsrDev {
 void launch_scatter(const BsrDev& A, int nFpc, const double* S, const double* S0, const int8_t* invperm,
                     const uint8_t* is_dir, const double* g, double* rhs, hipStream_t s);
 
+// Second half of the fused scatter (the element kernel has written every
+// off-diagonal block): diagonal block of face F = dside[2F] (+ dside[2F+1]
+// when F has a second cell), identity on Dirichlet faces; then the RHS as in
+// launch_scatter.  Same values, bit for bit, as launch_scatter on stored S.
+void launch_scatter_diag(const BsrDev& A, int nFpc, const double* dside, const double* S0, const int8_t* invperm,
+                         const uint8_t* is_dir, const double* g, double* rhs, hipStream_t s);
+
+// BSR block index of every (face f, face g) pair of every cell:
+// out[(e*nFpc + f)*nFpc + g] (the diagonal block when f == g).
+std::vector<int> cell_block_positions(const BsrPattern& P, int nCells, int nFpc, const int* cell2face);
+
 // u = U lambda_e + U0, q = Q lambda_e + Q0 (HDGSolver.cpp:741-775).
 void launch_reconstruct(int nEl, int nN, int nQ, int nFpc, int bs, const double* U, const double* U0, const double* Q,
                         const double* Q0, const int* cell2face, const int8_t* face_perm, const double* lambda, double* u,

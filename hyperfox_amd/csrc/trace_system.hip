@@ -99,6 +99,23 @@ __global__ void scatter_kernel(int nnzb, int nFpc, const int* __restrict__ col, 
   vals[t] = v;
 }
 
+template <int BS>
+__global__ void scatter_diag_kernel(int nb, const int* __restrict__ diag, const int* __restrict__ face_cells,
+                                    const double* __restrict__ dside, const uint8_t* __restrict__ is_dir,
+                                    double* __restrict__ vals) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nb * BS * BS) return;
+  const int F = (int)(t / (BS * BS)), rc = (int)(t % (BS * BS));
+  double v;
+  if (is_dir && is_dir[F]) {
+    v = (rc / BS == rc % BS) ? 1.0 : 0.0;
+  } else {
+    v = dside[(size_t)2 * F * BS * BS + rc];
+    if (face_cells[4 * (size_t)F + 2] >= 0) v += dside[(size_t)(2 * F + 1) * BS * BS + rc];
+  }
+  vals[(size_t)diag[F] * BS * BS + rc] = v;
+}
+
 __global__ void rhs_kernel(int nb, int bs, int nFpc, const int* __restrict__ face_cells, const double* __restrict__ S0,
                            const int8_t* __restrict__ invperm, const uint8_t* __restrict__ is_dir,
                            const double* __restrict__ g, double* __restrict__ rhs) {
@@ -165,6 +182,48 @@ void launch_scatter(const BsrDev& A, int nFpc, const double* S, const double* S0
                        invperm, is_dir, g, rhs);
     HFX_HIP(hipGetLastError());
   }
+}
+
+void launch_scatter_diag(const BsrDev& A, int nFpc, const double* dside, const double* S0, const int8_t* invperm,
+                         const uint8_t* is_dir, const double* g, double* rhs, hipStream_t s) {
+  const int nt = 256;
+  const long long n = (long long)A.nb * A.bs * A.bs;
+#define HFX_BS(B)                                                                                                 \
+  case B:                                                                                                         \
+    hipLaunchKernelGGL((scatter_diag_kernel<B>), dim3(ceil_div(n, nt)), dim3(nt), 0, s, A.nb, A.diag.p,           \
+                       A.face_cells.p, dside, is_dir, A.vals.p);                                                  \
+    break;
+  if (n > 0) {
+    switch (A.bs) {
+      HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(10)
+      default: throw std::runtime_error("scatter: unsupported block size");
+    }
+    HFX_HIP(hipGetLastError());
+  }
+#undef HFX_BS
+  const long long nr = A.nrows();
+  if (nr > 0) {
+    hipLaunchKernelGGL(rhs_kernel, dim3(ceil_div(nr, nt)), dim3(nt), 0, s, A.nb, A.bs, nFpc, A.face_cells.p, S0,
+                       invperm, is_dir, g, rhs);
+    HFX_HIP(hipGetLastError());
+  }
+}
+
+std::vector<int> cell_block_positions(const BsrPattern& P, int nCells, int nFpc, const int* cell2face) {
+  std::vector<int> out((size_t)nCells * nFpc * nFpc);
+  for (int e = 0; e < nCells; ++e)
+    for (int f = 0; f < nFpc; ++f) {
+      const int F = cell2face[(size_t)e * nFpc + f];
+      const int* c0 = P.col.data() + P.rowptr[F];
+      const int* c1 = P.col.data() + P.rowptr[F + 1];
+      for (int g = 0; g < nFpc; ++g) {
+        const int G = cell2face[(size_t)e * nFpc + g];
+        const int* it = std::lower_bound(c0, c1, G);
+        if (it == c1 || *it != G) throw std::runtime_error("bsr: a cell's face pair is missing from the pattern");
+        out[((size_t)e * nFpc + f) * nFpc + g] = (int)(it - P.col.data());
+      }
+    }
+  return out;
 }
 
 void launch_reconstruct(int nEl, int nN, int nQ, int nFpc, int bs, const double* U, const double* U0, const double* Q,

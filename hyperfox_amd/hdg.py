@@ -188,6 +188,21 @@ class HDGProblem:
         check(lib.hfx_hdg_get_trace_csr(self._h, _ptr(rowptr), _ptr(cols), _ptr(vals), _ptr(rhs)))
         return rowptr, cols, vals, rhs
 
+    def set_time_scheme(self, scheme, dt=0.0):
+        """0 none, 1 implicit Euler (the reference's Euler, isExplicit=false)
+        on the u rows with step dt; u_old is the current solution field."""
+        check(lib.hfx_hdg_set_time_scheme(self._h, scheme, dt))
+
+    def set_solution(self, u):
+        """Set the cell-wise nodal solution field (nCells x nN)."""
+        u = _f64(u)
+        check(lib.hfx_hdg_set_solution(self._h, _ptr(u)))
+
+    def set_fused_scatter(self, on):
+        """Fused trace scatter on straight-sided cells (default on): the
+        element kernel writes the trace matrix in place instead of storing S."""
+        check(lib.hfx_hdg_set_fused_scatter(self._h, 1 if on else 0))
+
     def set_krylov(self, method):
         """0 restarted FGMRES (default), 1 BiCGStab."""
         check(lib.hfx_hdg_set_krylov(self._h, method))

@@ -159,6 +159,28 @@ int hfx_hdg_trace_sizes(hfx_hdg* h, long long* out);
 /* Trace system as CSR (rows sorted by column) and its RHS (after assemble). */
 int hfx_hdg_get_trace_csr(hfx_hdg* h, long long* rowptr, int* cols, double* vals, double* rhs);
 
+/* Time scheme on the u rows of every element matrix (HDGModel::compute,
+ * HDGModel.cpp:39-47).  HFX_TS_BEULER is Euler with isExplicit = false
+ * (Euler::apply, Euler.cpp:18-37): u rows scaled by dt, plus the element
+ * mass M in Suu and M u_old in F_u, where u_old is the problem's "Solution"
+ * field at assembly (the last solve's u, or hfx_hdg_set_solution).
+ * HFX_TS_FEULER (isExplicit = true) is rejected.  dt must be > 0, as
+ * Euler::apply requires. */
+enum { HFX_TS_NONE = 0, HFX_TS_BEULER = 1, HFX_TS_FEULER = 2 };
+int hfx_hdg_set_time_scheme(hfx_hdg* h, int scheme, double dt);
+/* Set the "Solution" field u (nCells x nN, cell-wise nodal values), e.g. an
+ * initial condition; hfx_hdg_solve overwrites it. */
+int hfx_hdg_set_solution(hfx_hdg* h, const double* u);
+
+/* Trace scatter of hfx_hdg_assemble on straight-sided cells: fused (on = 1,
+ * the default: the element kernel writes each off-diagonal face block of the
+ * trace matrix in place and one face-diagonal pass adds the two cells'
+ * diagonal contributions, so S is never stored) or stored (on = 0: S is
+ * stored and gathered, as in HDGSolver::assembleSystem,
+ * HDGSolver.cpp:531-675).  Both give the same trace matrix bit for bit;
+ * hfx_hdg_get_condensed stores S on demand. */
+int hfx_hdg_set_fused_scatter(hfx_hdg* h, int on);
+
 /* Krylov method of hfx_hdg_solve: restarted FGMRES (default; the reference's
  * KSPGMRES) or BiCGStab with device-resident scalars (KSPBCGS). */
 enum { HFX_KRYLOV_FGMRES = 0, HFX_KRYLOV_BICGSTAB = 1 };

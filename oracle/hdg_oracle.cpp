@@ -702,6 +702,43 @@ extern "C" int orc_refel(int dim, int p, double* nodes, int* face_nodes, double*
   ORC_END
 }
 
+namespace {
+
+// Euler::apply with isExplicit = false (Euler.cpp:18-37), called from
+// HDGModel::compute (HDGModel.cpp:39-47) on the u rows of the local system:
+// u rows (every column) and F_u scaled by dt, then the element mass M added
+// to the u-u block and M u_old to F_u.  M is the u-u mass of the TimeScheme's
+// Mass operator (Mass.cpp:5-38), which is the first component block of
+// Sqq = I_d (x) M (HDGBase.cpp:139-157) on the same cubature.
+void apply_euler(const Ref& R, double dt, const double* u_old, double* K, double* F) {
+  const int n = R.n(), nN = R.nN, d = R.d;
+  std::vector<double> M((size_t)nN * nN);
+  for (int i = 0; i < nN; ++i)
+    for (int j = 0; j < nN; ++j) M[(size_t)i * nN + j] = K[(size_t)(nN + i * d) * n + nN + j * d];
+  for (int r = 0; r < nN; ++r) {
+    for (int c = 0; c < n; ++c) K[(size_t)r * n + c] *= dt;
+    F[r] *= dt;
+  }
+  for (int i = 0; i < nN; ++i) {
+    double mu = 0.0;
+    for (int j = 0; j < nN; ++j) {
+      K[(size_t)i * n + j] += M[(size_t)i * nN + j];
+      mu += M[(size_t)i * nN + j] * u_old[j];
+    }
+    F[i] += mu;
+  }
+}
+
+}  // namespace
+
+extern "C" int orc_apply_euler(int dim, int p, double dt, const double* u_old, double* K, double* F) {
+  ORC_TRY
+  check_dim(dim, p);
+  if (!(dt > 0.0)) throw std::runtime_error("Euler: the time step needs to be set and it should not be 0");
+  apply_euler(ref(dim, p), dt, u_old, K, F);
+  ORC_END
+}
+
 extern "C" int orc_local_system(int dim, int p, orc_model model, const double* x, const double* D, const double* b,
                                 const double* tau, const double* f_ip, const double* r_ip, double* K, double* F) {
   ORC_TRY
@@ -742,6 +779,8 @@ struct orc_system {
   std::vector<int> perm;              // nEl*nF*nNf : face2CellMap
   std::vector<char> is_bnd;           // nFaces
   std::vector<double> S, S0, U, Q, U0, Q0;
+  double ts_dt = 0.0;          // implicit Euler step (0: none)
+  std::vector<double> u_old;   // nEl * nN "Solution" field
   std::vector<long long> rowptr;
   std::vector<int> cols;
   std::vector<double> vals, rhs;
@@ -776,6 +815,7 @@ void system_assemble(orc_system* s, int nEl, int nthreads) {
       local_system(R, s->model, x.data(), s->hasD ? Dl.data() : nullptr, s->hasB ? bl.data() : nullptr, tl.data(),
                    s->hasF ? &s->f_ip[(size_t)e * R.nip] : nullptr, s->hasR ? &s->r_ip[(size_t)e * R.nip] : nullptr,
                    K.data(), F.data());
+      if (s->ts_dt > 0.0) apply_euler(R, s->ts_dt, &s->u_old[(size_t)e * nN], K.data(), F.data());
       double* Se = &s->S[(size_t)e * ll * ll];
       double* S0e = &s->S0[(size_t)e * ll];
       condense(R, K.data(), F.data(), Se, S0e, &s->U[(size_t)e * lu * ll], &s->Q[(size_t)e * lq * ll],
@@ -978,6 +1018,15 @@ extern "C" orc_system* orc_system_build(int dim, int p, orc_model model, int nNo
 }
 
 extern "C" void orc_system_free(orc_system* s) { delete s; }
+
+extern "C" int orc_system_set_euler(orc_system* s, double dt, const double* u_old, int nthreads) {
+  ORC_TRY
+  if (dt < 0.0) throw std::runtime_error("Euler: negative time step");
+  s->ts_dt = dt;
+  if (dt > 0.0) s->u_old.assign(u_old, u_old + (size_t)s->nEl * s->R->nN);
+  system_assemble(s, s->nEl, nthreads > 0 ? nthreads : omp_get_max_threads());
+  ORC_END
+}
 
 extern "C" int orc_system_sizes(const orc_system* s, long long* out) {
   out[0] = s->ndofs;

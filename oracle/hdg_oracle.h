@@ -57,6 +57,11 @@ typedef struct {
 int orc_local_system(int dim, int p, orc_model model, const double* x, const double* D, const double* b,
                      const double* tau, const double* f_ip, const double* r_ip, double* K, double* F);
 
+/* Implicit Euler on the u rows of a local system, in place (Euler::apply,
+ * isExplicit = false, Euler.cpp:18-37): u rows and F_u times dt, plus the
+ * element mass M and M u_old (u_old: nN cell-wise nodal values). */
+int orc_apply_euler(int dim, int p, double dt, const double* u_old, double* K, double* F);
+
 /* Physical coordinates of the element IPs (nIP*dim). */
 int orc_ip_coords(int dim, int p, const double* x, double* out);
 
@@ -82,6 +87,9 @@ orc_system* orc_system_build(int dim, int p, orc_model model, int nNodes, const 
                              int tau_sides, const double* f_ip, const double* r_ip, const double* dirichlet,
                              int nthreads);
 void orc_system_free(orc_system* s);
+/* Implicit Euler with step dt (0: none) and "Solution" field u_old
+ * (nEl*nN) on every element, then reassemble the trace system. */
+int orc_system_set_euler(orc_system* s, double dt, const double* u_old, int nthreads);
 /* sizes: out[0]=ndofs, out[1]=nnz */
 int orc_system_sizes(const orc_system* s, long long* out);
 int orc_system_csr(const orc_system* s, long long* rowptr, int* cols, double* vals, double* rhs);

@@ -34,6 +34,8 @@ def _load():
         'orc_local_system': ([I, I, Model] + [P] * 8, I),
         'orc_ip_coords': ([I, I, P, P], I),
         'orc_condense': ([I, I] + [P] * 8, I),
+        'orc_apply_euler': ([I, I, D, P, P, P], I),
+        'orc_system_set_euler': ([P, D, P, I], I),
         'orc_system_build': ([I, I, Model, I, P, I, P, I, P, P, P, P, P, P, I, P, P, P, I], P),
         'orc_system_free': ([P], None),
         'orc_system_sizes': ([P, P], I),
@@ -99,6 +101,14 @@ def local_system(dim, p, kind, x, D=None, ndD=0, b=None, tau=None, f_ip=None, r_
     return K, F
 
 
+def apply_euler(dim, p, dt, u_old, K, F):
+    """Implicit Euler on the u rows of a local system (Euler::apply,
+    Euler.cpp:18-37); returns new (K, F)."""
+    K2, F2 = np.array(K, dtype=np.float64, order='C'), np.array(F, dtype=np.float64)
+    _chk(lib.orc_apply_euler(dim, p, dt, _p(np.ascontiguousarray(u_old, dtype=np.float64)), _p(K2), _p(F2)))
+    return K2, F2
+
+
 def ip_coords(dim, p, x):
     s = sizes(dim, p)
     out = np.zeros((s['nIP'], dim))
@@ -144,6 +154,12 @@ class System:
         if getattr(self, '_h', None):
             lib.orc_system_free(self._h)
             self._h = None
+
+    def set_euler(self, dt, u_old=None, nthreads=0):
+        """Implicit Euler with step dt (0: none) and solution field u_old
+        (nCells x nN) on every element, then reassemble."""
+        self._uold = None if u_old is None else np.ascontiguousarray(u_old, dtype=np.float64)
+        _chk(lib.orc_system_set_euler(self._h, dt, _p(self._uold), nthreads))
 
     def csr(self):
         rowptr = np.zeros(self.ndofs + 1, dtype=np.int64)
