@@ -1,8 +1,9 @@
-// Row of a dense row-major BS x BS block times a BS-vector, accumulated into
-// acc in column order (the same rounding as the plain loop).  Even block
-// sizes read the row and the vector as 16-byte pairs: a block row starts at a
-// multiple of BS doubles, so the pairs are aligned, and one thread's row
-// takes BS/2 loads instead of BS.
+// Row r of dense row-major BS x BS blocks times BS-vector segments, for the
+// thread-per-row BSR kernels.  Columns are taken in pairs,
+// acc += a[c] x[c] + a[c+1] x[c+1], which halves the dependent add chain of a
+// plain loop; with one block per loop step the compiler merges the row and
+// segment loads into 16-byte loads and pipelines the next block's column
+// index (measured on gfx950 for 3-D p=3 blocks: 228 -> 185 us per SpMV).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -10,38 +11,27 @@
 namespace hfx {
 
 template <int BS>
-__device__ __forceinline__ double bsr_row_acc(double acc, const double* __restrict__ a, const double* __restrict__ x) {
-  if constexpr (BS % 2 == 0) {
-    const double2* a2 = reinterpret_cast<const double2*>(a);
-    const double2* x2 = reinterpret_cast<const double2*>(x);
+__device__ __forceinline__ double bsr_row_dot(const double* __restrict__ a, const double* __restrict__ x) {
+  double s = 0.0;
 #pragma unroll
-    for (int c = 0; c < BS / 2; ++c) {
-      const double2 u = a2[c], v = x2[c];
-      acc += u.x * v.x;
-      acc += u.y * v.y;
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < BS; ++c) acc += a[c] * x[c];
-  }
-  return acc;
+  for (int c = 0; c + 1 < BS; c += 2) s += a[c] * x[c] + a[c + 1] * x[c + 1];
+  if constexpr (BS % 2 == 1) s += a[BS - 1] * x[BS - 1];
+  return s;
 }
 
-// acc - row . x, subtracting in column order
+// sum over the blocks k in [k0, k1) (stored at pos[k], or at k when pos is
+// null) of row r of the block times the x segment of column col[k]
 template <int BS>
-__device__ __forceinline__ double bsr_row_sub(double acc, const double* __restrict__ a, const double* __restrict__ x) {
-  if constexpr (BS % 2 == 0) {
-    const double2* a2 = reinterpret_cast<const double2*>(a);
-    const double2* x2 = reinterpret_cast<const double2*>(x);
+__device__ __forceinline__ double bsr_row_sum(int k0, int k1, int r, const int* __restrict__ pos,
+                                              const int* __restrict__ col, const double* __restrict__ vals,
+                                              const double* __restrict__ x) {
+  double acc = 0.0;
+  for (int k = k0; k < k1; ++k) {
+    const double* a = vals + ((size_t)(pos ? pos[k] : k) * BS + r) * BS;
+    const double* xv = x + (size_t)col[k] * BS;
 #pragma unroll
-    for (int c = 0; c < BS / 2; ++c) {
-      const double2 u = a2[c], v = x2[c];
-      acc -= u.x * v.x;
-      acc -= u.y * v.y;
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < BS; ++c) acc -= a[c] * x[c];
+    for (int c = 0; c + 1 < BS; c += 2) acc += a[c] * xv[c] + a[c + 1] * xv[c + 1];
+    if constexpr (BS % 2 == 1) acc += a[BS - 1] * xv[BS - 1];
   }
   return acc;
 }

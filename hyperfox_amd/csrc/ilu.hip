@@ -142,10 +142,7 @@ __global__ __launch_bounds__(AT) void ilu_fwd_kernel(int nrows, const int* __res
   const int t = blockIdx.x * AT + threadIdx.x;
   if (t >= nrows * BS) return;
   const int i = rows[t / BS], r = t % BS;
-  double acc = x[(size_t)i * BS + r];
-  for (int l = lptr[i]; l < lptr[i + 1]; ++l)
-    acc = bsr_row_sub<BS>(acc, lu + ((size_t)lpos[l] * BS + r) * BS, y + (size_t)lcol[l] * BS);
-  y[(size_t)i * BS + r] = acc;
+  y[(size_t)i * BS + r] = x[(size_t)i * BS + r] - bsr_row_sum<BS>(lptr[i], lptr[i + 1], r, lpos, lcol, lu, y);
 }
 
 // Backward sweep over one level: z_i = Dinv_i y_i - sum_j Uhat_ij z_j.
@@ -158,10 +155,8 @@ __global__ __launch_bounds__(AT) void ilu_bwd_kernel(int nrows, const int* __res
   const int t = blockIdx.x * AT + threadIdx.x;
   if (t >= nrows * BS) return;
   const int i = rows[t / BS], r = t % BS;
-  double acc = bsr_row_acc<BS>(0.0, dinv + ((size_t)i * BS + r) * BS, y + (size_t)i * BS);
-  for (int u = uptr[i]; u < uptr[i + 1]; ++u)
-    acc = bsr_row_sub<BS>(acc, lu + ((size_t)upos[u] * BS + r) * BS, z + (size_t)ucol[u] * BS);
-  z[(size_t)i * BS + r] = acc;
+  z[(size_t)i * BS + r] = bsr_row_dot<BS>(dinv + ((size_t)i * BS + r) * BS, y + (size_t)i * BS) -
+                          bsr_row_sum<BS>(uptr[i], uptr[i + 1], r, upos, ucol, lu, z);
 }
 
 template <int BS>

@@ -36,10 +36,7 @@ __global__ void spmv_kernel(int nb, const int* __restrict__ rowptr, const int* _
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nb * BS) return;
   const int F = (int)(t / BS), r = (int)(t % BS);
-  double acc = 0.0;
-  for (int k = rowptr[F]; k < rowptr[F + 1]; ++k) {
-    acc = bsr_row_acc<BS>(acc, vals + ((size_t)k * BS + r) * BS, x + (size_t)col[k] * BS);
-  }
+  const double acc = bsr_row_sum<BS>(rowptr[F], rowptr[F + 1], r, nullptr, col, vals, x);
   y[t] = b ? b[t] - acc : acc;
 }
 
@@ -90,7 +87,7 @@ __global__ void block_diag_apply_kernel(int nb, const double* __restrict__ dinv,
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nb * BS) return;
   const int F = (int)(t / BS), r = (int)(t % BS);
-  y[t] = bsr_row_acc<BS>(0.0, dinv + ((size_t)F * BS + r) * BS, x + (size_t)F * BS);
+  y[t] = bsr_row_dot<BS>(dinv + ((size_t)F * BS + r) * BS, x + (size_t)F * BS);
 }
 
 __global__ void point_diag_apply_kernel(long long n, const double* __restrict__ d, const double* __restrict__ x,
@@ -279,9 +276,7 @@ __global__ __launch_bounds__(RT) void spmv_dot_kernel(int nb, const int* __restr
   double a1 = 0.0, a2 = 0.0;
   if (t < (long long)nb * BS) {
     const int F = (int)(t / BS), r = (int)(t % BS);
-    double acc = 0.0;
-    for (int k = rowptr[F]; k < rowptr[F + 1]; ++k)
-      acc = bsr_row_acc<BS>(acc, vals + ((size_t)k * BS + r) * BS, x + (size_t)col[k] * BS);
+    const double acc = bsr_row_sum<BS>(rowptr[F], rowptr[F + 1], r, nullptr, col, vals, x);
     y[t] = acc;
     a1 = w1[t] * acc;
     a2 = acc * acc;
