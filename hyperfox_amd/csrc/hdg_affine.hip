@@ -454,33 +454,41 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     {
       constexpr int MT = (NN + 15) / 16, KD = D * NN, KS = (KD + 3) / 4;
       const int li = lane & 15, lk = lane >> 4;
-#pragma unroll 1
-      for (int nt = 0; nt < MT; ++nt) {
-        const int jb = nt * 16 + li;
-        dvec4 acc[MT];
+      // every output tile's MFMA chain in flight at once: with one wave per
+      // SIMD, independent accumulators are what hides the MFMA latency
+      dvec4 acc[MT][MT];
 #pragma unroll
-        for (int m = 0; m < MT; ++m) acc[m] = dvec4{0.0, 0.0, 0.0, 0.0};
+      for (int nt = 0; nt < MT; ++nt)
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[nt][m] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 3
-        for (int ks = 0; ks < KS; ++ks) {
-          const int kk = ks * 4 + lk;
-          const bool kv = kk < KD;
-          const int a2 = kv ? kk / NN : 0, l = kv ? kk % NN : 0;
+      for (int ks = 0; ks < KS; ++ks) {
+        const int kk = ks * 4 + lk;
+        const bool kv = kk < KD;
+        const int a2 = kv ? kk / NN : 0, l = kv ? kk % NN : 0;
+        double av[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const int i = m * 16 + li;
+          av[m] = (kv && i < NN) ? SUQ[(a2 * NN + i) * NN + l] : 0.0;
+        }
+#pragma unroll
+        for (int nt = 0; nt < MT; ++nt) {
+          const int jb = nt * 16 + li;
           const double bv = (kv && jb < NN) ? AH[(a2 * NN + l) * NN + jb] : 0.0;
 #pragma unroll
-          for (int m = 0; m < MT; ++m) {
-            const int i = m * 16 + li;
-            const double av = (kv && i < NN) ? SUQ[(a2 * NN + i) * NN + l] : 0.0;
-            acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[m], 0, 0, 0);
-          }
+          for (int m = 0; m < MT; ++m) acc[nt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv, acc[nt][m], 0, 0, 0);
         }
+      }
+#pragma unroll
+      for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = m * 16 + 4 * r + lk;
-            if (i < NN && jb < NN) KM[i * NN + jb] -= acc[m][r];
+            const int i = m * 16 + 4 * r + lk, jb = nt * 16 + li;
+            if (i < NN && jb < NN) KM[i * NN + jb] -= acc[nt][m][r];
           }
-      }
     }
     wave_sync();
     // K^-1 by Gauss-Jordan with partial pivoting, one row of K per lane in
@@ -552,43 +560,57 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     {
       constexpr int MT = (NN + 15) / 16, NT = (NL + 15) / 16, KD = D * NN, KS = (KD + 3) / 4;
       const int li = lane & 15, lk = lane >> 4;
-#pragma unroll 1
+      // all NT x MT tiles accumulated together (independent MFMA chains)
+      int ggs[NT], bbs[NT];
+      bool cvs[NT];
+      double ntc[NT][3];
+#pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
         const int cc = nt * 16 + li;
-        const bool cv = cc < NL;
-        const int gg = cv ? cc / NNF : 0, bb = cv ? cc % NNF : 0;
-        const double rg = FG[gg * (D + 1) + D] / det;
-        double ntc[3] = {0.0, 0.0, 0.0};
+        cvs[nt] = cc < NL;
+        ggs[nt] = cvs[nt] ? cc / NNF : 0;
+        bbs[nt] = cvs[nt] ? cc % NNF : 0;
+        const double rg = FG[ggs[nt] * (D + 1) + D] / det;
+        for (int a2 = 0; a2 < 3; ++a2) ntc[nt][a2] = 0.0;
         for (int a2 = 0; a2 < D; ++a2) {
           double s = 0.0;
-          for (int k = 0; k < D; ++k) s += J[a2 * D + k] * FG[gg * (D + 1) + k];
-          ntc[a2] = cv ? rg * s : 0.0;
+          for (int k = 0; k < D; ++k) s += J[a2 * D + k] * FG[ggs[nt] * (D + 1) + k];
+          ntc[nt][a2] = cvs[nt] ? rg * s : 0.0;
         }
-        dvec4 acc[MT];
+      }
+      dvec4 acc[NT][MT];
 #pragma unroll
-        for (int m = 0; m < MT; ++m) acc[m] = dvec4{0.0, 0.0, 0.0, 0.0};
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int m = 0; m < MT; ++m) acc[nt][m] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll 3
-        for (int ks = 0; ks < KS; ++ks) {
-          const int kk = ks * 4 + lk;
-          const bool kv = kk < KD;
-          const int a2 = kv ? kk / NN : 0, j = kv ? kk % NN : 0;
-          const double ntv = a2 == 0 ? ntc[0] : (a2 == 1 ? ntc[1] : ntc[2]);
-          const double bv = kv ? ntv * GT[(gg * NN + j) * NNF + bb] : 0.0;
+      for (int ks = 0; ks < KS; ++ks) {
+        const int kk = ks * 4 + lk;
+        const bool kv = kk < KD;
+        const int a2 = kv ? kk / NN : 0, j = kv ? kk % NN : 0;
+        double av[MT];
 #pragma unroll
-          for (int m = 0; m < MT; ++m) {
-            const int i = m * 16 + li;
-            const double av = (kv && i < NN) ? SUQ[(a2 * NN + i) * NN + j] : 0.0;
-            acc[m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[m], 0, 0, 0);
-          }
+        for (int m = 0; m < MT; ++m) {
+          const int i = m * 16 + li;
+          av[m] = (kv && i < NN) ? SUQ[(a2 * NN + i) * NN + j] : 0.0;
         }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const double ntv = a2 == 0 ? ntc[nt][0] : (a2 == 1 ? ntc[nt][1] : ntc[nt][2]);
+          const double bv = kv ? ntv * GT[(ggs[nt] * NN + j) * NNF + bbs[nt]] : 0.0;
+#pragma unroll
+          for (int m = 0; m < MT; ++m) acc[nt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv, acc[nt][m], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = m * 16 + 4 * r + lk;
-            if (cv && i < NN) RB[i * NCOL + cc] += acc[m][r];
+            const int i = m * 16 + 4 * r + lk, cc = nt * 16 + li;
+            if (cvs[nt] && i < NN) RB[i * NCOL + cc] += acc[nt][m][r];
           }
-      }
     }
     wave_sync();
     // U = -K^-1 R on FP64 MFMA, in place: every tile is accumulated before
@@ -723,22 +745,31 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         const bool dirf = (flg & 2) != 0;
         const int sidef = flg & 1;
         const int* permf = IX + L::IX_PERM + f * NNF;
-#pragma unroll 1
+        // the NT column tiles of face f accumulated together (independent
+        // MFMA chains sharing the A operand)
+        dvec4 accs[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) accs[nt] = dvec4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const int kk = ks * 4 + lk;
+          const bool kv = kk < KF;
+          const int b = kv ? (kk < NNF ? kk : kk - NNF) : 0;
+          double av = 0.0;
+          if (kv && li < NNF) av = kk < NNF ? fmt[li * NNF + b] : -fmd[li * NNF + b];
+          const int rowb = kk < NNF ? s_fn[f * NNF + b] * NCOL : -1;
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) {
+            const int cc = nt * 16 + li;
+            double bv = 0.0;
+            if (kv && cc < NCOL) bv = kk < NNF ? UB[rowb + cc] : QN[(f * NNF + b) * NCOL + cc];
+            accs[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, accs[nt], 0, 0, 0);
+          }
+        }
+#pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           const int cc = nt * 16 + li;
-          dvec4 acc = dvec4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-          for (int ks = 0; ks < KS; ++ks) {
-            const int kk = ks * 4 + lk;
-            const bool kv = kk < KF;
-            const int b = kv ? (kk < NNF ? kk : kk - NNF) : 0;
-            double bv = 0.0;
-            if (kv && cc < NCOL)
-              bv = kk < NNF ? UB[s_fn[f * NNF + b] * NCOL + cc] : QN[(f * NNF + b) * NCOL + cc];
-            double av = 0.0;
-            if (kv && li < NNF) av = kk < NNF ? fmt[li * NNF + b] : -fmd[li * NNF + b];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
-          }
+          const dvec4 acc = accs[nt];
           if (cc <= NL) {
             const bool trc = cc < NL;
             const int gg = trc ? cc / NNF : 0, bb = trc ? cc % NNF : 0;
