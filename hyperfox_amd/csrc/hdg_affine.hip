@@ -104,30 +104,27 @@ struct ALay {
   static constexpr int PER = END1 > END2 ? END1 : END2;
 };
 
-// Wave-wide argmax of a non-negative value over lanes < N; ties go to the
-// lowest lane.  The key keeps the top 26 bits of |v| as a float and the lane
-// in the low 6 bits; every lane posts its key to LDS, reads the N candidate
-// keys back (broadcast reads) and takes the maximum.  `keys` is per-wave
-// scratch of N words.
-template <int N>
-__device__ inline int lds_argmax(unsigned* keys, double v, bool valid) {
+// Wave-wide argmax of a non-negative value over the valid lanes among 0..31;
+// ties go to the lowest lane.  The key keeps the top 26 bits of |v| as a
+// float and the lane in the low 6 bits; a DPP prefix max inside each 16-lane
+// row (row_shr 1, 2, 4, 8) leaves each row's maximum in its last lane, and
+// two readlanes combine rows 0 and 1.
+__device__ inline int dpp_argmax(double v, bool valid) {
   unsigned key = 0u;
   if (valid) {
     float fv = (float)v;
-    if (!(fv >= 0.0f)) fv = 0.0f;  // NaN
+    if (!(fv >= 0.0f)) fv = 0.0f;
     key = (__float_as_uint(fv) & ~63u) | (63u - (unsigned)(threadIdx.x & 63));
     key += 1u << 6;
   }
-  const int lane = threadIdx.x & 63;
-  if (lane < N) keys[lane] = key;
-  wave_sync();
-  unsigned best = 0u;
-#pragma unroll
-  for (int l = 0; l < N; ++l) {
-    const unsigned k2 = keys[l];
-    best = k2 > best ? k2 : best;
-  }
-  wave_sync();  // every lane has read the keys before they are posted again
+  unsigned m = key;
+  m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x111, 0xF, 0xF, true));
+  m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x112, 0xF, 0xF, true));
+  m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x114, 0xF, 0xF, true));
+  m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x118, 0xF, 0xF, true));
+  const unsigned b0 = (unsigned)__builtin_amdgcn_readlane((int)m, 15);
+  const unsigned b1 = (unsigned)__builtin_amdgcn_readlane((int)m, 31);
+  const unsigned best = b0 > b1 ? b0 : b1;
   return 63 - (int)(best & 63u);
 }
 
@@ -491,43 +488,43 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           }
     }
     wave_sync();
-    // K^-1 by Gauss-Jordan with partial pivoting, one row of K per lane in
-    // registers (the pivot row is fetched with readlane, no LDS traffic)
+    // K^-1 by Gauss-Jordan with partial pivoting in exchange form, in place:
+    // one row of K per lane in registers, the pivot row fetched with readlane.
+    // Pivoting on (p, k) with d = K[p][k] makes row p express unknown k:
+    //   K[p][k] <- 1/d, K[p][j] <- -K[p][j]/d, K[i][k] <- K[i][k]/d,
+    //   K[i][j] <- K[i][j] - K[i][k] K[p][j]/d   (i != p, j != k);
+    // after all NN pivots, K^-1[k][p(j)] = K[p(k)][j] with p(k) the pivot row
+    // of column k.  Half the row length of [K | I] elimination.
     {
-      double kr[NN], ir[NN];
+      double kr[NN];
       const int rl = lane < NN ? lane : 0;
 #pragma unroll
-      for (int j = 0; j < NN; ++j) {
-        kr[j] = KM[rl * NN + j];
-        ir[j] = (j == lane) ? 1.0 : 0.0;
-      }
-      unsigned* KEYS = reinterpret_cast<unsigned*>(ws + L::O_OV);  // dead overlay between the phases
+      for (int j = 0; j < NN; ++j) kr[j] = KM[rl * NN + j];
+      static_assert(NN <= 32, "pivot search covers lanes 0..31");
       unsigned long long used = 0ull;
-      double mypiv = 1.0;
       int myk = 0;
+      int pl[NN];
 #pragma unroll
       for (int k = 0; k < NN; ++k) {
-        const int p = __builtin_amdgcn_readfirstlane(
-            lds_argmax<NN>(KEYS, fabs(kr[k]), lane < NN && !((used >> lane) & 1ull)));
+        const bool cand = lane < NN && !((used >> lane) & 1ull);
+        const int p = __builtin_amdgcn_readfirstlane(dpp_argmax(fabs(kr[k]), cand));
         used |= 1ull << p;
-        if (lane == p) { mypiv = kr[k]; myk = k; }
-        const double fac = kr[k] / readlane_d(kr[k], p);
-#pragma unroll
-        for (int j = k + 1; j < NN; ++j) {
-          const double pj = readlane_d(kr[j], p);
-          if (lane != p) kr[j] -= fac * pj;
-        }
+        pl[k] = p;
+        const bool piv = lane == p;
+        if (piv) myk = k;
+        const double aik = kr[k];
+        const double dinv = 1.0 / readlane_d(aik, p);
 #pragma unroll
         for (int j = 0; j < NN; ++j) {
-          const double pj = readlane_d(ir[j], p);
-          if (lane != p) ir[j] -= fac * pj;
+          if (j == k) continue;
+          const double pj = readlane_d(kr[j], p) * dinv;
+          kr[j] = piv ? -pj : kr[j] - aik * pj;
         }
+        kr[k] = piv ? dinv : aik * dinv;
       }
-      // lane holding variable myk writes row myk of K^-1
       if (lane < NN) {
-        const double r = 1.0 / mypiv;
 #pragma unroll
-        for (int j = 0; j < NN; ++j) KM[myk * NN + j] = ir[j] * r;
+        for (int j = 0; j < NN; ++j) KM[myk * NN + pl[j]] = kr[j];
       }
     }
     wave_sync();
