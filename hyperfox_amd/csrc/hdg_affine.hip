@@ -842,8 +842,21 @@ void launch_aff(const ElemArgs& a, int grid_cap, hipStream_t s) {
     HFX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr_set = true;
   }
+  // persistent grid by default: as many workgroups as are resident at once
+  // (the LDS allows one per CU for 3-D p=3), each looping over its cells, so
+  // that the reference tables are staged once per workgroup instead of once
+  // per W cells (C3: 5.28 -> 4.98 ms)
+  static int resident = 0;
+  if (resident == 0) {
+    int dev = 0, cus = 0, per = 0;
+    HFX_HIP(hipGetDevice(&dev));
+    HFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HFX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, 64 * W, lds));
+    resident = (per > 0 ? per : 1) * (cus > 0 ? cus : 1);
+  }
   long long grid = ((long long)a.nEl + W - 1) / W;
-  if (grid_cap > 0 && grid > grid_cap) grid = grid_cap;
+  const long long cap = grid_cap > 0 ? grid_cap : resident;
+  if (grid > cap) grid = cap;
   if (grid <= 0) return;
   hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * W), lds, s, a);
   HFX_HIP(hipGetLastError());

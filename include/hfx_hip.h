@@ -127,8 +127,10 @@ int hfx_hdg_ip_coords(hfx_hdg* h, double* out);
 /* Source and reaction values at the element IPs (nCells*nIP each, or NULL). */
 int hfx_hdg_set_source(hfx_hdg* h, const double* f_ip, const double* r_ip);
 
-/* Launch shape of the element kernel: threads per element (64/128/256) and
- * a cap on the grid (0 = one workgroup per element). */
+/* Launch shape of the element kernel: threads per element of the general
+ * kernel (64/128/256) and a cap on the grid (0 = default: one workgroup per
+ * element for the general kernel; for the affine kernel as many workgroups
+ * as are resident on the device at once, each looping over its cells). */
 int hfx_hdg_set_launch(hfx_hdg* h, int threads, int grid_cap);
 
 /* Element kernel selection: 0 automatic (the affine-cell kernel when every
