@@ -56,27 +56,39 @@ def elem_bytes(dim, nN, nNf, nIP, nnz_dofs_in):
     return 8 * (out + nnz_dofs_in) + 4 * nN
 
 
-def cpu_baseline(n_cpu, p, threads):
-    """Oracle (FP64 C++ restatement of the reference path) on a smaller mesh of
-    the same workload: per-element cost does not depend on the mesh size."""
+def cpu_system(n, p, threads):
+    """The oracle's whole-mesh pipeline on the benchmark workload at n^3 x 6."""
     sys.path.insert(0, os.path.join(ROOT, 'oracle'))
     import oracle as orc
     import hyperfox_amd.problems as pr
-    w = pr.convection_diffusion_3d(n_cpu, p)
+    w = pr.convection_diffusion_3d(n, p)
     a = w.mesh.arrays()
     X = a['coords'][a['cells']]
-    s0 = orc.sizes(3, p)
     # source at IPs on the CPU (same formula as the GPU leg)
     t = orc.refel(3, p)
     xip = np.einsum('qn,end->eqd', t['phi'], X)
     f = w.source_at(xip)
     s = orc.System(3, p, 2, a, w.tau, tau_sides=2, D=w.D, ndD=1, b=w.b, f_ip=f, dirichlet=w.dirichlet,
                    nthreads=threads)
-    best = min(s.time_assembly(w.mesh.n_cells, threads) for _ in range(2))
-    dofs = w.mesh.n_cells * s0['nN']
+    return s, w.mesh.n_cells * orc.sizes(3, p)['nN'], w.mesh.n_cells
+
+
+def cpu_baseline(n_cpu, p, threads, n_cpu1=8):
+    """Oracle (FP64 C++ restatement of the reference path: assembly, Householder
+    condensation, Dirichlet rows, CSR scatter) timed on the host cores: on all
+    `threads` OpenMP threads over n_cpu^3 x 6 cells (best of 2), and on one core
+    over n_cpu1^3 x 6 cells.  Per-cell cost does not depend on the mesh size."""
+    s, dofs, cells = cpu_system(n_cpu, p, threads)
+    best = min(s.time_assembly(cells, threads) for _ in range(2))
+    del s
+    s1, dofs1, cells1 = cpu_system(n_cpu1, p, 1)
+    best1 = min(s1.time_assembly(cells1, 1) for _ in range(2))
     return {'value': dofs / best, 'unit': 'DOF/s', 'cores': threads, 'kind': 'port',
-            'sample': '%d^3x6 = %d tets, same p/model/fields; assembly+condensation+BC+CSR scatter, %.2f s'
-                      % (n_cpu, w.mesh.n_cells, best)}
+            'sample': '%d^3x6 = %d tets, same p/model/fields; assembly+condensation+BC+CSR scatter, best of 2: %.2f s on '
+                      '%d threads' % (n_cpu, cells, best, threads),
+            'value_1core': dofs1 / best1,
+            'sample_1core': '%d^3x6 = %d tets on 1 thread, best of 2: %.2f s' % (n_cpu1, cells1, best1),
+            'host_cpus': os.cpu_count()}
 
 
 PC_NAMES = {'jacobi': 1, 'block-jacobi': 2, 'learned': 3, 'ilu0': 4, 'ilu0-natural': 5}
@@ -286,7 +298,7 @@ def main():
     ap.add_argument('--p', type=int, default=3)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-solve', action='store_true')
-    ap.add_argument('--cpu-n', type=int, default=12)
+    ap.add_argument('--cpu-n', type=int, default=24)
     ap.add_argument('--traffic', type=float, nargs=2, metavar=('FETCH_KIB', 'WRITE_KIB'), default=None,
                     help='HBM KiB per element-kernel dispatch from rocprofv3 --pmc runs, already scaled')
     ap.add_argument('--copy-calibration', type=int, default=0, metavar='MiB')
