@@ -51,6 +51,8 @@ _SIGS = {
     'hfx_hdg_set_fused_scatter': ([_P, _I], _I),
     'hfx_hdg_set_time_scheme': ([_P, _I, _D], _I),
     'hfx_hdg_set_solution': ([_P, _P], _I),
+    'hfx_hdg_set_solver_type': ([_P, _I], _I),
+    'hfx_hdg_set_flux': ([_P, _P], _I),
     'hfx_hdg_ilu_factors': ([_P, _I, _P, _P, _P], _I),
     'hfx_ilu_analyse': ([_I, _P, _P, _I, _P, _P, _P, _P, _P], _I),
     'hfx_hdg_get_solution': ([_P] * 4, _I),

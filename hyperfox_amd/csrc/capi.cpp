@@ -325,6 +325,7 @@ extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
   h->u.alloc((size_t)m.nCells * nN);
   HFX_HIP(hipMemsetAsync(h->u.p, 0, h->u.n * sizeof(double), s));  // "Solution" starts at 0
   h->q.alloc((size_t)m.nCells * nQ);
+  HFX_HIP(hipMemsetAsync(h->q.p, 0, h->q.n * sizeof(double), s));  // "Flux" starts at 0
   h->g.alloc((size_t)m.nFaces * m.nNf);
   HFX_HIP(hipMemsetAsync(h->g.p, 0, h->g.n * sizeof(double), s));
   HFX_HIP(hipStreamSynchronize(s));
@@ -538,6 +539,24 @@ std::function<void(const double*, double*)> make_pc(hfx_hdg* h, int pc) {
 void do_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, hfx::KrylovInfo& info) {
   const long long n = h->A.nrows();
   hipStream_t s = h->stream;
+  if (h->hdg_type == HFX_HDG_SEXPLICIT) {
+    // face-local direct solves (HDGSolver.cpp:709-729): the trace matrix is
+    // block diagonal, lambda_F = S_FF^-1 S0_F
+    h->dinv.alloc((size_t)h->A.nb * h->A.bs * h->A.bs);
+    hfx::bsr_invert_diag(h->A, h->dinv.p, s);
+    hfx::apply_block_diag(h->A.nb, h->A.bs, h->dinv.p, h->rhs.p, h->lambda.p, s);
+    hfx::Reducer red;
+    red.init(n);
+    hfx::DevBuf<double> r;
+    r.alloc(n);
+    hfx::bsr_spmv(h->A, h->lambda.p, r.p, h->rhs.p, s);
+    const double bn = red.norm(n, h->rhs.p, s), rn = red.norm(n, r.p, s);
+    info = hfx::KrylovInfo();
+    info.iters = 1;
+    info.true_relres = bn > 0.0 ? rn / bn : rn;
+    info.reason = hfx::CONVERGED_RTOL;
+    return;
+  }
   std::function<void(const double*, double*)> M = make_pc(h, pc);
   HFX_HIP(hipMemsetAsync(h->lambda.p, 0, n * sizeof(double), s));
   hfx::KrylovOpts o;
@@ -571,6 +590,27 @@ extern "C" int hfx_hdg_set_solution(hfx_hdg* h, const double* u) {
   require(u != nullptr, "hfx_hdg_set_solution: null field");
   h->set_device();
   h->u.upload(u, h->u.n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_solver_type(hfx_hdg* h, int type) {
+  HFX_TRY
+  require(h != nullptr, "hfx_hdg_set_solver_type: null handle");
+  require(type == HFX_HDG_IMPLICIT || type == HFX_HDG_WEXPLICIT || type == HFX_HDG_SEXPLICIT,
+          "hfx_hdg_set_solver_type: unknown solver type");
+  h->hdg_type = type;
+  h->assembled = h->solved = false;
+  ++h->args_version;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_flux(hfx_hdg* h, const double* q) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_set_flux: set the mesh first");
+  require(q != nullptr, "hfx_hdg_set_flux: null field");
+  h->set_device();
+  h->q.upload(q, h->q.n, h->stream);
   HFX_HIP(hipStreamSynchronize(h->stream));
   HFX_END
 }

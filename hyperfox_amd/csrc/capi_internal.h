@@ -69,6 +69,9 @@ struct hfx_hdg {
   // problem's current "Solution" field u
   int ts_scheme = HFX_TS_NONE;
   double ts_dt = 0.0;
+  // HDGSolverOpts::type (hfx_hdg_set_solver_type); the explicit modes read the
+  // problem's "Solution" u and "Flux" q fields at assembly
+  int hdg_type = HFX_HDG_IMPLICIT;
   // bumped by every call that changes the element kernel's arguments, so
   // that a batch holding a copy of them rebuilds it
   unsigned long long args_version = 0;
@@ -110,6 +113,11 @@ struct hfx_hdg {
     a.f_ip = f_ip.p;
     a.r_ip = r_ip.p;
     a.S = S.p; a.S0 = S0.p; a.U = U.p; a.Q = Q.p; a.U0 = U0.p; a.Q0 = Q0.p;
+    if (hdg_type != HFX_HDG_IMPLICIT) {
+      a.hdg_explicit = 1;
+      a.sol_old = u.p;
+      a.flux_old = q.p;
+    }
     if (ts_scheme == HFX_TS_BEULER) {
       a.ts_dt = ts_dt;
       a.ts_idt = 1.0 / ts_dt;

@@ -724,7 +724,20 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     // [U at the face nodes; n.q at the face nodes] (2 NNF x NCOL), i.e.
     // S[(f, aa)][c] = sum_b FMt[aa][b] U[fn_b][c] - FMd[aa][b] QN[(f, b)][c],
     // plus -FMt[aa][bp] + FMb[aa][bp] on the trace columns of face f (Sll);
-    // the F_u column gives S0 = -(...)
+    // the F_u column gives S0 = -(...).  Explicit HDG modes: S = Sll only, and
+    // the F_u column (U0 and Q0 already stored) is replaced by the old
+    // solution and the normal old flux at the face nodes.
+    const bool expl = a.hdg_explicit != 0;
+    if (expl) {
+      for (int i = lane; i < NN; i += 64) UB[i * NCOL + NL] = a.sol_old[(size_t)e * NN + i];
+      for (int t = lane; t < NF * NNF; t += 64) {
+        const int f = t / NNF, node = s_fn[t];
+        double qn = 0.0;
+        for (int k = 0; k < D; ++k) qn += FG[f * (D + 1) + k] * a.flux_old[((size_t)e * NN + node) * D + k];
+        QN[t * NCOL + NL] = qn;
+      }
+      wave_sync();
+    }
     {
       constexpr int NT = (NCOL + 15) / 16, KF = 2 * NNF, KS = (KF + 3) / 4;
       const int li = lane & 15, lk = lane >> 4;
@@ -782,7 +795,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
             for (int r = 0; r < 4; ++r) {
               const int aa = 4 * r + lk;
               if (aa >= NNF) continue;
-              double sv = acc[r];
+              double sv = (expl && trc) ? 0.0 : acc[r];
               if (trc) {
                 if (f == gg) sv += -fmt[aa * NNF + bb] + fmb[aa * NNF + bb];
                 if (fused)  // streaming store (measured faster, the diagonal sides included)

@@ -72,6 +72,13 @@ struct ElemArgs {
   // blocks.  The local-matrix export scales the u rows back by dt.
   double ts_dt = 0.0, ts_idt = 0.0;
   const double* u_old = nullptr;
+  // optional: explicit HDG modes (WEXPLICIT / SEXPLICIT, HDGSolver.cpp:346-353):
+  // S = Sll and S0 = F_lambda - Slu sol_old - Slq flux_old, with sol_old
+  // (nEl x nN, "Solution") and flux_old (nEl x nN x D, "Flux"); U, Q, U0, Q0
+  // as in the implicit mode
+  int hdg_explicit = 0;
+  const double* sol_old = nullptr;
+  const double* flux_old = nullptr;
 };
 
 // Launch the element kernel for (dim, order) on `count` elements starting at

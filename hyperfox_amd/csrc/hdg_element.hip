@@ -466,11 +466,12 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
     }
     __syncthreads();
     // S = Slu U + sum_k Slq_k Q_k + Sll ; S0 = -Slu U0 - sum_k Slq_k Q0_k
+    // (explicit HDG modes: S = Sll, S0 = -Slu sol_old - Slq flux_old)
     double* Se = a.S + (size_t)e * NL * NL;
     for (int t = tid; t < NL * NL; t += NT) {
       const int r = t / NL, c = t % NL, f = r / NNF, aa = r % NNF;
       double s = 0.0;
-      for (int b = 0; b < NNF; ++b) {
+      for (int b = 0; b < NNF && !a.hdg_explicit; ++b) {
         const int j = fn[f * NNF + b];
         s += FM[((f * NS + 0) * NNF + aa) * NNF + b] * AUG2[perm2[j] * NC2 + NN + c];
         for (int k = 0; k < D; ++k) s -= FM[((f * NS + 2 + D + k) * NNF + aa) * NNF + b] * QL[(k * NN + j) * NL + c];
@@ -486,8 +487,12 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
       double s = 0.0;
       for (int b = 0; b < NNF; ++b) {
         const int j = fn[f * NNF + b];
-        s -= FM[((f * NS + 0) * NNF + aa) * NNF + b] * AUG2[perm2[j] * NC2 + NN + NL];
-        for (int k = 0; k < D; ++k) s += FM[((f * NS + 2 + D + k) * NNF + aa) * NNF + b] * Q0L[k * NN + j];
+        const double uj = a.hdg_explicit ? a.sol_old[(size_t)e * NN + j] : AUG2[perm2[j] * NC2 + NN + NL];
+        s -= FM[((f * NS + 0) * NNF + aa) * NNF + b] * uj;
+        for (int k = 0; k < D; ++k) {
+          const double qkj = a.hdg_explicit ? a.flux_old[((size_t)e * NN + j) * D + k] : Q0L[k * NN + j];
+          s += FM[((f * NS + 2 + D + k) * NNF + aa) * NNF + b] * qkj;
+        }
       }
       a.S0[(size_t)e * NL + r] = s;
     }

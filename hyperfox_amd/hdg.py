@@ -198,6 +198,15 @@ class HDGProblem:
         u = _f64(u)
         check(lib.hfx_hdg_set_solution(self._h, _ptr(u)))
 
+    def set_solver_type(self, kind):
+        """HDGSolverOpts::type: 0 IMPLICIT, 1 WEXPLICIT, 2 SEXPLICIT."""
+        check(lib.hfx_hdg_set_solver_type(self._h, kind))
+
+    def set_flux(self, q):
+        """Set the cell-wise nodal flux field (nCells x nN*dim)."""
+        q = _f64(q)
+        check(lib.hfx_hdg_set_flux(self._h, _ptr(q)))
+
     def set_fused_scatter(self, on):
         """Fused trace scatter on straight-sided cells (default on): the
         element kernel writes the trace matrix in place instead of storing S."""

@@ -174,6 +174,19 @@ int hfx_hdg_set_time_scheme(hfx_hdg* h, int scheme, double dt);
  * initial condition; hfx_hdg_solve overwrites it. */
 int hfx_hdg_set_solution(hfx_hdg* h, const double* u);
 
+/* HDGSolverOpts::type (src/solver/HDGSolverOpts.h): IMPLICIT (default),
+ * WEXPLICIT or SEXPLICIT.  The explicit modes condense with S = Sll and
+ * S0 = F_lambda - Slu u - Slq q, u and q being the problem's current
+ * "Solution" and "Flux" fields (HDGSolver.cpp:346-353); WEXPLICIT solves the
+ * resulting block-diagonal trace system with the Krylov method and
+ * preconditioner of hfx_hdg_solve, SEXPLICIT by face-local direct solves
+ * (HDGSolver.cpp:709-729; hfx_hdg_solve then reports one iteration and the
+ * true residual). */
+enum { HFX_HDG_IMPLICIT = 0, HFX_HDG_WEXPLICIT = 1, HFX_HDG_SEXPLICIT = 2 };
+int hfx_hdg_set_solver_type(hfx_hdg* h, int type);
+/* Set the "Flux" field q (nCells x nN x dim, index (i*dim + k)). */
+int hfx_hdg_set_flux(hfx_hdg* h, const double* q);
+
 /* Trace scatter of hfx_hdg_assemble on straight-sided cells: fused (on = 1,
  * the default: the element kernel writes each off-diagonal face block of the
  * trace matrix in place and one face-diagonal pass adds the two cells'

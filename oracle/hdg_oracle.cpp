@@ -645,6 +645,21 @@ void condense(const Ref& R, const double* K, const double* F, double* S, double*
   }
 }
 
+// Explicit HDG modes (HDGSolver.cpp:346-353): S = Sll and
+// S0 = F_lambda - Slu sol - Slq flux with the element's old solution and flux.
+void explicit_blocks(const Ref& R, const double* K, const double* F, const double* sol, const double* flux,
+                     double* S, double* S0) {
+  const int n = R.n(), lu = R.lenU(), lq = R.lenQ(), ll = R.lenL();
+  const int ou = 0, oq = lu, ol = lu + lq;
+  for (int i = 0; i < ll; ++i) {
+    for (int j = 0; j < ll; ++j) S[(size_t)i * ll + j] = K[(size_t)(ol + i) * n + ol + j];
+    double v = F[ol + i];
+    for (int k = 0; k < lu; ++k) v -= K[(size_t)(ol + i) * n + ou + k] * sol[k];
+    for (int k = 0; k < lq; ++k) v -= K[(size_t)(ol + i) * n + oq + k] * flux[k];
+    S0[i] = v;
+  }
+}
+
 void check_dim(int dim, int p) {
   if (dim < 2 || dim > 3 || p < 1 || p > 5) throw std::runtime_error("oracle supports dim 2-3 and order 1-5");
 }
@@ -747,6 +762,14 @@ extern "C" int orc_local_system(int dim, int p, orc_model model, const double* x
   ORC_END
 }
 
+extern "C" int orc_explicit_blocks(int dim, int p, const double* K, const double* F, const double* sol,
+                                   const double* flux, double* S, double* S0) {
+  ORC_TRY
+  check_dim(dim, p);
+  explicit_blocks(ref(dim, p), K, F, sol, flux, S, S0);
+  ORC_END
+}
+
 extern "C" int orc_ip_coords(int dim, int p, const double* x, double* out) {
   ORC_TRY
   check_dim(dim, p);
@@ -781,6 +804,8 @@ struct orc_system {
   std::vector<double> S, S0, U, Q, U0, Q0;
   double ts_dt = 0.0;          // implicit Euler step (0: none)
   std::vector<double> u_old;   // nEl * nN "Solution" field
+  int hdg_type = 0;            // HDGSolverType: 0 IMPLICIT, 1 WEXPLICIT, 2 SEXPLICIT
+  std::vector<double> sol, flux;  // nEl * nN and nEl * nN * dim for the explicit modes
   std::vector<long long> rowptr;
   std::vector<int> cols;
   std::vector<double> vals, rhs;
@@ -820,6 +845,8 @@ void system_assemble(orc_system* s, int nEl, int nthreads) {
       double* S0e = &s->S0[(size_t)e * ll];
       condense(R, K.data(), F.data(), Se, S0e, &s->U[(size_t)e * lu * ll], &s->Q[(size_t)e * lq * ll],
                &s->U0[(size_t)e * lu], &s->Q0[(size_t)e * lq]);
+      if (s->hdg_type != 0)
+        explicit_blocks(R, K.data(), F.data(), &s->sol[(size_t)e * lu], &s->flux[(size_t)e * lq], Se, S0e);
       // Dirichlet (DirichletModel: Set/Set, identity matrix, rhs = g in the
       // face's node order, which is this cell's order on a boundary face)
       for (int f = 0; f < nF; ++f) {
@@ -1018,6 +1045,18 @@ extern "C" orc_system* orc_system_build(int dim, int p, orc_model model, int nNo
 }
 
 extern "C" void orc_system_free(orc_system* s) { delete s; }
+
+extern "C" int orc_system_set_hdg_type(orc_system* s, int type, const double* sol, const double* flux, int nthreads) {
+  ORC_TRY
+  if (type < 0 || type > 2) throw std::runtime_error("HDGSolver: unknown solver type");
+  s->hdg_type = type;
+  if (type != 0) {
+    s->sol.assign(sol, sol + (size_t)s->nEl * s->R->lenU());
+    s->flux.assign(flux, flux + (size_t)s->nEl * s->R->lenQ());
+  }
+  system_assemble(s, s->nEl, nthreads > 0 ? nthreads : omp_get_max_threads());
+  ORC_END
+}
 
 extern "C" int orc_system_set_euler(orc_system* s, double dt, const double* u_old, int nthreads) {
   ORC_TRY

@@ -62,6 +62,11 @@ int orc_local_system(int dim, int p, orc_model model, const double* x, const dou
  * element mass M and M u_old (u_old: nN cell-wise nodal values). */
 int orc_apply_euler(int dim, int p, double dt, const double* u_old, double* K, double* F);
 
+/* Explicit HDG modes (HDGSolver.cpp:346-353) on one local system: S = Sll,
+ * S0 = F_lambda - Slu sol - Slq flux (sol nN, flux nN*dim). */
+int orc_explicit_blocks(int dim, int p, const double* K, const double* F, const double* sol, const double* flux,
+                        double* S, double* S0);
+
 /* Physical coordinates of the element IPs (nIP*dim). */
 int orc_ip_coords(int dim, int p, const double* x, double* out);
 
@@ -90,6 +95,10 @@ void orc_system_free(orc_system* s);
 /* Implicit Euler with step dt (0: none) and "Solution" field u_old
  * (nEl*nN) on every element, then reassemble the trace system. */
 int orc_system_set_euler(orc_system* s, double dt, const double* u_old, int nthreads);
+/* HDGSolverOpts::type 0 IMPLICIT, 1 WEXPLICIT, 2 SEXPLICIT: the explicit modes
+ * set S = Sll and S0 = F_lambda - Slu sol - Slq flux per element (sol nEl*nN,
+ * flux nEl*nN*dim), then reassemble the trace system. */
+int orc_system_set_hdg_type(orc_system* s, int type, const double* sol, const double* flux, int nthreads);
 /* sizes: out[0]=ndofs, out[1]=nnz */
 int orc_system_sizes(const orc_system* s, long long* out);
 int orc_system_csr(const orc_system* s, long long* rowptr, int* cols, double* vals, double* rhs);

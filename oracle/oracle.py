@@ -35,7 +35,9 @@ def _load():
         'orc_ip_coords': ([I, I, P, P], I),
         'orc_condense': ([I, I] + [P] * 8, I),
         'orc_apply_euler': ([I, I, D, P, P, P], I),
+        'orc_explicit_blocks': ([I, I] + [P] * 6, I),
         'orc_system_set_euler': ([P, D, P, I], I),
+        'orc_system_set_hdg_type': ([P, I, P, P, I], I),
         'orc_system_build': ([I, I, Model, I, P, I, P, I, P, P, P, P, P, P, I, P, P, P, I], P),
         'orc_system_free': ([P], None),
         'orc_system_sizes': ([P, P], I),
@@ -109,6 +111,17 @@ def apply_euler(dim, p, dt, u_old, K, F):
     return K2, F2
 
 
+def explicit_blocks(dim, p, K, F, sol, flux):
+    """Explicit HDG modes on one local system: (S, S0) with S = Sll and
+    S0 = F_lambda - Slu sol - Slq flux (HDGSolver.cpp:346-353)."""
+    s = sizes(dim, p)
+    nL = s['lenL']
+    S, S0 = np.zeros((nL, nL)), np.zeros(nL)
+    arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (K, F, sol, flux)]
+    _chk(lib.orc_explicit_blocks(dim, p, *[_p(x) for x in arrs], _p(S), _p(S0)))
+    return S, S0
+
+
 def ip_coords(dim, p, x):
     s = sizes(dim, p)
     out = np.zeros((s['nIP'], dim))
@@ -160,6 +173,14 @@ class System:
         (nCells x nN) on every element, then reassemble."""
         self._uold = None if u_old is None else np.ascontiguousarray(u_old, dtype=np.float64)
         _chk(lib.orc_system_set_euler(self._h, dt, _p(self._uold), nthreads))
+
+    def set_hdg_type(self, kind, sol=None, flux=None, nthreads=0):
+        """HDGSolverType 0 IMPLICIT, 1 WEXPLICIT, 2 SEXPLICIT (explicit modes
+        use the solution field sol (nCells x nN) and flux (nCells x nN*dim)),
+        then reassemble."""
+        self._sol = None if sol is None else np.ascontiguousarray(sol, dtype=np.float64)
+        self._flux = None if flux is None else np.ascontiguousarray(flux, dtype=np.float64)
+        _chk(lib.orc_system_set_hdg_type(self._h, kind, _p(self._sol), _p(self._flux), nthreads))
 
     def csr(self):
         rowptr = np.zeros(self.ndofs + 1, dtype=np.int64)
