@@ -387,6 +387,11 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     {
       constexpr int MT = (NN + 15) / 16, KS = (NIP + 3) / 4, NX = D + 1;
       const int li = lane & 15, lk = lane >> 4;
+      double nfe[NF][D];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int k = 0; k < D; ++k) nfe[f][k] = FG[f * (D + 1) + k];
 #pragma unroll 1
       for (int m = 0; m < MT; ++m) {
         const int ia = m * 16 + li;  // A row this lane supplies
@@ -432,7 +437,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
               if (ai != 0xFF && bj != 0xFF) {
                 const double* F0 = FM + (f * NSF) * NNF * NNF + ai * NNF + bj;
                 au += F0[0];
-                for (int k = 0; k < D; ++k) aq[k] -= FG[f * (D + 1) + k] * F0[2 * NNF * NNF];
+                for (int k = 0; k < D; ++k) aq[k] -= nfe[f][k] * F0[2 * NNF * NNF];
               }
             }
             for (int r2 = 0; r2 < D; ++r2) {
@@ -494,7 +499,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     //   K[p][k] <- 1/d, K[p][j] <- -K[p][j]/d, K[i][k] <- K[i][k]/d,
     //   K[i][j] <- K[i][j] - K[i][k] K[p][j]/d   (i != p, j != k);
     // after all NN pivots, K^-1[k][p(j)] = K[p(k)][j] with p(k) the pivot row
-    // of column k.  Half the row length of [K | I] elimination.
+    // of column k.  Half the row length of [K | I] elimination.  Each lane
+    // updates its row as one branch-free FMA per entry,
+    //   K[i][j] <- mlt K[p][j] + own K[i][j],
+    // with (mlt, own) = (-K[i][k]/d, 1) off the pivot row and (0, -1/d) on it.
     {
       double kr[NN];
       const int rl = lane < NN ? lane : 0;
@@ -514,11 +522,12 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         if (piv) myk = k;
         const double aik = kr[k];
         const double dinv = 1.0 / readlane_d(aik, p);
+        const double mlt = piv ? 0.0 : -aik * dinv;
+        const double own = piv ? -dinv : 1.0;
 #pragma unroll
         for (int j = 0; j < NN; ++j) {
           if (j == k) continue;
-          const double pj = readlane_d(kr[j], p) * dinv;
-          kr[j] = piv ? -pj : kr[j] - aik * pj;
+          kr[j] = fma(mlt, readlane_d(kr[j], p), own * kr[j]);
         }
         kr[k] = piv ? dinv : aik * dinv;
       }
@@ -656,6 +665,13 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     {
       constexpr int MT = (NN + 15) / 16, NT = (NCOL + 15) / 16, KS = (NN + 3) / 4;
       const int li = lane & 15, lk = lane >> 4;
+      // face normals in registers: the QN stores below would otherwise force
+      // the compiler to reload them from LDS after every store
+      double nfk[NF][D];
+#pragma unroll
+      for (int f = 0; f < NF; ++f)
+#pragma unroll
+        for (int k = 0; k < D; ++k) nfk[f][k] = FG[f * (D + 1) + k];
 #pragma unroll 1
       for (int nt = 0; nt < NT; ++nt) {
         const int cc = nt * 16 + li;
@@ -683,6 +699,9 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           const bool trc = cc < NL;
           const int gg = trc ? cc / NNF : 0, bb = trc ? cc % NNF : 0;
           const double rg = FG[gg * (D + 1) + D] / det;
+          double ng[D];
+#pragma unroll
+          for (int k = 0; k < D; ++k) ng[k] = FG[gg * (D + 1) + k];
           double* Qe = trc ? a.Q + (size_t)e * NN * D * NL : a.Q0 + (size_t)e * NN * D;
           double* Ue = trc ? a.U + (size_t)e * NN * NL : a.U0 + (size_t)e * NN;
 #pragma unroll
@@ -696,7 +715,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
               for (int k = 0; k < D; ++k) {
                 double s = 0.0;
                 for (int a2 = 0; a2 < D; ++a2) s -= JI[k * D + a2] * y[m][a2][r];
-                qk[k] = s + FG[gg * (D + 1) + k] * gb;
+                qk[k] = s + ng[k] * gb;
               }
               if (trc) {
                 // streaming stores: U and Q are read again only by the reconstruction
@@ -711,7 +730,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
                 const int pf = (int)(pw >> (8 * f) & 0xFFu);
                 if (pf != 0xFF) {
                   double sq = 0.0;
-                  for (int k = 0; k < D; ++k) sq += FG[f * (D + 1) + k] * qk[k];
+                  for (int k = 0; k < D; ++k) sq += nfk[f][k] * qk[k];
                   QN[(f * NNF + pf) * NCOL + cc] = sq;
                 }
               }
