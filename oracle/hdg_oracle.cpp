@@ -725,11 +725,17 @@ namespace {
 // to the u-u block and M u_old to F_u.  M is the u-u mass of the TimeScheme's
 // Mass operator (Mass.cpp:5-38), which is the first component block of
 // Sqq = I_d (x) M (HDGBase.cpp:139-157) on the same cubature.
-void apply_euler(const Ref& R, double dt, const double* u_old, double* K, double* F) {
+std::vector<double> time_mass(const Ref& R, const double* K) {
   const int n = R.n(), nN = R.nN, d = R.d;
   std::vector<double> M((size_t)nN * nN);
   for (int i = 0; i < nN; ++i)
     for (int j = 0; j < nN; ++j) M[(size_t)i * nN + j] = K[(size_t)(nN + i * d) * n + nN + j * d];
+  return M;
+}
+
+void apply_euler(const Ref& R, double dt, const double* u_old, double* K, double* F) {
+  const int n = R.n(), nN = R.nN;
+  const std::vector<double> M = time_mass(R, K);
   for (int r = 0; r < nN; ++r) {
     for (int c = 0; c < n; ++c) K[(size_t)r * n + c] *= dt;
     F[r] *= dt;
@@ -744,7 +750,83 @@ void apply_euler(const Ref& R, double dt, const double* u_old, double* K, double
   }
 }
 
+// Euler::apply with isExplicit = true (Euler.cpp:30-35): u rows and F_u
+// scaled by dt, then F_u += (M - Suu) u_old with the scaled Suu, and Suu = M.
+void apply_euler_explicit(const Ref& R, double dt, const double* u_old, double* K, double* F) {
+  const int n = R.n(), nN = R.nN;
+  const std::vector<double> M = time_mass(R, K);
+  for (int r = 0; r < nN; ++r) {
+    for (int c = 0; c < n; ++c) K[(size_t)r * n + c] *= dt;
+    F[r] *= dt;
+  }
+  for (int i = 0; i < nN; ++i) {
+    double acc = 0.0;
+    for (int j = 0; j < nN; ++j) acc += (M[(size_t)i * nN + j] - K[(size_t)i * n + j]) * u_old[j];
+    F[i] += acc;
+    for (int j = 0; j < nN; ++j) K[(size_t)i * n + j] = M[(size_t)i * nN + j];
+  }
+}
+
+// RungeKutta::apply (RungeKutta.cpp:113-159) on the u rows [Suu | Suq | Sul]
+// of a local system, for the auxiliary fields {Flux, Trace}: a = row `stage`
+// of the Butcher table without its first column (nst entries), ut = the
+// cell-local [OldSolution | OldFlux | OldTrace] and kst[j * n ..] the
+// cell-local [RKStage_j | RKStage_Flux_j | RKStage_Trace_j], in the local
+// system's column order.  Steps, as the reference takes them:
+//   uj = dt sum_{i<stage} a_i k_i + ut;  K_u *= dt;  F_u *= dt;
+//   F_u -= K_u uj;  K_u *= a_stage;  Suu += M;  F_u += K_u ut.
+void apply_rk(const Ref& R, double dt, int nst, const double* a, int stage, const double* ut, const double* kst,
+              double* K, double* F) {
+  const int n = R.n(), nN = R.nN;
+  const std::vector<double> M = time_mass(R, K);
+  std::vector<double> uj(n, 0.0);
+  for (int i = 0; i < stage; ++i)
+    for (int c = 0; c < n; ++c) uj[c] += a[i] * kst[(size_t)i * n + c];
+  for (int c = 0; c < n; ++c) uj[c] = uj[c] * dt + ut[c];
+  for (int r = 0; r < nN; ++r) {
+    for (int c = 0; c < n; ++c) K[(size_t)r * n + c] *= dt;
+    F[r] *= dt;
+  }
+  for (int r = 0; r < nN; ++r) {
+    double acc = 0.0;
+    for (int c = 0; c < n; ++c) acc += K[(size_t)r * n + c] * uj[c];
+    F[r] -= acc;
+  }
+  for (int r = 0; r < nN; ++r)
+    for (int c = 0; c < n; ++c) K[(size_t)r * n + c] *= a[stage];
+  for (int i = 0; i < nN; ++i)
+    for (int j = 0; j < nN; ++j) K[(size_t)i * n + j] += M[(size_t)i * nN + j];
+  for (int r = 0; r < nN; ++r) {
+    double acc = 0.0;
+    for (int c = 0; c < n; ++c) acc += K[(size_t)r * n + c] * ut[c];
+    F[r] += acc;
+  }
+}
+
+void check_rk(double dt, int nst, int stage) {
+  if (dt == 0.0)
+    throw std::runtime_error("RungeKutta: the time step needs to be set before applying and it should not be 0");
+  if (nst < 1 || stage < 0 || stage >= nst) throw std::runtime_error("RungeKutta: stage out of range");
+}
+
 }  // namespace
+
+extern "C" int orc_apply_euler_explicit(int dim, int p, double dt, const double* u_old, double* K, double* F) {
+  ORC_TRY
+  check_dim(dim, p);
+  if (!(dt > 0.0)) throw std::runtime_error("Euler: the time step needs to be set and it should not be 0");
+  apply_euler_explicit(ref(dim, p), dt, u_old, K, F);
+  ORC_END
+}
+
+extern "C" int orc_apply_rk(int dim, int p, double dt, int nst, const double* a, int stage, const double* ut,
+                            const double* kst, double* K, double* F) {
+  ORC_TRY
+  check_dim(dim, p);
+  check_rk(dt, nst, stage);
+  apply_rk(ref(dim, p), dt, nst, a, stage, ut, kst, K, F);
+  ORC_END
+}
 
 extern "C" int orc_apply_euler(int dim, int p, double dt, const double* u_old, double* K, double* F) {
   ORC_TRY
@@ -802,8 +884,13 @@ struct orc_system {
   std::vector<int> perm;              // nEl*nF*nNf : face2CellMap
   std::vector<char> is_bnd;           // nFaces
   std::vector<double> S, S0, U, Q, U0, Q0;
-  double ts_dt = 0.0;          // implicit Euler step (0: none)
-  std::vector<double> u_old;   // nEl * nN "Solution" field
+  double ts_dt = 0.0;          // time step (0: no time scheme)
+  int ts_kind = 0;             // 1 implicit Euler, 2 explicit Euler, 3 Runge-Kutta stage
+  std::vector<double> u_old;   // nEl * nN "Solution" field (Euler)
+  int rk_nst = 0, rk_stage = 0;
+  std::vector<double> rk_a;    // Butcher row of the stage (nst)
+  std::vector<double> rk_old;  // nEl * n cell-local [OldSolution | OldFlux | OldTrace]
+  std::vector<double> rk_k;    // nEl * nst * n cell-local stage vectors
   int hdg_type = 0;            // HDGSolverType: 0 IMPLICIT, 1 WEXPLICIT, 2 SEXPLICIT
   std::vector<double> sol, flux;  // nEl * nN and nEl * nN * dim for the explicit modes
   std::vector<long long> rowptr;
@@ -840,7 +927,11 @@ void system_assemble(orc_system* s, int nEl, int nthreads) {
       local_system(R, s->model, x.data(), s->hasD ? Dl.data() : nullptr, s->hasB ? bl.data() : nullptr, tl.data(),
                    s->hasF ? &s->f_ip[(size_t)e * R.nip] : nullptr, s->hasR ? &s->r_ip[(size_t)e * R.nip] : nullptr,
                    K.data(), F.data());
-      if (s->ts_dt > 0.0) apply_euler(R, s->ts_dt, &s->u_old[(size_t)e * nN], K.data(), F.data());
+      if (s->ts_kind == 1) apply_euler(R, s->ts_dt, &s->u_old[(size_t)e * nN], K.data(), F.data());
+      else if (s->ts_kind == 2) apply_euler_explicit(R, s->ts_dt, &s->u_old[(size_t)e * nN], K.data(), F.data());
+      else if (s->ts_kind == 3)
+        apply_rk(R, s->ts_dt, s->rk_nst, s->rk_a.data(), s->rk_stage, &s->rk_old[(size_t)e * n],
+                 &s->rk_k[(size_t)e * s->rk_nst * n], K.data(), F.data());
       double* Se = &s->S[(size_t)e * ll * ll];
       double* S0e = &s->S0[(size_t)e * ll];
       condense(R, K.data(), F.data(), Se, S0e, &s->U[(size_t)e * lu * ll], &s->Q[(size_t)e * lq * ll],
@@ -1062,8 +1153,53 @@ extern "C" int orc_system_set_euler(orc_system* s, double dt, const double* u_ol
   ORC_TRY
   if (dt < 0.0) throw std::runtime_error("Euler: negative time step");
   s->ts_dt = dt;
+  s->ts_kind = dt > 0.0 ? 1 : 0;
   if (dt > 0.0) s->u_old.assign(u_old, u_old + (size_t)s->nEl * s->R->nN);
   system_assemble(s, s->nEl, nthreads > 0 ? nthreads : omp_get_max_threads());
+  ORC_END
+}
+
+extern "C" int orc_system_set_time_scheme(orc_system* s, int kind, double dt, int nst, const double* a, int stage,
+                                          const double* u_old, const double* old_local, const double* stage_local,
+                                          int nthreads) {
+  ORC_TRY
+  const int n = s->R->n();
+  if (kind < 0 || kind > 3) throw std::runtime_error("time scheme: unknown kind");
+  if (kind != 0 && !(dt > 0.0)) throw std::runtime_error("time scheme: the time step should be > 0");
+  s->ts_kind = kind;
+  s->ts_dt = kind ? dt : 0.0;
+  if (kind == 1 || kind == 2) s->u_old.assign(u_old, u_old + (size_t)s->nEl * s->R->nN);
+  if (kind == 3) {
+    check_rk(dt, nst, stage);
+    s->rk_nst = nst;
+    s->rk_stage = stage;
+    s->rk_a.assign(a, a + nst);
+    s->rk_old.assign(old_local, old_local + (size_t)s->nEl * n);
+    s->rk_k.assign((size_t)s->nEl * nst * n, 0.0);
+    for (int j = 0; j < nst; ++j)
+      for (int e = 0; e < s->nEl; ++e)
+        std::copy(stage_local + ((size_t)j * s->nEl + e) * n, stage_local + ((size_t)j * s->nEl + e + 1) * n,
+                  &s->rk_k[((size_t)e * nst + j) * n]);
+  }
+  system_assemble(s, s->nEl, nthreads > 0 ? nthreads : omp_get_max_threads());
+  ORC_END
+}
+
+extern "C" int orc_system_local_vectors(const orc_system* s, const double* u, const double* q, const double* lambda,
+                                        double* out) {
+  ORC_TRY
+  const Ref& R = *s->R;
+  const int n = R.n(), nF = R.nF, nNf = R.nNf, lu = R.lenU(), lq = R.lenQ();
+  for (int e = 0; e < s->nEl; ++e) {
+    double* o = out + (size_t)e * n;
+    for (int i = 0; i < lu; ++i) o[i] = u[(size_t)e * lu + i];
+    for (int i = 0; i < lq; ++i) o[lu + i] = q[(size_t)e * lq + i];
+    for (int f = 0; f < nF; ++f) {
+      const int fc = s->c2f[(size_t)e * nF + f];
+      for (int j = 0; j < nNf; ++j)
+        o[lu + lq + f * nNf + j] = lambda[(size_t)fc * nNf + s->perm[((size_t)e * nF + f) * nNf + j]];
+    }
+  }
   ORC_END
 }
 

@@ -61,6 +61,16 @@ int orc_local_system(int dim, int p, orc_model model, const double* x, const dou
  * isExplicit = false, Euler.cpp:18-37): u rows and F_u times dt, plus the
  * element mass M and M u_old (u_old: nN cell-wise nodal values). */
 int orc_apply_euler(int dim, int p, double dt, const double* u_old, double* K, double* F);
+/* Explicit Euler (Euler::apply, isExplicit = true, Euler.cpp:30-35): u rows
+ * and F_u times dt, F_u += (M - Suu) u_old, then Suu = M. */
+int orc_apply_euler_explicit(int dim, int p, double dt, const double* u_old, double* K, double* F);
+/* One Runge-Kutta stage on the u rows (RungeKutta::apply,
+ * RungeKutta.cpp:113-159) with auxiliary fields {Flux, Trace}: a = Butcher
+ * row `stage` without its first column (nst entries), ut = cell-local
+ * [OldSolution | OldFlux | OldTrace] (n), kst = nst cell-local stage vectors
+ * (nst x n). */
+int orc_apply_rk(int dim, int p, double dt, int nst, const double* a, int stage, const double* ut, const double* kst,
+                 double* K, double* F);
 
 /* Explicit HDG modes (HDGSolver.cpp:346-353) on one local system: S = Sll,
  * S0 = F_lambda - Slu sol - Slq flux (sol nN, flux nN*dim). */
@@ -99,6 +109,16 @@ int orc_system_set_euler(orc_system* s, double dt, const double* u_old, int nthr
  * set S = Sll and S0 = F_lambda - Slu sol - Slq flux per element (sol nEl*nN,
  * flux nEl*nN*dim), then reassemble the trace system. */
 int orc_system_set_hdg_type(orc_system* s, int type, const double* sol, const double* flux, int nthreads);
+/* Time scheme on every element, then reassemble: kind 0 none, 1 implicit
+ * Euler, 2 explicit Euler (u_old nEl*nN), 3 Runge-Kutta stage `stage` with
+ * Butcher row a (nst), old_local (nEl x n) and stage_local (nst x nEl x n)
+ * cell-local vectors as orc_apply_rk takes them. */
+int orc_system_set_time_scheme(orc_system* s, int kind, double dt, int nst, const double* a, int stage,
+                               const double* u_old, const double* old_local, const double* stage_local, int nthreads);
+/* Cell-local vectors [u | q | lambda] (nEl x n) of the fields u (nEl*nN),
+ * q (nEl*nN*dim) and lambda (nFaces*nNf), the trace in each cell's face-node
+ * order (Solver::prepareLocalFieldMap, Solver.cpp:14-57). */
+int orc_system_local_vectors(const orc_system* s, const double* u, const double* q, const double* lambda, double* out);
 /* sizes: out[0]=ndofs, out[1]=nnz */
 int orc_system_sizes(const orc_system* s, long long* out);
 int orc_system_csr(const orc_system* s, long long* rowptr, int* cols, double* vals, double* rhs);

@@ -38,6 +38,10 @@ def _load():
         'orc_explicit_blocks': ([I, I] + [P] * 6, I),
         'orc_system_set_euler': ([P, D, P, I], I),
         'orc_system_set_hdg_type': ([P, I, P, P, I], I),
+        'orc_apply_euler_explicit': ([I, I, D, P, P, P], I),
+        'orc_apply_rk': ([I, I, D, I, P, I, P, P, P, P], I),
+        'orc_system_set_time_scheme': ([P, I, D, I, P, I, P, P, P, I], I),
+        'orc_system_local_vectors': ([P] * 5, I),
         'orc_system_build': ([I, I, Model, I, P, I, P, I, P, P, P, P, P, P, I, P, P, P, I], P),
         'orc_system_free': ([P], None),
         'orc_system_sizes': ([P, P], I),
@@ -111,6 +115,102 @@ def apply_euler(dim, p, dt, u_old, K, F):
     return K2, F2
 
 
+def apply_euler_explicit(dim, p, dt, u_old, K, F):
+    """Explicit Euler on the u rows of a local system (Euler::apply with
+    isExplicit = true, Euler.cpp:30-35); returns new (K, F)."""
+    K2, F2 = np.array(K, dtype=np.float64, order='C'), np.array(F, dtype=np.float64)
+    _chk(lib.orc_apply_euler_explicit(dim, p, dt, _p(np.ascontiguousarray(u_old, dtype=np.float64)), _p(K2), _p(F2)))
+    return K2, F2
+
+
+def apply_rk(dim, p, dt, a_row, stage, ut, kst, K, F):
+    """One Runge-Kutta stage on the u rows of a local system
+    (RungeKutta::apply, RungeKutta.cpp:113-159, auxiliary fields Flux and
+    Trace): a_row = Butcher row `stage` without its first column, ut = the
+    cell-local old vector [u | q | lambda], kst = (nst, n) stage vectors;
+    returns new (K, F)."""
+    a = np.ascontiguousarray(a_row, dtype=np.float64)
+    K2, F2 = np.array(K, dtype=np.float64, order='C'), np.array(F, dtype=np.float64)
+    ut = np.ascontiguousarray(ut, dtype=np.float64)
+    kst = np.ascontiguousarray(kst, dtype=np.float64)
+    _chk(lib.orc_apply_rk(dim, p, dt, a.size, _p(a), stage, _p(ut), _p(kst), _p(K2), _p(F2)))
+    return K2, F2
+
+
+# Butcher tables of RungeKutta::setUpDB (RungeKutta.cpp:197-285), in RKType
+# order (RKType.h): row i < nst is [c_i | a_i,0 .. a_i,nst-1], the last row
+# [0 | b_0 .. b_nst-1].
+_G = 1.0 - np.sqrt(2.0) / 2.0
+RK_TABLES = {
+    'FEuler': [[0, 0], [0, 1]],
+    'EMidpoint': [[0, 0, 0], [0.5, 0.5, 0], [0, 0, 1]],
+    'Heun': [[0, 0, 0], [1, 1, 0], [0, 0.5, 0.5]],
+    'Kutta3': [[0, 0, 0, 0], [0.5, 0.5, 0, 0], [1, -1, 2, 0], [0, 1 / 6, 2 / 3, 1 / 6]],
+    'Heun3': [[0, 0, 0, 0], [1 / 3, 1 / 3, 0, 0], [2 / 3, 0, 2 / 3, 0], [0, 1 / 4, 0, 3 / 4]],
+    'SSPRK3': [[0, 0, 0, 0], [1, 1, 0, 0], [1 / 2, 1 / 4, 1 / 4, 0], [0, 1 / 6, 1 / 6, 2 / 3]],
+    'RK4': [[0, 0, 0, 0, 0], [1 / 2, 1 / 2, 0, 0, 0], [1 / 2, 0, 1 / 2, 0, 0], [1, 0, 0, 1, 0],
+            [0, 1 / 6, 1 / 3, 1 / 3, 1 / 6]],
+    'BEuler': [[1, 1], [0, 1]],
+    'IMidpoint': [[1 / 2, 1 / 2], [0, 1]],
+    'CrankNicolson': [[0, 0, 0], [1, 1 / 2, 1 / 2], [0, 1 / 2, 1 / 2]],
+    'KS2': [[1 / 2, 1 / 2, 0], [3 / 2, -1 / 2, 2], [0, -1 / 2, 3 / 2]],
+    'QZ2': [[1 / 4, 1 / 4, 0], [3 / 4, 1 / 2, 1 / 4], [0, 1 / 2, 1 / 2]],
+    'ALX2': [[_G, _G, 0], [1, 1 - _G, _G], [0, 1 - _G, _G]],
+    'RK43': [[1 / 2, 1 / 2, 0, 0, 0], [2 / 3, 1 / 6, 1 / 2, 0, 0], [1 / 2, -1 / 2, 1 / 2, 1 / 2, 0],
+             [1, 3 / 2, -3 / 2, 1 / 2, 1 / 2], [0, 3 / 2, -3 / 2, 1 / 2, 1 / 2]],
+}
+RK_NAMES = list(RK_TABLES)
+
+
+class RungeKutta:
+    """Stage bookkeeping of RungeKutta (RungeKutta.cpp:161-253) over whole-mesh
+    fields: fields = {'Solution': u, 'Flux': q, 'Trace': lambda} (numpy
+    arrays updated in place), with Old* copies and per-stage RKStage arrays."""
+
+    def __init__(self, table, dt):
+        if isinstance(table, str):  # setButcherTable(RKType): a table of the database
+            t = np.asarray(RK_TABLES[table], dtype=np.float64)
+        else:  # setButcherTable(EMatrix) and its checks (RungeKutta.cpp:17-31)
+            t = np.asarray(table, dtype=np.float64)
+            if t.ndim != 2 or t.shape[0] != t.shape[1]:
+                raise OracleError('RungeKutta: the Butcher table should be square')
+            if np.any(t[np.triu_indices(t.shape[0], 1)] != 0):
+                raise OracleError('RungeKutta: the upper triangular part of the Butcher table should be null')
+        self.table, self.dt = t, float(dt)
+        self.nst = t.shape[0] - 1
+        self.stage = 0
+
+    def a_row(self, s):
+        return self.table[s, 1:]
+
+    def start_step(self, fields):
+        self.old = {k: np.array(v, dtype=np.float64) for k, v in fields.items()}
+        self.k = {k: np.zeros((self.nst,) + np.shape(v)) for k, v in fields.items()}
+        self.stage = 0
+
+    def compute_stage(self, fields):
+        if self.stage >= self.nst:
+            raise OracleError('RungeKutta: cannot compute more stages than the method allows')
+        a = self.a_row(self.stage)
+        for name, cur in fields.items():
+            self.k[name][self.stage] = (cur - self.old[name]) / self.dt
+            buf = np.zeros_like(cur)
+            for j in range(self.stage + 1):
+                buf += a[j] * self.k[name][j]
+            cur[...] = self.old[name] + self.dt * buf
+        self.stage += 1
+
+    def compute_solution(self, fields):
+        if self.stage != self.nst:
+            raise OracleError('RungeKutta: all stages must be computed before computing the solution')
+        b = self.table[self.nst, 1:]
+        for name, cur in fields.items():
+            cur[...] = self.old[name]
+            for j in range(self.nst):
+                cur += (b[j] * self.dt) * self.k[name][j]
+        self.stage = 0
+
+
 def explicit_blocks(dim, p, K, F, sol, flux):
     """Explicit HDG modes on one local system: (S, S0) with S = Sll and
     S0 = F_lambda - Slu sol - Slq flux (HDGSolver.cpp:346-353)."""
@@ -181,6 +281,25 @@ class System:
         self._sol = None if sol is None else np.ascontiguousarray(sol, dtype=np.float64)
         self._flux = None if flux is None else np.ascontiguousarray(flux, dtype=np.float64)
         _chk(lib.orc_system_set_hdg_type(self._h, kind, _p(self._sol), _p(self._flux), nthreads))
+
+    def set_time_scheme(self, kind, dt=0.0, a_row=None, stage=0, u_old=None, old_local=None, stage_local=None,
+                        nthreads=0):
+        """Time scheme on every element, then reassemble: kind 0 none,
+        1 implicit Euler, 2 explicit Euler (u_old), 3 Runge-Kutta stage
+        `stage` with Butcher row a_row, old_local (nCells x n) and
+        stage_local (nst x nCells x n) cell-local vectors."""
+        self._ts = [None if v is None else np.ascontiguousarray(v, dtype=np.float64)
+                    for v in (a_row, u_old, old_local, stage_local)]
+        a, uo, ol, sl = self._ts
+        nst = 0 if a is None else a.size
+        _chk(lib.orc_system_set_time_scheme(self._h, kind, dt, nst, _p(a), stage, _p(uo), _p(ol), _p(sl), nthreads))
+
+    def local_vectors(self, u, q, lam):
+        """Cell-local [u | q | lambda] vectors (nCells x n) of whole-mesh fields."""
+        out = np.zeros((self.n_cells, self.s['n_local']))
+        arrs = [np.ascontiguousarray(v, dtype=np.float64) for v in (u, q, lam)]
+        _chk(lib.orc_system_local_vectors(self._h, *[_p(x) for x in arrs], _p(out)))
+        return out
 
     def csr(self):
         rowptr = np.zeros(self.ndofs + 1, dtype=np.int64)
