@@ -7,6 +7,7 @@ for the tests and the benchmark; it has no CPU fallback: importing
 """
 from ._lib import lib, LIB_PATH, HfxError  # noqa: F401
 from .hdg import Mesh, HDGProblem, Batch, refel_sizes, refel_tables, cubature, ilu_analyse  # noqa: F401
+from .hdg import RK_NAMES, RK_CUSTOM, rk_table  # noqa: F401
 from .linalg import HipLinAlgebra, HipOpts  # noqa: F401
 
 MODEL_LAPLACE = 0
@@ -15,5 +16,5 @@ MODEL_CDR_SOURCE = 2
 PC_NONE, PC_JACOBI, PC_BJACOBI, PC_LEARNED, PC_ILU0, PC_ILU0_NATURAL = 0, 1, 2, 3, 4, 5
 ILU_MULTICOLOUR, ILU_NATURAL = 0, 1
 KRYLOV_FGMRES, KRYLOV_BICGSTAB = 0, 1
-TS_NONE, TS_BEULER, TS_FEULER = 0, 1, 2
+TS_NONE, TS_BEULER, TS_FEULER, TS_RK = 0, 1, 2, 3
 HDG_IMPLICIT, HDG_WEXPLICIT, HDG_SEXPLICIT = 0, 1, 2

@@ -53,6 +53,13 @@ _SIGS = {
     'hfx_hdg_set_solution': ([_P, _P], _I),
     'hfx_hdg_set_solver_type': ([_P, _I], _I),
     'hfx_hdg_set_flux': ([_P, _P], _I),
+    'hfx_hdg_set_butcher': ([_P, _I, _P, _I], _I),
+    'hfx_rk_table': ([_I, _P, _P], _I),
+    'hfx_hdg_rk_start_step': ([_P], _I),
+    'hfx_hdg_rk_compute_stage': ([_P], _I),
+    'hfx_hdg_rk_compute_solution': ([_P], _I),
+    'hfx_hdg_rk_stages': ([_P, _P], _I),
+    'hfx_hdg_set_trace': ([_P, _P], _I),
     'hfx_hdg_ilu_factors': ([_P, _I, _P, _P, _P], _I),
     'hfx_ilu_analyse': ([_I, _P, _P, _I, _P, _P, _P, _P, _P], _I),
     'hfx_hdg_get_solution': ([_P] * 4, _I),

@@ -282,6 +282,8 @@ extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
   h->mesh_serial = ++serial;
   h->lpc_state.reset();
   h->ilu.reset();
+  h->rk_started = false;  // Runge-Kutta fields are sized for the mesh
+  h->rk_stage = 0;
   h->mesh = mm->m;
   h->mesh.order = h->order;
   h->affine = mesh_is_affine(*h->re, h->mesh);
@@ -433,6 +435,7 @@ extern "C" int hfx_hdg_local_matrices(hfx_hdg* h, int first, int count, double* 
   hfx::DevBuf<double> dK, dF;
   dK.alloc((size_t)count * n * n);
   dF.alloc((size_t)count * n);
+  h->prepare_time_scheme();
   hfx::ElemArgs a = h->elem_args();
   a.el_offset = first;
   a.nEl = count;
@@ -572,15 +575,162 @@ void do_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, hfx::Kry
 extern "C" int hfx_hdg_set_time_scheme(hfx_hdg* h, int scheme, double dt) {
   HFX_TRY
   require(h != nullptr, "hfx_hdg_set_time_scheme: null handle");
-  require(scheme == HFX_TS_NONE || scheme == HFX_TS_BEULER || scheme == HFX_TS_FEULER,
+  require(scheme == HFX_TS_NONE || scheme == HFX_TS_BEULER || scheme == HFX_TS_FEULER || scheme == HFX_TS_RK,
           "hfx_hdg_set_time_scheme: unknown scheme");
-  require(scheme != HFX_TS_FEULER, "hfx_hdg_set_time_scheme: explicit Euler is not supported on the GPU path");
   require(scheme == HFX_TS_NONE || (dt > 0.0 && std::isfinite(dt)),
           "hfx_hdg_set_time_scheme: the time step needs to be set and it should not be 0");
   h->ts_scheme = scheme;
   h->ts_dt = scheme == HFX_TS_NONE ? 0.0 : dt;
   h->assembled = h->solved = false;
   ++h->args_version;
+  HFX_END
+}
+
+namespace {
+
+// Butcher tables of RungeKutta::setUpDB (RungeKutta.cpp:197-285) in RKType
+// order: row i < n-1 is [c_i | a_i0 ..], the last row [0 | b_0 ..].
+std::vector<double> rk_db(int type, int* n) {
+  const double g = 1.0 - std::sqrt(2.0) / 2.0;
+  switch (type) {
+    case HFX_RK_FEULER: *n = 2; return {0, 0, 0, 1};
+    case HFX_RK_EMIDPOINT: *n = 3; return {0, 0, 0, 0.5, 0.5, 0, 0, 0, 1};
+    case HFX_RK_HEUN: *n = 3; return {0, 0, 0, 1, 1, 0, 0, 0.5, 0.5};
+    case HFX_RK_KUTTA3: *n = 4; return {0, 0, 0, 0, 0.5, 0.5, 0, 0, 1, -1, 2, 0, 0, 1.0 / 6, 2.0 / 3, 1.0 / 6};
+    case HFX_RK_HEUN3: *n = 4; return {0, 0, 0, 0, 1.0 / 3, 1.0 / 3, 0, 0, 2.0 / 3, 0, 2.0 / 3, 0, 0, 0.25, 0, 0.75};
+    case HFX_RK_SSPRK3: *n = 4; return {0, 0, 0, 0, 1, 1, 0, 0, 0.5, 0.25, 0.25, 0, 0, 1.0 / 6, 1.0 / 6, 2.0 / 3};
+    case HFX_RK_RK4:
+      *n = 5;
+      return {0, 0, 0, 0, 0, 0.5, 0.5, 0, 0, 0, 0.5, 0, 0.5, 0, 0, 1, 0, 0, 1, 0, 0, 1.0 / 6, 1.0 / 3, 1.0 / 3, 1.0 / 6};
+    case HFX_RK_BEULER: *n = 2; return {1, 1, 0, 1};
+    case HFX_RK_IMIDPOINT: *n = 2; return {0.5, 0.5, 0, 1};
+    case HFX_RK_CRANKNICOLSON: *n = 3; return {0, 0, 0, 1, 0.5, 0.5, 0, 0.5, 0.5};
+    case HFX_RK_KS2: *n = 3; return {0.5, 0.5, 0, 1.5, -0.5, 2, 0, -0.5, 1.5};
+    case HFX_RK_QZ2: *n = 3; return {0.25, 0.25, 0, 0.75, 0.5, 0.25, 0, 0.5, 0.5};
+    case HFX_RK_ALX2: *n = 3; return {g, g, 0, 1, 1 - g, g, 0, 1 - g, g};
+    case HFX_RK_RK43:
+      *n = 5;
+      return {0.5, 0.5, 0, 0, 0, 2.0 / 3, 1.0 / 6, 0.5, 0, 0, 0.5, -0.5, 0.5, 0.5, 0, 1, 1.5, -1.5, 0.5, 0.5, 0, 1.5,
+              -1.5, 0.5, 0.5};
+    default: throw std::runtime_error("RungeKutta: unknown Butcher table");
+  }
+}
+
+hfx::RkCoef rk_row(const hfx_hdg* h, int r) {
+  hfx::RkCoef c{};
+  for (int j = 0; j < h->rk_stages(); ++j) c.c[j] = h->rk_entry(r, 1 + j);
+  return c;
+}
+
+}  // namespace
+
+extern "C" int hfx_rk_table(int type, double* table, int* n) {
+  HFX_TRY
+  require(n != nullptr, "hfx_rk_table: null size");
+  const std::vector<double> t = rk_db(type, n);
+  if (table) std::copy(t.begin(), t.end(), table);
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_butcher(hfx_hdg* h, int type, const double* table, int n) {
+  HFX_TRY
+  require(h != nullptr, "hfx_hdg_set_butcher: null handle");
+  std::vector<double> t;
+  int nn = 0;
+  if (type == HFX_RK_CUSTOM) {
+    require(table != nullptr && n >= 2, "hfx_hdg_set_butcher: a custom table needs at least 2 x 2 values");
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j)
+        require(table[(size_t)i * n + j] == 0.0,
+                "RungeKutta: the upper triangular part of the Butcher table should be null (no fully implicit "
+                "implementation as of yet)");
+    t.assign(table, table + (size_t)n * n);
+    nn = n;
+  } else {
+    t = rk_db(type, &nn);
+  }
+  require(nn - 1 <= hfx::RK_MAX_STAGES, "hfx_hdg_set_butcher: at most 8 stages");
+  for (double v : t) require(std::isfinite(v), "hfx_hdg_set_butcher: table entries must be finite");
+  h->rk_table = t;
+  h->rk_n = nn;
+  h->rk_stage = 0;
+  h->rk_started = false;
+  h->assembled = h->solved = false;
+  ++h->args_version;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_rk_start_step(hfx_hdg* h) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_rk_start_step: set the mesh first");
+  require(h->rk_n >= 2, "hfx_hdg_rk_start_step: set a Butcher table first");
+  h->set_device();
+  const int nst = h->rk_stages();
+  h->rk_old_u.alloc(h->u.n);
+  h->rk_old_q.alloc(h->q.n);
+  h->rk_old_l.alloc(h->lambda.n);
+  h->rk_ku.alloc(h->u.n * nst);
+  h->rk_kq.alloc(h->q.n * nst);
+  h->rk_kl.alloc(h->lambda.n * nst);
+  HFX_HIP(hipMemcpyAsync(h->rk_old_u.p, h->u.p, h->u.n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  HFX_HIP(hipMemcpyAsync(h->rk_old_q.p, h->q.p, h->q.n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+  HFX_HIP(hipMemcpyAsync(h->rk_old_l.p, h->lambda.p, h->lambda.n * sizeof(double), hipMemcpyDeviceToDevice,
+                         h->stream));
+  HFX_HIP(hipMemsetAsync(h->rk_ku.p, 0, h->rk_ku.n * sizeof(double), h->stream));
+  HFX_HIP(hipMemsetAsync(h->rk_kq.p, 0, h->rk_kq.n * sizeof(double), h->stream));
+  HFX_HIP(hipMemsetAsync(h->rk_kl.p, 0, h->rk_kl.n * sizeof(double), h->stream));
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  h->rk_stage = 0;
+  h->rk_started = true;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_rk_compute_stage(hfx_hdg* h) {
+  HFX_TRY
+  require(h && h->rk_started, "hfx_hdg_rk_compute_stage: start the time step first");
+  require(h->rk_stage < h->rk_stages(),
+          "RungeKutta: cannot compute more stages than the method allows, think about computing the solution");
+  h->set_device();
+  const hfx::RkCoef a = rk_row(h, h->rk_stage);
+  hfx::launch_rk_stage(h->u.n, h->u.p, h->rk_old_u.p, h->rk_ku.p, h->rk_stage, a, h->ts_dt, h->stream);
+  hfx::launch_rk_stage(h->q.n, h->q.p, h->rk_old_q.p, h->rk_kq.p, h->rk_stage, a, h->ts_dt, h->stream);
+  hfx::launch_rk_stage(h->lambda.n, h->lambda.p, h->rk_old_l.p, h->rk_kl.p, h->rk_stage, a, h->ts_dt, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  ++h->rk_stage;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_rk_compute_solution(hfx_hdg* h) {
+  HFX_TRY
+  require(h && h->rk_started, "hfx_hdg_rk_compute_solution: start the time step first");
+  require(h->rk_stage == h->rk_stages(), "RungeKutta: all stages must be computed before computing the solution");
+  h->set_device();
+  const int nst = h->rk_stages();
+  const hfx::RkCoef b = rk_row(h, nst);
+  hfx::launch_rk_solution(h->u.n, h->u.p, h->rk_old_u.p, h->rk_ku.p, nst, b, h->ts_dt, h->stream);
+  hfx::launch_rk_solution(h->q.n, h->q.p, h->rk_old_q.p, h->rk_kq.p, nst, b, h->ts_dt, h->stream);
+  hfx::launch_rk_solution(h->lambda.n, h->lambda.p, h->rk_old_l.p, h->rk_kl.p, nst, b, h->ts_dt, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  h->rk_stage = 0;
+  h->rk_started = false;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_rk_stages(hfx_hdg* h, int* out) {
+  HFX_TRY
+  require(h && out, "hfx_hdg_rk_stages: null argument");
+  out[0] = h->rk_n ? h->rk_stages() : 0;
+  out[1] = h->rk_stage;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_trace(hfx_hdg* h, const double* lambda) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_set_trace: set the mesh first");
+  require(lambda != nullptr, "hfx_hdg_set_trace: null field");
+  h->set_device();
+  h->lambda.upload(lambda, h->lambda.n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
   HFX_END
 }
 

@@ -98,6 +98,8 @@ struct hfx_batch {
     fused.clear();
     versions.clear();
     for (hfx_hdg* h : probs) {
+      if (h->ts_scheme == HFX_TS_FEULER || h->ts_scheme == HFX_TS_RK)
+        throw std::runtime_error("batched assembly supports implicit Euler only among the time schemes");
       args.push_back(h->affine_args(false));
       fused.push_back(h->fuse_scatter);
       versions.push_back(h->args_version);

@@ -16,6 +16,7 @@
 #include "lpc.h"
 #include "mesh.h"
 #include "refel.h"
+#include "time_scheme.h"
 #include "trace_system.h"
 
 namespace hfx {
@@ -69,6 +70,13 @@ struct hfx_hdg {
   // problem's current "Solution" field u
   int ts_scheme = HFX_TS_NONE;
   double ts_dt = 0.0;
+  // Runge-Kutta (hfx_hdg_set_butcher, hfx_hdg_rk_*): Butcher table
+  // (rk_n x rk_n, row-major), stage counter, Old* fields and stage fields of
+  // Solution / Flux / Trace, and the cell-local vectors of the element kernels
+  std::vector<double> rk_table;
+  int rk_n = 0, rk_stage = 0;
+  bool rk_started = false;
+  hfx::DevBuf<double> rk_old_u, rk_old_q, rk_old_l, rk_ku, rk_kq, rk_kl, ts_w;
   // HDGSolverOpts::type (hfx_hdg_set_solver_type); the explicit modes read the
   // problem's "Solution" u and "Flux" q fields at assembly
   int hdg_type = HFX_HDG_IMPLICIT;
@@ -123,7 +131,70 @@ struct hfx_hdg {
       a.ts_idt = 1.0 / ts_dt;
       a.u_old = u.p;
     }
+    if (ts_scheme == HFX_TS_FEULER || ts_scheme == HFX_TS_RK) {
+      // the kernels read w (nCells x NLOC) and u_old for every cell
+      const size_t nloc = (size_t)mesh.nN * (dim + 1) + (size_t)mesh.nFpc * mesh.nNf;
+      const hfx::DevBuf<double>& uo = ts_scheme == HFX_TS_RK ? rk_old_u : u;
+      if (!ts_w.p || ts_w.n != (size_t)mesh.nCells * nloc || !uo.p || uo.n != (size_t)mesh.nCells * mesh.nN)
+        throw std::runtime_error("time scheme: the local vectors are not prepared for this mesh");
+    }
+    if (ts_scheme == HFX_TS_FEULER) {
+      // Euler, isExplicit = true (Euler.cpp:30-35), rows divided by dt:
+      // Suu = M / dt, [Suq | Sul] kept, F_u + M u / dt - Suu u
+      a.ts_gen = 1;
+      a.ts_auu = 0.0;
+      a.ts_arest = 1.0;
+      a.ts_mu = 1.0 / ts_dt;
+      a.ts_phi = 1.0;
+      a.ts_dt = ts_dt;
+      a.u_old = u.p;
+      a.ts_w = ts_w.p;
+    } else if (ts_scheme == HFX_TS_RK) {
+      // RungeKutta::apply (RungeKutta.cpp:113-159) at stage s with
+      // a = a_ss: rows divided by a dt (a != 0) or dt (a = 0, where the
+      // model's rows drop out):  Suu + M / (a dt), F_u / a + M u_old / (a dt)
+      // - [Suu | Suq | Sul] w  resp.  M / dt, F_u + M u_old / dt - ... w
+      if (rk_n < 2 || rk_stage >= rk_stages()) throw std::runtime_error("RungeKutta: no stage to assemble");
+      const double ass = rk_diag();
+      a.ts_gen = 1;
+      a.ts_auu = a.ts_arest = ass != 0.0 ? 1.0 : 0.0;
+      a.ts_mu = ass != 0.0 ? 1.0 / (ass * ts_dt) : 1.0 / ts_dt;
+      a.ts_phi = ass != 0.0 ? 1.0 / ass : 1.0;
+      a.ts_dt = ass != 0.0 ? ass * ts_dt : ts_dt;
+      a.u_old = rk_old_u.p;
+      a.ts_w = ts_w.p;
+    }
     return a;
+  }
+  int rk_stages() const { return rk_n - 1; }
+  double rk_entry(int r, int c) const { return rk_table[(size_t)r * rk_n + c]; }
+  double rk_diag() const { return rk_entry(rk_stage, 1 + rk_stage); }
+  // cell-local vectors w of the general time transform, before an element
+  // launch: explicit Euler w = [u | 0 | 0]; Runge-Kutta stage s with
+  // a = a_ss, ut the Old* fields and uj = dt sum_{j<s} a_sj k_j:
+  // w = ((1 - a) ut + uj) / a (a != 0), ut + uj (a = 0)
+  void prepare_time_scheme() {
+    if (ts_scheme != HFX_TS_FEULER && ts_scheme != HFX_TS_RK) return;
+    const hfx::Mesh& m = mesh;
+    const int nloc = m.nN * (dim + 1) + m.nFpc * m.nNf;
+    ts_w.alloc((size_t)m.nCells * nloc);
+    hfx::RkCoef ck{};
+    if (ts_scheme == HFX_TS_FEULER) {
+      hfx::launch_ts_local(m.nCells, m.nN, dim, m.nFpc, m.nNf, c2f.p, perm.p, 1.0, u.p, nullptr, nullptr, 0, ck,
+                           nullptr, nullptr, nullptr, 0, 0, 0, ts_w.p, stream);
+      return;
+    }
+    if (rk_n < 2) throw std::runtime_error("RungeKutta: set a Butcher table before assembling");
+    if (!rk_started)
+      throw std::runtime_error("RungeKutta: start the time step (hfx_hdg_rk_start_step) before assembling a stage");
+    if (rk_stage >= rk_stages())
+      throw std::runtime_error("RungeKutta: every stage is computed; compute the solution");
+    const double ass = rk_diag();
+    const double sc = ass != 0.0 ? 1.0 / ass : 1.0;
+    for (int j = 0; j < rk_stage; ++j) ck.c[j] = ts_dt * rk_entry(rk_stage, 1 + j) * sc;
+    hfx::launch_ts_local(m.nCells, m.nN, dim, m.nFpc, m.nNf, c2f.p, perm.p, ass != 0.0 ? (1.0 - ass) / ass : 1.0,
+                         rk_old_u.p, rk_old_q.p, rk_old_l.p, rk_stage, ck, rk_ku.p, rk_kq.p, rk_kl.p, u.n, q.n,
+                         lambda.n, ts_w.p, stream);
   }
   int threads() const {
     if (nt) return nt;
@@ -155,6 +226,7 @@ struct hfx_hdg {
   }
   // element kernel; store_S forces the stored-S path (exports)
   void run_element(bool store_S = false) {
+    prepare_time_scheme();
     if (use_affine()) {
       const hfx::ElemArgs a = affine_args(store_S);
       hfx::launch_affine(dim, order, false, affine_waves(), a, grid_cap, stream);

@@ -221,7 +221,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     }
     for (int t = lane; t < NIP; t += 64) {
       double fq = (a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + t] : 0.0;
-      if (a.u_old) {  // implicit Euler: + u_old(x_q) / dt
+      if (a.u_old && a.ts_idt != 0.0) {  // implicit Euler: + u_old(x_q) / dt
         double uo = 0.0;
         for (int j = 0; j < NN; ++j) uo += PHI[t * NN + j] * a.u_old[(size_t)e * NN + j];
         fq += a.ts_idt * uo;
@@ -451,6 +451,45 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       }
     }
     wave_sync();
+    if (a.ts_gen) {
+      // general time transform of the u rows (a uniform branch): lane i
+      // applies the model's row i [Suu | Suq | Sul] to the cell's w and
+      // rescales its own rows of KM and SUQ, entry by entry, with the mass
+      // M_ij = det sum_q W_q phi_i(q) phi_j(q).  Suq_k = sum_a J[a][k] SUQ_a,
+      // so sum_k Suq_k w_q,k = sum_a SUQ_a (sum_k J[a][k] w_q,k).
+      constexpr int NLOC = NN * (D + 1) + NF * NNF;
+      if (lane < NN) {
+        const int i = lane;
+        const double* w = a.ts_w + (size_t)e * NLOC;
+        const double* uo = a.u_old + (size_t)e * NN;
+        double c = 0.0, mu = 0.0;
+#pragma unroll 1
+        for (int j = 0; j < NN; ++j) {
+          double m = 0.0;
+          for (int q = 0; q < NIP; ++q) m += WQ[q] * PHI[q * NN + i] * PHI[q * NN + j];
+          m *= det;
+          mu += m * uo[j];
+          c += KM[i * NN + j] * w[j];
+          KM[i * NN + j] = a.ts_auu * KM[i * NN + j] + a.ts_mu * m;
+          for (int a2 = 0; a2 < D; ++a2) {
+            double wt = 0.0;
+            for (int k = 0; k < D; ++k) wt += J[a2 * D + k] * w[NN + j * D + k];
+            c += SUQ[(a2 * NN + i) * NN + j] * wt;
+            SUQ[(a2 * NN + i) * NN + j] *= a.ts_arest;
+          }
+        }
+        const unsigned pw = (unsigned)s_pos[i];
+        for (int f = 0; f < NF; ++f) {
+          const int pf = (int)(pw >> (8 * f) & 0xFFu);
+          if (pf == 0xFF) continue;
+          for (int b = 0; b < NNF; ++b)
+            c += (-FM[((f * NSF + 0) * NNF + pf) * NNF + b] + FM[((f * NSF + 1) * NNF + pf) * NNF + b]) *
+                 w[NN * (D + 1) + f * NNF + b];
+        }
+        FU[i] = a.ts_phi * FU[i] + a.ts_mu * mu - c;
+      }
+      wave_sync();
+    }
     // K = Suu - sum_k Suq_k A_k = Suu - sum_a SUQ_a Ahat_a on FP64 MFMA, in
     // place in KM: [SUQ_0 | .. | SUQ_{D-1}] (NN x D NN) times [Ahat_0; ..] (D NN x NN)
     {
@@ -554,7 +593,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll
         for (int a2 = 0; a2 < NNF; ++a2)
           RB[s_fn[g * NNF + a2] * NCOL + c] =
-              -FM[((g * NSF + 0) * NNF + a2) * NNF + bp] + FM[((g * NSF + 1) * NNF + a2) * NNF + bp];
+              a.ts_arest * (-FM[((g * NSF + 0) * NNF + a2) * NNF + bp] + FM[((g * NSF + 1) * NNF + a2) * NNF + bp]);
       }
     }
     wave_sync();

@@ -72,6 +72,18 @@ struct ElemArgs {
   // blocks.  The local-matrix export scales the u rows back by dt.
   double ts_dt = 0.0, ts_idt = 0.0;
   const double* u_old = nullptr;
+  // optional: general time transform of the u rows (explicit Euler,
+  // Runge-Kutta stages; ts_gen = 1, ts_idt = 0): with the model's rows
+  // [Suu | Suq | Sul], F_u and the mass M,
+  //   Suu <- ts_auu Suu + ts_mu M,   [Suq | Sul] <- ts_arest [Suq | Sul],
+  //   F_u <- ts_phi F_u + ts_mu M u_old - [Suu | Suq | Sul] w,
+  // w = the cell's local vector [u | q | lambda] in ts_w (nEl x NLOC, lambda
+  // in cell-local face order).  Each scheme's rows divided by a constant
+  // (condensation does not see it); ts_dt is the factor that restores them
+  // in the local-matrix export.
+  int ts_gen = 0;
+  double ts_auu = 1.0, ts_arest = 1.0, ts_mu = 0.0, ts_phi = 1.0;
+  const double* ts_w = nullptr;
   // optional: explicit HDG modes (WEXPLICIT / SEXPLICIT, HDGSolver.cpp:346-353):
   // S = Sll and S0 = F_lambda - Slu sol_old - Slq flux_old, with sol_old
   // (nEl x nN, "Solution") and flux_old (nEl x nN x D, "Flux"); U, Q, U0, Q0

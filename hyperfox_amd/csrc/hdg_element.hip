@@ -211,7 +211,7 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
     }
     for (int t = tid; t < NIP; t += NT) {
       double fq = (a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + t] : 0.0;
-      if (a.u_old) {  // implicit Euler: + u_old(x_q) / dt
+      if (a.u_old && a.ts_idt != 0.0) {  // implicit Euler: + u_old(x_q) / dt
         double uo = 0.0;
         for (int j = 0; j < NN; ++j) uo += PHI[t * NN + j] * a.u_old[(size_t)e * NN + j];
         fq += a.ts_idt * uo;
@@ -363,6 +363,32 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
       FU[i] = s;
     }
     __syncthreads();
+    if (a.ts_gen) {
+      // general time transform of the u rows: F_u first (it applies the
+      // model's rows to w), then the rows; M is the mass block of AUG1
+      const double* w = a.ts_w + (size_t)e * Z::NLOC;
+      const double* uo = a.u_old + (size_t)e * NN;
+      for (int i = tid; i < NN; i += NT) {
+        double c = 0.0, mu = 0.0;
+        for (int j = 0; j < NN; ++j) {
+          c += SUU[i * NN + j] * w[j];
+          mu += AUG1[i * NC1 + j] * uo[j];
+          for (int k = 0; k < D; ++k) c += SUQ[(k * NN + i) * NN + j] * w[NN + j * D + k];
+        }
+        for (int f = 0; f < NF; ++f) {
+          const int ai = pos[f * NN + i];
+          if (ai < 0) continue;
+          for (int b = 0; b < NNF; ++b)
+            c += (-FM[((f * NS + 0) * NNF + ai) * NNF + b] + FM[((f * NS + 1) * NNF + ai) * NNF + b]) *
+                 w[NN * (D + 1) + f * NNF + b];
+        }
+        FU[i] = a.ts_phi * FU[i] + a.ts_mu * mu - c;
+      }
+      __syncthreads();
+      for (int t = tid; t < NN * NN; t += NT) SUU[t] = a.ts_auu * SUU[t] + a.ts_mu * AUG1[(t / NN) * NC1 + t % NN];
+      for (int t = tid; t < D * NN * NN; t += NT) SUQ[t] *= a.ts_arest;
+      __syncthreads();
+    }
 
     if (a.Kloc) {
       // assembly-only path: write the local matrix in the reference layout
@@ -378,7 +404,8 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
           else if (c < OL) v = SUQ[(((c - OQ) % D) * NN + i) * NN + (c - OQ) / D];
           else {
             const int f = (c - OL) / NNF, b = (c - OL) % NNF, ai = pos[f * NN + i];
-            if (ai >= 0) v = -FM[((f * NS + 0) * NNF + ai) * NNF + b] + FM[((f * NS + 1) * NNF + ai) * NNF + b];
+            if (ai >= 0)
+              v = a.ts_arest * (-FM[((f * NS + 0) * NNF + ai) * NNF + b] + FM[((f * NS + 1) * NNF + ai) * NNF + b]);
           }
         } else if (r < OL) {
           const int i = (r - OQ) / D, k = (r - OQ) % D;
@@ -401,7 +428,7 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
             if (g == f) v = -FM[((f * NS + 0) * NNF + aa) * NNF + b] + FM[((f * NS + 1) * NNF + aa) * NNF + b];
           }
         }
-        // implicit Euler: the reference's u rows are dt x (these rows)
+        // time schemes: the reference's u rows are ts_dt x (these rows)
         K[t] = (a.ts_dt > 0.0 && r < OQ) ? a.ts_dt * v : v;
       }
       double* F = a.Floc + (size_t)(e - a.el_offset) * NLOC;
@@ -432,7 +459,7 @@ __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
       const int i = t / NL, c = t % NL, f = c / NNF, b = c % NNF;
       double r = 0.0;
       const int ai = pos[f * NN + i];
-      if (ai >= 0) r = -FM[((f * NS + 0) * NNF + ai) * NNF + b] + FM[((f * NS + 1) * NNF + ai) * NNF + b];
+      if (ai >= 0) r = a.ts_arest * (-FM[((f * NS + 0) * NNF + ai) * NNF + b] + FM[((f * NS + 1) * NNF + ai) * NNF + b]);
       for (int k = 0; k < D; ++k)
         for (int aa = 0; aa < NNF; ++aa)
           r += ZB[(k * NN + i) * NN + fn[f * NNF + aa]] * FM[((f * NS + 2 + k) * NNF + aa) * NNF + b];

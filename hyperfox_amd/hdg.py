@@ -8,7 +8,13 @@ import ctypes
 
 import numpy as np
 
-from ._lib import check, lib
+from ._lib import HfxError, check, lib
+
+# Butcher tables of the reference's RungeKutta database, in RKType order
+# (src/operator/RKType.h); RK_CUSTOM selects a caller's table
+RK_NAMES = ['FEuler', 'EMidpoint', 'Heun', 'Kutta3', 'Heun3', 'SSPRK3', 'RK4', 'BEuler', 'IMidpoint',
+            'CrankNicolson', 'KS2', 'QZ2', 'ALX2', 'RK43']
+RK_CUSTOM = -1
 
 
 def _ptr(a):
@@ -189,9 +195,55 @@ class HDGProblem:
         return rowptr, cols, vals, rhs
 
     def set_time_scheme(self, scheme, dt=0.0):
-        """0 none, 1 implicit Euler (the reference's Euler, isExplicit=false)
-        on the u rows with step dt; u_old is the current solution field."""
+        """0 none, 1 implicit Euler, 2 explicit Euler (the reference's Euler,
+        isExplicit false / true; u_old is the current solution field), 3
+        Runge-Kutta (set_butcher, rk_*), on the u rows with step dt."""
         check(lib.hfx_hdg_set_time_scheme(self._h, scheme, dt))
+
+    def set_butcher(self, table):
+        """Butcher table: a name of RK_NAMES (the reference's database) or a
+        square array [[c | A], [0 | b]] with zeros above the diagonal."""
+        if isinstance(table, str):
+            check(lib.hfx_hdg_set_butcher(self._h, RK_NAMES.index(table), None, 0))
+        else:
+            t = _f64(table)
+            if t.ndim != 2 or t.shape[0] != t.shape[1]:
+                raise HfxError('RungeKutta: the Butcher table should be square')
+            check(lib.hfx_hdg_set_butcher(self._h, RK_CUSTOM, _ptr(t), t.shape[0]))
+
+    def rk_start_step(self):
+        """Old fields = current u, q, lambda; stage fields zero."""
+        check(lib.hfx_hdg_rk_start_step(self._h))
+
+    def rk_compute_stage(self):
+        check(lib.hfx_hdg_rk_compute_stage(self._h))
+
+    def rk_compute_solution(self):
+        check(lib.hfx_hdg_rk_compute_solution(self._h))
+
+    def rk_stages(self):
+        """(number of stages, stage counter)."""
+        out = np.zeros(2, dtype=np.int32)
+        check(lib.hfx_hdg_rk_stages(self._h, _ptr(out)))
+        return int(out[0]), int(out[1])
+
+    def rk_step(self, pc=2, rtol=1e-10, maxits=10000, restart=30):
+        """One Runge-Kutta time step as the reference's drivers take it: per
+        stage assemble, solve and computeStage, then computeSolution.
+        Returns the solve info of every stage."""
+        self.rk_start_step()
+        infos = []
+        for _ in range(self.rk_stages()[0]):
+            self.assemble()
+            infos.append(self.solve(pc, rtol, maxits, restart))
+            self.rk_compute_stage()
+        self.rk_compute_solution()
+        return infos
+
+    def set_trace(self, lam):
+        """Set the trace field (nFaces x nNf)."""
+        lam = _f64(lam)
+        check(lib.hfx_hdg_set_trace(self._h, _ptr(lam)))
 
     def set_solution(self, u):
         """Set the cell-wise nodal solution field (nCells x nN)."""
@@ -253,6 +305,14 @@ class HDGProblem:
         ms = np.zeros(reps)
         check(lib.hfx_hdg_time(self._h, what, warmup, reps, _ptr(ms)))
         return ms
+
+
+def rk_table(name):
+    """The library's Butcher table of the database entry `name` (n x n)."""
+    t = np.zeros(25)
+    n = np.zeros(1, dtype=np.int32)
+    check(lib.hfx_rk_table(RK_NAMES.index(name), _ptr(t), _ptr(n)))
+    return t[:n[0] * n[0]].reshape(n[0], n[0])
 
 
 def ilu_analyse(rowptr, col, ordering):

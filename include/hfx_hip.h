@@ -166,10 +166,46 @@ int hfx_hdg_get_trace_csr(hfx_hdg* h, long long* rowptr, int* cols, double* vals
  * (Euler::apply, Euler.cpp:18-37): u rows scaled by dt, plus the element
  * mass M in Suu and M u_old in F_u, where u_old is the problem's "Solution"
  * field at assembly (the last solve's u, or hfx_hdg_set_solution).
- * HFX_TS_FEULER (isExplicit = true) is rejected.  dt must be > 0, as
- * Euler::apply requires. */
-enum { HFX_TS_NONE = 0, HFX_TS_BEULER = 1, HFX_TS_FEULER = 2 };
+ * HFX_TS_FEULER is Euler with isExplicit = true (Euler.cpp:30-35): u rows
+ * scaled by dt, F_u += (M - Suu) u_old, Suu = M.  HFX_TS_RK is RungeKutta
+ * (RungeKutta.cpp:113-159) with the auxiliary fields Flux and Trace, driven
+ * by hfx_hdg_set_butcher and hfx_hdg_rk_*.  dt must be > 0, as the
+ * reference's apply requires. */
+enum { HFX_TS_NONE = 0, HFX_TS_BEULER = 1, HFX_TS_FEULER = 2, HFX_TS_RK = 3 };
 int hfx_hdg_set_time_scheme(hfx_hdg* h, int scheme, double dt);
+
+/* Butcher tables of RungeKutta::setUpDB (RungeKutta.cpp:197-285), in the
+ * order of RKType (src/operator/RKType.h); HFX_RK_CUSTOM takes a table. */
+enum {
+  HFX_RK_FEULER = 0, HFX_RK_EMIDPOINT = 1, HFX_RK_HEUN = 2, HFX_RK_KUTTA3 = 3, HFX_RK_HEUN3 = 4,
+  HFX_RK_SSPRK3 = 5, HFX_RK_RK4 = 6, HFX_RK_BEULER = 7, HFX_RK_IMIDPOINT = 8, HFX_RK_CRANKNICOLSON = 9,
+  HFX_RK_KS2 = 10, HFX_RK_QZ2 = 11, HFX_RK_ALX2 = 12, HFX_RK_RK43 = 13, HFX_RK_CUSTOM = -1
+};
+/* RungeKutta::setButcherTable: a table of the database (type >= 0), or the
+ * n x n row-major `table` (HFX_RK_CUSTOM), which must have zeros above its
+ * diagonal, as setButcherTable(EMatrix) requires (RungeKutta.cpp:17-31).
+ * Row i < n-1 is [c_i | a_i0 .. ], the last row [0 | b_0 .. ]; at most 8
+ * stages.  Resets the stage counter. */
+int hfx_hdg_set_butcher(hfx_hdg* h, int type, const double* table, int n);
+/* The database table `type`: n[0] = its size, table (n x n row-major, room
+ * for 25 values) may be NULL. */
+int hfx_rk_table(int type, double* table, int* n);
+/* Start a Runge-Kutta time step: OldSolution / OldFlux / OldTrace = the
+ * current Solution u, Flux q and Trace lambda, every RKStage field zero,
+ * stage counter 0 (the caller's step set-up in the reference's drivers,
+ * e.g. tests/parallel/TestParHDGConvectionDiffusionReactionSource.cpp:281-292).
+ * Each stage is then hfx_hdg_assemble, hfx_hdg_solve and
+ * hfx_hdg_rk_compute_stage; hfx_hdg_rk_compute_solution ends the step. */
+int hfx_hdg_rk_start_step(hfx_hdg* h);
+/* RungeKutta::computeStage (RungeKutta.cpp:161-203) on u, q and lambda. */
+int hfx_hdg_rk_compute_stage(hfx_hdg* h);
+/* RungeKutta::computeSolution (RungeKutta.cpp:205-241) on u, q and lambda. */
+int hfx_hdg_rk_compute_solution(hfx_hdg* h);
+/* out[0] = stages of the table (RungeKutta::getNumStages), out[1] = stage counter. */
+int hfx_hdg_rk_stages(hfx_hdg* h, int* out);
+/* Set the "Trace" field lambda (nFaces x nNf, face-node order), e.g. an
+ * initial condition; hfx_hdg_solve overwrites it. */
+int hfx_hdg_set_trace(hfx_hdg* h, const double* lambda);
 /* Set the "Solution" field u (nCells x nN, cell-wise nodal values), e.g. an
  * initial condition; hfx_hdg_solve overwrites it. */
 int hfx_hdg_set_solution(hfx_hdg* h, const double* u);
