@@ -176,6 +176,22 @@ def transient(hf, pr, prob, w, steps=3, dt=1e-2):
             k + 1, rows[-1]['assemble_ms'], its, rows[-1]['solve_ms']), file=sys.stderr, flush=True)
     ms = float(np.median(prob.time(1, warmup=1, reps=5)))
     dofs = w.mesh.n_cells * prob.sizes['nN']
+    # diagonally implicit Runge-Kutta (ALX2, two stages; the reference's
+    # RungeKutta with the auxiliary fields Flux and Trace) from the last
+    # solution: per stage assemble, solve, computeStage; then computeSolution
+    prob.set_time_scheme(hf.TS_RK, dt)
+    prob.set_butcher('ALX2')
+    rk = []
+    for k in range(2):
+        t0 = time.perf_counter()
+        infos = prob.rk_step(pc=hf.PC_ILU0, rtol=1e-10, maxits=20000)
+        t1 = time.perf_counter()
+        rk.append({'step_ms': (t1 - t0) * 1e3, 'stage_iters': [i for i, _, _ in infos],
+                   'stage_reasons': [r for _, r, _ in infos], 'max_u': float(np.abs(prob.solution()[0]).max())})
+        print('# ALX2 step %d: %.1f ms, stage iterations %s' % (k + 1, rk[-1]['step_ms'], rk[-1]['stage_iters']),
+              file=sys.stderr, flush=True)
+    prob.rk_start_step()
+    ms_rk = float(np.median(prob.time(1, warmup=1, reps=5)))  # stage-0 assembly (local vectors included)
     prob.set_time_scheme(hf.TS_NONE)
     # the reference's strongly explicit HDG mode: S = Sll, S0 from the current
     # solution and flux fields, face-local trace solves
@@ -194,6 +210,7 @@ def transient(hf, pr, prob, w, steps=3, dt=1e-2):
         ex[-1]['assemble_ms'], ex[-1]['face_solves_and_reconstruction_ms']), file=sys.stderr, flush=True)
     return {'scheme': 'implicit Euler (Euler, isExplicit=false)', 'dt': dt, 'steps': rows,
             'assembly_ms_with_scheme': ms, 'assembly_dof_per_s_with_scheme': dofs / (ms * 1e-3),
+            'rk_alx2': {'steps': rk, 'stage_assembly_ms': ms_rk, 'stage_assembly_dof_per_s': dofs / (ms_rk * 1e-3)},
             'sexplicit_steps': ex}
 
 

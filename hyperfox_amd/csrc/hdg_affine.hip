@@ -152,7 +152,7 @@ __device__ __forceinline__ void affine_load_tables(const double* __restrict__ ta
 // problem, one wavefront per cell, with the tables already staged: assembles
 // and condenses each cell.  Shared by the per-problem kernel and the batched
 // kernels.
-template <int D, int P, bool TD, int W>
+template <int D, int P, bool TD, int W, bool TSG = false>
 __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int e_end, int e_stride, double* sm,
                                              int* s_pos, int* s_fn) {
   using L = ALay<D, P, TD>;
@@ -451,31 +451,39 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       }
     }
     wave_sync();
-    if (a.ts_gen) {
-      // general time transform of the u rows (a uniform branch): lane i
-      // applies the model's row i [Suu | Suq | Sul] to the cell's w and
-      // rescales its own rows of KM and SUQ, entry by entry, with the mass
-      // M_ij = det sum_q W_q phi_i(q) phi_j(q).  Suq_k = sum_a J[a][k] SUQ_a,
-      // so sum_k Suq_k w_q,k = sum_a SUQ_a (sum_k J[a][k] w_q,k).
+    if constexpr (TSG) {
+      // general time transform of the u rows (the TSG kernel variant).  The
+      // cell's w, u_old and u_old at the cubature points go to the phase-1
+      // overlay (dead after the entries); lane i applies the model's row i
+      // [Suu | Suq | Sul] to w and forms (M u_old)_i = det sum_q W_q phi_i(q)
+      // u_old(x_q); then, spread over the lanes, every KM entry takes
+      // ts_auu KM + ts_mu M with M_ij = det sum_q W_q phi_i(q) phi_j(q), and
+      // SUQ is scaled.  Suq_k = sum_a J[a][k] SUQ_a, so
+      // sum_k Suq_k w_q,k = sum_a SUQ_a (sum_k J[a][k] w_q,k).
       constexpr int NLOC = NN * (D + 1) + NF * NNF;
+      static_assert(NLOC + NN + NIP <= L::PER - L::O_OV, "time-transform staging fits the overlay");
+      double* TW = ws + L::O_OV;
+      double* TU = TW + NLOC;
+      double* UQ = TU + NN;
+      for (int t = lane; t < NLOC; t += 64) TW[t] = a.ts_w[(size_t)e * NLOC + t];
+      for (int t = lane; t < NN; t += 64) TU[t] = a.u_old[(size_t)e * NN + t];
+      wave_sync();
+      for (int q = lane; q < NIP; q += 64) {
+        double v = 0.0;
+        for (int j = 0; j < NN; ++j) v += PHI[q * NN + j] * TU[j];
+        UQ[q] = v;
+      }
+      wave_sync();
       if (lane < NN) {
         const int i = lane;
-        const double* w = a.ts_w + (size_t)e * NLOC;
-        const double* uo = a.u_old + (size_t)e * NN;
         double c = 0.0, mu = 0.0;
-#pragma unroll 1
+        for (int q = 0; q < NIP; ++q) mu += WQ[q] * PHI[q * NN + i] * UQ[q];
         for (int j = 0; j < NN; ++j) {
-          double m = 0.0;
-          for (int q = 0; q < NIP; ++q) m += WQ[q] * PHI[q * NN + i] * PHI[q * NN + j];
-          m *= det;
-          mu += m * uo[j];
-          c += KM[i * NN + j] * w[j];
-          KM[i * NN + j] = a.ts_auu * KM[i * NN + j] + a.ts_mu * m;
+          c += KM[i * NN + j] * TW[j];
           for (int a2 = 0; a2 < D; ++a2) {
             double wt = 0.0;
-            for (int k = 0; k < D; ++k) wt += J[a2 * D + k] * w[NN + j * D + k];
+            for (int k = 0; k < D; ++k) wt += J[a2 * D + k] * TW[NN + j * D + k];
             c += SUQ[(a2 * NN + i) * NN + j] * wt;
-            SUQ[(a2 * NN + i) * NN + j] *= a.ts_arest;
           }
         }
         const unsigned pw = (unsigned)s_pos[i];
@@ -484,10 +492,18 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           if (pf == 0xFF) continue;
           for (int b = 0; b < NNF; ++b)
             c += (-FM[((f * NSF + 0) * NNF + pf) * NNF + b] + FM[((f * NSF + 1) * NNF + pf) * NNF + b]) *
-                 w[NN * (D + 1) + f * NNF + b];
+                 TW[NN * (D + 1) + f * NNF + b];
         }
-        FU[i] = a.ts_phi * FU[i] + a.ts_mu * mu - c;
+        FU[i] = a.ts_phi * FU[i] + a.ts_mu * (det * mu) - c;
       }
+      wave_sync();  // every row has read KM and SUQ before they change
+      for (int t = lane; t < NN * NN; t += 64) {
+        const int i = t / NN, j = t % NN;
+        double m = 0.0;
+        for (int q = 0; q < NIP; ++q) m += WQ[q] * PHI[q * NN + i] * PHI[q * NN + j];
+        KM[t] = a.ts_auu * KM[t] + a.ts_mu * (det * m);
+      }
+      for (int t = lane; t < D * NN * NN; t += 64) SUQ[t] *= a.ts_arest;
       wave_sync();
     }
     // K = Suu - sum_k Suq_k A_k = Suu - sum_a SUQ_a Ahat_a on FP64 MFMA, in
@@ -588,12 +604,13 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     // R[:, c] = Sul[:, c] - sum_k Suq_k B_k[:, c] (or -F_u on lane NL), with
     // B_k[:, (g,b)] = -rho_g n_gk G_g[:, b] and sum_k n_gk Suq_k = sum_a nt_a SUQ_a
     if (c <= NL) {
+      const double sul_s = TSG ? a.ts_arest : 1.0;  // the time transform scales Sul
       for (int i = 0; i < NN; ++i) RB[i * NCOL + c] = tr ? 0.0 : -FU[i];
       if (tr) {
 #pragma unroll
         for (int a2 = 0; a2 < NNF; ++a2)
           RB[s_fn[g * NNF + a2] * NCOL + c] =
-              a.ts_arest * (-FM[((g * NSF + 0) * NNF + a2) * NNF + bp] + FM[((g * NSF + 1) * NNF + a2) * NNF + bp]);
+              sul_s * (-FM[((g * NSF + 0) * NNF + a2) * NNF + bp] + FM[((g * NSF + 1) * NNF + a2) * NNF + bp]);
       }
     }
     wave_sync();
@@ -871,14 +888,15 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
   }
 }
 
-template <int D, int P, bool TD, int W>
+template <int D, int P, bool TD, int W, bool TSG>
 __global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
   using L = ALay<D, P, TD>;
   extern __shared__ double sm[];
   __shared__ int s_pos[L::NN];
   __shared__ int s_fn[L::NF * L::NNF];
   affine_load_tables<D, P, TD, W>(a.tabs, sm, s_pos, s_fn);
-  affine_cells<D, P, TD, W>(a, a.el_offset + blockIdx.x * W, a.el_offset + a.nEl, gridDim.x * W, sm, s_pos, s_fn);
+  affine_cells<D, P, TD, W, TSG>(a, a.el_offset + blockIdx.x * W, a.el_offset + a.nEl, gridDim.x * W, sm, s_pos,
+                                 s_fn);
 }
 
 // Persistent kernel over the cells of many 2-D problems of one order P:
@@ -903,11 +921,11 @@ __global__ __launch_bounds__(64 * W) void hdg_affine_batch_kernel(const ElemArgs
   }
 }
 
-template <int D, int P, bool TD, int W>
+template <int D, int P, bool TD, int W, bool TSG = false>
 void launch_aff(const ElemArgs& a, int grid_cap, hipStream_t s) {
   using L = ALay<D, P, TD>;
   const size_t lds = (size_t)(L::TAB + W * L::PER) * sizeof(double);
-  auto kern = hdg_affine_kernel<D, P, TD, W>;
+  auto kern = hdg_affine_kernel<D, P, TD, W, TSG>;
   static bool attr_set = false;
   if (!attr_set) {
     HFX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -950,8 +968,32 @@ size_t affine_lds_bytes(int dim, int order, bool tensor_d, int waves) {
   throw std::runtime_error("affine element kernel: (dim, order) not supported");
 }
 
+// waves per workgroup that fit the LDS (with 4 KiB to spare) for (D, P);
+// the host picks the same number
+template <int D, int P>
+constexpr int aff_wave_fit() {
+  using L = ALay<D, P, false>;
+  for (int w = 4; w > 1; --w)
+    if ((L::TAB + w * L::PER) * 8 + 4096 <= 160 * 1024) return w;
+  return 1;
+}
+
 void launch_affine(int dim, int order, bool tensor_d, int waves, const ElemArgs& a, int grid_cap, hipStream_t s) {
   if (tensor_d) throw std::runtime_error("affine element kernel: tensor diffusion uses the general kernel");
+  if (a.ts_gen) {
+    // the time-transform variant exists at the LDS-fitting wave count only
+#define HFX_CASE(D, P)                                                                     \
+    if (dim == D && order == P) {                                                          \
+      if (waves != aff_wave_fit<D, P>())                                                   \
+        throw std::runtime_error("affine element kernel: time transform needs the default waves per workgroup"); \
+      launch_aff<D, P, false, aff_wave_fit<D, P>(), true>(a, grid_cap, s);                 \
+      return;                                                                              \
+    }
+    HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
+    HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3)
+#undef HFX_CASE
+    throw std::runtime_error("affine element kernel: (dim, order) not supported");
+  }
 #define HFX_CASE(D, P)                                                   \
   if (dim == D && order == P) {                                          \
     switch (waves) {                                                     \
