@@ -17,4 +17,5 @@ PC_NONE, PC_JACOBI, PC_BJACOBI, PC_LEARNED, PC_ILU0, PC_ILU0_NATURAL = 0, 1, 2, 
 ILU_MULTICOLOUR, ILU_NATURAL = 0, 1
 KRYLOV_FGMRES, KRYLOV_BICGSTAB = 0, 1
 TS_NONE, TS_BEULER, TS_FEULER, TS_RK = 0, 1, 2, 3
+BC_DIRICHLET, BC_INTEGRATED_DIRICHLET = 0, 1
 HDG_IMPLICIT, HDG_WEXPLICIT, HDG_SEXPLICIT = 0, 1, 2

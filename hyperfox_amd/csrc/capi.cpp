@@ -284,6 +284,8 @@ extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
   h->ilu.reset();
   h->rk_started = false;  // Runge-Kutta fields are sized for the mesh
   h->rk_stage = 0;
+  h->bc_kind.clear();     // boundary models are per face of the mesh
+  h->n_bc_int = 0;
   h->mesh = mm->m;
   h->mesh.order = h->order;
   h->affine = mesh_is_affine(*h->re, h->mesh);
@@ -721,6 +723,36 @@ extern "C" int hfx_hdg_rk_stages(hfx_hdg* h, int* out) {
   require(h && out, "hfx_hdg_rk_stages: null argument");
   out[0] = h->rk_n ? h->rk_stages() : 0;
   out[1] = h->rk_stage;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_boundary_model(hfx_hdg* h, int kind, const int* faces, int n) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_set_boundary_model: set the mesh first");
+  require(kind == HFX_BC_DIRICHLET || kind == HFX_BC_INTEGRATED_DIRICHLET, "hfx_hdg_set_boundary_model: unknown kind");
+  const hfx::Mesh& m = h->mesh;
+  std::vector<char> bnd(m.nFaces, 0);
+  for (int f : m.boundary) bnd[f] = 1;
+  if (h->bc_kind.size() != (size_t)m.nFaces) h->bc_kind.assign(m.nFaces, HFX_BC_DIRICHLET);
+  if (!faces) {
+    for (int F = 0; F < m.nFaces; ++F)
+      if (bnd[F]) h->bc_kind[F] = (char)kind;
+  } else {
+    require(n >= 0, "hfx_hdg_set_boundary_model: negative face count");
+    for (int i = 0; i < n; ++i)
+      require(faces[i] >= 0 && faces[i] < m.nFaces && bnd[faces[i]],
+              "hfx_hdg_set_boundary_model: not a boundary face");
+    for (int i = 0; i < n; ++i) h->bc_kind[faces[i]] = (char)kind;
+  }
+  std::vector<int> lst;
+  for (int F = 0; F < m.nFaces; ++F)
+    if (h->bc_kind[F] == HFX_BC_INTEGRATED_DIRICHLET) lst.push_back(F);
+  h->set_device();
+  h->bc_int_faces.upload(lst, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  h->n_bc_int = (int)lst.size();
+  h->assembled = h->solved = false;
+  ++h->args_version;
   HFX_END
 }
 

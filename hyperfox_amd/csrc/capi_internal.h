@@ -77,6 +77,11 @@ struct hfx_hdg {
   int rk_n = 0, rk_stage = 0;
   bool rk_started = false;
   hfx::DevBuf<double> rk_old_u, rk_old_q, rk_old_l, rk_ku, rk_kq, rk_kl, ts_w;
+  // boundary model per face (hfx_hdg_set_boundary_model): 0 DirichletModel,
+  // 1 IntegratedDirichletModel; the device list of the latter
+  std::vector<char> bc_kind;
+  hfx::DevBuf<int> bc_int_faces;
+  int n_bc_int = 0;
   // HDGSolverOpts::type (hfx_hdg_set_solver_type); the explicit modes read the
   // problem's "Solution" u and "Flux" q fields at assembly
   int hdg_type = HFX_HDG_IMPLICIT;
@@ -236,12 +241,21 @@ struct hfx_hdg {
       last_fused = false;
     }
   }
-  // the trace-system pass that completes the last element launch
+  // the trace-system pass that completes the last element launch, then the
+  // IntegratedDirichletModel rows
   void scatter_on(hipStream_t s, bool fused) {
     if (fused)
       hfx::launch_scatter_diag(A, mesh.nFpc, dside.p, S0.p, invperm.p, is_dir.p, g.p, rhs.p, s);
     else
       hfx::launch_scatter(A, mesh.nFpc, S.p, S0.p, invperm.p, is_dir.p, g.p, rhs.p, s);
+    if (n_bc_int > 0) {
+      const int nN = re->nN, nNf = re->nNf(), nip = re->nip(), nipf = re->face->nip();
+      const size_t t_fphi = (size_t)nip * nN * (1 + dim), t_fdphi = t_fphi + (size_t)nipf * nNf;
+      const size_t t_fw = t_fdphi + (size_t)nipf * nNf * (dim - 1) + nip;
+      hfx::launch_integrated_dirichlet(A, n_bc_int, bc_int_faces.p, dim, nN, nipf, mesh.nFpc, faceNodes.p, perm.p,
+                                       cells.p, coords.p, tabs.p + t_fphi, tabs.p + t_fdphi, tabs.p + t_fw, g.p,
+                                       rhs.p, s);
+    }
   }
   void run_scatter() { scatter_on(stream, last_fused); }
 };

@@ -50,6 +50,15 @@ void launch_scatter(const BsrDev& A, int nFpc, const double* S, const double* S0
 void launch_scatter_diag(const BsrDev& A, int nFpc, const double* dside, const double* S0, const int8_t* invperm,
                          const uint8_t* is_dir, const double* g, double* rhs, hipStream_t s);
 
+// IntegratedDirichletModel on the boundary faces bfaces[nbf], after either
+// scatter: their diagonal blocks become the face mass of the face's nodes
+// (first cell's nodes through fnodes / face_perm; face tables fphi
+// [nipf][bs], fdphi [nipf][bs][dim-1], fw [nipf]) and their RHS rows M g.
+void launch_integrated_dirichlet(const BsrDev& A, int nbf, const int* bfaces, int dim, int nN, int nipf, int nFpc,
+                                 const int* fnodes, const int8_t* face_perm, const int* cells, const double* coords,
+                                 const double* fphi, const double* fdphi, const double* fw, const double* g,
+                                 double* rhs, hipStream_t s);
+
 // BSR block index of every (face f, face g) pair of every cell:
 // out[(e*nFpc + f)*nFpc + g] (the diagonal block when f == g).
 std::vector<int> cell_block_positions(const BsrPattern& P, int nCells, int nFpc, const int* cell2face);

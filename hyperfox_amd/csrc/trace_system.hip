@@ -136,6 +136,65 @@ __global__ void rhs_kernel(int nb, int bs, int nFpc, const int* __restrict__ fac
   rhs[t] = v;
 }
 
+// IntegratedDirichletModel rows (IntegratedDirichletModel.cpp:35-56): the
+// diagonal block of boundary face F = bfaces[ib] becomes the face mass
+// M_ab = sum_q w_q sqrt(det(J_f J_f^T)) psi_a psi_b of the face's nodes,
+// taken from its cell (face position pp[a] of cell-local face node a).  One
+// thread per block entry.
+__global__ void integrated_dirichlet_kernel(int nbf, const int* __restrict__ bfaces, int dim, int nN, int nNf,
+                                            int nipf, int nFpc, const int* __restrict__ face_cells,
+                                            const int* __restrict__ fnodes, const int8_t* __restrict__ face_perm,
+                                            const int* __restrict__ cells, const double* __restrict__ coords,
+                                            const double* __restrict__ fphi, const double* __restrict__ fdphi,
+                                            const double* __restrict__ fw, const int* __restrict__ diag,
+                                            double* __restrict__ vals) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nbf * nNf * nNf) return;
+  const int ib = (int)(t / (nNf * nNf)), a = (int)((t / nNf) % nNf), b = (int)(t % nNf);
+  const int F = bfaces[ib];
+  const int e = face_cells[4 * (size_t)F], lf = face_cells[4 * (size_t)F + 1];
+  const int* cn = cells + (size_t)e * nN;
+  const int fd = dim - 1;
+  double m = 0.0;
+  for (int q = 0; q < nipf; ++q) {
+    double J[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // fd x dim
+    for (int j = 0; j < nNf; ++j) {
+      const double* x = coords + (size_t)cn[fnodes[lf * nNf + j]] * dim;
+      for (int r = 0; r < fd; ++r) {
+        const double dp = fdphi[(q * nNf + j) * fd + r];
+        for (int c = 0; c < dim; ++c) J[r * dim + c] += dp * x[c];
+      }
+    }
+    double meas;
+    if (dim == 2) {
+      meas = sqrt(J[0] * J[0] + J[1] * J[1]);
+    } else {
+      const double g00 = J[0] * J[0] + J[1] * J[1] + J[2] * J[2];
+      const double g01 = J[0] * J[3] + J[1] * J[4] + J[2] * J[5];
+      const double g11 = J[3] * J[3] + J[4] * J[4] + J[5] * J[5];
+      meas = sqrt(g00 * g11 - g01 * g01);
+    }
+    m += fphi[q * nNf + a] * fphi[q * nNf + b] * (fw[q] * meas);
+  }
+  const int8_t* pp = face_perm + ((size_t)e * nFpc + lf) * nNf;
+  vals[(size_t)diag[F] * nNf * nNf + pp[a] * nNf + pp[b]] = m;
+}
+
+// rhs of those rows: M g (g in face-node order; null: zero)
+__global__ void integrated_dirichlet_rhs_kernel(int nbf, const int* __restrict__ bfaces, int nNf,
+                                                const int* __restrict__ diag, const double* __restrict__ vals,
+                                                const double* __restrict__ g, double* __restrict__ rhs) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nbf * nNf) return;
+  const int F = bfaces[t / nNf], r = (int)(t % nNf);
+  double v = 0.0;
+  if (g) {
+    const double* row = vals + (size_t)diag[F] * nNf * nNf + r * nNf;
+    for (int c = 0; c < nNf; ++c) v += row[c] * g[(size_t)F * nNf + c];
+  }
+  rhs[(size_t)F * nNf + r] = v;
+}
+
 __global__ void reconstruct_kernel(int nEl, int nN, int nQ, int nFpc, int bs, const double* __restrict__ U,
                                    const double* __restrict__ U0, const double* __restrict__ Q,
                                    const double* __restrict__ Q0, const int* __restrict__ cell2face,
@@ -207,6 +266,22 @@ void launch_scatter_diag(const BsrDev& A, int nFpc, const double* dside, const d
                        invperm, is_dir, g, rhs);
     HFX_HIP(hipGetLastError());
   }
+}
+
+void launch_integrated_dirichlet(const BsrDev& A, int nbf, const int* bfaces, int dim, int nN, int nipf, int nFpc,
+                                 const int* fnodes, const int8_t* face_perm, const int* cells, const double* coords,
+                                 const double* fphi, const double* fdphi, const double* fw, const double* g,
+                                 double* rhs, hipStream_t s) {
+  if (nbf <= 0) return;
+  const int nt = 256, bs = A.bs;
+  const long long n = (long long)nbf * bs * bs;
+  hipLaunchKernelGGL(integrated_dirichlet_kernel, dim3(ceil_div(n, nt)), dim3(nt), 0, s, nbf, bfaces, dim, nN, bs,
+                     nipf, nFpc, A.face_cells.p, fnodes, face_perm, cells, coords, fphi, fdphi, fw, A.diag.p,
+                     A.vals.p);
+  HFX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(integrated_dirichlet_rhs_kernel, dim3(ceil_div((long long)nbf * bs, nt)), dim3(nt), 0, s, nbf,
+                     bfaces, bs, A.diag.p, A.vals.p, g, rhs);
+  HFX_HIP(hipGetLastError());
 }
 
 std::vector<int> cell_block_positions(const BsrPattern& P, int nCells, int nFpc, const int* cell2face) {

@@ -240,6 +240,15 @@ class HDGProblem:
         self.rk_compute_solution()
         return infos
 
+    def set_boundary_model(self, kind, faces=None):
+        """Boundary model of the listed boundary faces (None: all): 0
+        DirichletModel (default), 1 IntegratedDirichletModel."""
+        if faces is None:
+            check(lib.hfx_hdg_set_boundary_model(self._h, kind, None, 0))
+        else:
+            f = np.ascontiguousarray(faces, dtype=np.int32)
+            check(lib.hfx_hdg_set_boundary_model(self._h, kind, _ptr(f), f.size))
+
     def set_trace(self, lam):
         """Set the trace field (nFaces x nNf)."""
         lam = _f64(lam)

@@ -203,6 +203,15 @@ int hfx_hdg_rk_compute_stage(hfx_hdg* h);
 int hfx_hdg_rk_compute_solution(hfx_hdg* h);
 /* out[0] = stages of the table (RungeKutta::getNumStages), out[1] = stage counter. */
 int hfx_hdg_rk_stages(hfx_hdg* h, int* out);
+/* Boundary model of the boundary faces `faces` (n of them; NULL: every
+ * boundary face), applied by hfx_hdg_assemble (HDGSolver::
+ * applyBoundaryConditions, HDGSolver.cpp:361-529): HFX_BC_DIRICHLET is
+ * DirichletModel (the default: identity rows, rhs g; DirichletModel.cpp:19-44),
+ * HFX_BC_INTEGRATED_DIRICHLET is IntegratedDirichletModel (rows = the face
+ * mass M of the face's nodes, rhs M g; IntegratedDirichletModel.cpp:35-56).
+ * Both Set the face's rows and RHS.  A non-boundary face is an error. */
+enum { HFX_BC_DIRICHLET = 0, HFX_BC_INTEGRATED_DIRICHLET = 1 };
+int hfx_hdg_set_boundary_model(hfx_hdg* h, int kind, const int* faces, int n);
 /* Set the "Trace" field lambda (nFaces x nNf, face-node order), e.g. an
  * initial condition; hfx_hdg_solve overwrites it. */
 int hfx_hdg_set_trace(hfx_hdg* h, const double* lambda);

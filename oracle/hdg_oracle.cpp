@@ -803,6 +803,32 @@ void apply_rk(const Ref& R, double dt, int nst, const double* a, int stage, cons
   }
 }
 
+// Mass operator on the face element (Mass.cpp:5-38) with the face's own
+// nodes xf (nNf x d, face-node order), as IntegratedDirichletModel computes
+// it (IntegratedDirichletModel.cpp:35-44): Jacobians of the face map
+// (Operator.cpp:14-36), measure sqrt(det(J J^T)) times the face weights,
+// M_ab = sum_q psi_a psi_b dV_q.
+void face_mass(const Ref& R, const double* xf, double* M) {
+  const int d = R.d, fd = d - 1, nNf = R.nNf;
+  std::fill(M, M + (size_t)nNf * nNf, 0.0);
+  for (int q = 0; q < R.nipf; ++q) {
+    double J[6] = {0};
+    for (int i = 0; i < nNf; ++i)
+      for (int a = 0; a < fd; ++a)
+        for (int b = 0; b < d; ++b) J[a * d + b] += R.fdphi[(q * nNf + i) * fd + a] * xf[i * d + b];
+    double JJt[4];
+    for (int a = 0; a < fd; ++a)
+      for (int c = 0; c < fd; ++c) {
+        double s = 0;
+        for (int b = 0; b < d; ++b) s += J[a * d + b] * J[c * d + b];
+        JJt[a * fd + c] = s;
+      }
+    const double dv = R.fipw[q] * std::sqrt(det(fd, JJt));
+    for (int a = 0; a < nNf; ++a)
+      for (int b = 0; b < nNf; ++b) M[a * nNf + b] += R.fphi[q * nNf + a] * R.fphi[q * nNf + b] * dv;
+  }
+}
+
 void check_rk(double dt, int nst, int stage) {
   if (dt == 0.0)
     throw std::runtime_error("RungeKutta: the time step needs to be set before applying and it should not be 0");
@@ -816,6 +842,13 @@ extern "C" int orc_apply_euler_explicit(int dim, int p, double dt, const double*
   check_dim(dim, p);
   if (!(dt > 0.0)) throw std::runtime_error("Euler: the time step needs to be set and it should not be 0");
   apply_euler_explicit(ref(dim, p), dt, u_old, K, F);
+  ORC_END
+}
+
+extern "C" int orc_face_mass(int dim, int p, const double* xf, double* M) {
+  ORC_TRY
+  check_dim(dim, p);
+  face_mass(ref(dim, p), xf, M);
   ORC_END
 }
 
@@ -883,6 +916,7 @@ struct orc_system {
   std::vector<int> cells, faces, c2f, f2c;
   std::vector<int> perm;              // nEl*nF*nNf : face2CellMap
   std::vector<char> is_bnd;           // nFaces
+  std::vector<char> bc_kind;          // per face: 0 DirichletModel, 1 IntegratedDirichletModel
   std::vector<double> S, S0, U, Q, U0, Q0;
   double ts_dt = 0.0;          // time step (0: no time scheme)
   int ts_kind = 0;             // 1 implicit Euler, 2 explicit Euler, 3 Runge-Kutta stage
@@ -939,14 +973,35 @@ void system_assemble(orc_system* s, int nEl, int nthreads) {
       if (s->hdg_type != 0)
         explicit_blocks(R, K.data(), F.data(), &s->sol[(size_t)e * lu], &s->flux[(size_t)e * lq], Se, S0e);
       // Dirichlet (DirichletModel: Set/Set, identity matrix, rhs = g in the
-      // face's node order, which is this cell's order on a boundary face)
+      // face's node order, which is this cell's order on a boundary face;
+      // IntegratedDirichletModel: Set/Set, the face mass M and rhs M g,
+      // HDGSolver.cpp:489-502)
       for (int f = 0; f < nF; ++f) {
         int fc = s->c2f[(size_t)e * nF + f];
         if (!s->is_bnd[fc]) continue;
+        const bool integ = !s->bc_kind.empty() && s->bc_kind[fc] == 1;
+        std::vector<double> xf, M;
+        if (integ) {
+          xf.resize((size_t)nNf * d);
+          M.resize((size_t)nNf * nNf);
+          for (int j = 0; j < nNf; ++j)
+            for (int k = 0; k < d; ++k) xf[j * d + k] = s->coords[(size_t)s->faces[(size_t)fc * nNf + j] * d + k];
+          face_mass(R, xf.data(), M.data());
+        }
         for (int a = 0; a < nNf; ++a) {
           for (int j = 0; j < ll; ++j) Se[(size_t)(f * nNf + a) * ll + j] = 0.0;
-          Se[(size_t)(f * nNf + a) * ll + f * nNf + a] += 1.0;
-          S0e[f * nNf + a] = s->dir.empty() ? 0.0 : s->dir[(size_t)fc * nNf + a];
+          S0e[f * nNf + a] = 0.0;
+          if (!integ) {
+            Se[(size_t)(f * nNf + a) * ll + f * nNf + a] += 1.0;
+            S0e[f * nNf + a] += s->dir.empty() ? 0.0 : s->dir[(size_t)fc * nNf + a];
+          } else {
+            double r = 0.0;
+            for (int b = 0; b < nNf; ++b) {
+              Se[(size_t)(f * nNf + a) * ll + f * nNf + b] += M[a * nNf + b];
+              r += M[a * nNf + b] * (s->dir.empty() ? 0.0 : s->dir[(size_t)fc * nNf + b]);
+            }
+            S0e[f * nNf + a] += r;
+          }
         }
       }
     }
@@ -1180,6 +1235,24 @@ extern "C" int orc_system_set_time_scheme(orc_system* s, int kind, double dt, in
       for (int e = 0; e < s->nEl; ++e)
         std::copy(stage_local + ((size_t)j * s->nEl + e) * n, stage_local + ((size_t)j * s->nEl + e + 1) * n,
                   &s->rk_k[((size_t)e * nst + j) * n]);
+  }
+  system_assemble(s, s->nEl, nthreads > 0 ? nthreads : omp_get_max_threads());
+  ORC_END
+}
+
+extern "C" int orc_system_set_boundary_model(orc_system* s, int kind, const int* faces, int nfaces, int nthreads) {
+  ORC_TRY
+  if (kind < 0 || kind > 1) throw std::runtime_error("boundary model: unknown kind");
+  if (s->bc_kind.empty()) s->bc_kind.assign(s->nFaces, 0);
+  if (!faces) {
+    for (int F = 0; F < s->nFaces; ++F)
+      if (s->is_bnd[F]) s->bc_kind[F] = (char)kind;
+  } else {
+    for (int i = 0; i < nfaces; ++i) {
+      if (faces[i] < 0 || faces[i] >= s->nFaces || !s->is_bnd[faces[i]])
+        throw std::runtime_error("boundary model: not a boundary face");
+      s->bc_kind[faces[i]] = (char)kind;
+    }
   }
   system_assemble(s, s->nEl, nthreads > 0 ? nthreads : omp_get_max_threads());
   ORC_END

@@ -115,6 +115,14 @@ int orc_system_set_hdg_type(orc_system* s, int type, const double* sol, const do
  * cell-local vectors as orc_apply_rk takes them. */
 int orc_system_set_time_scheme(orc_system* s, int kind, double dt, int nst, const double* a, int stage,
                                const double* u_old, const double* old_local, const double* stage_local, int nthreads);
+/* Boundary model of the boundary faces `faces` (nfaces; NULL: every
+ * boundary face), then reassemble: 0 DirichletModel (Set/Set, identity rows,
+ * rhs g), 1 IntegratedDirichletModel (Set/Set, the face mass M and rhs M g,
+ * IntegratedDirichletModel.cpp:35-56). */
+int orc_system_set_boundary_model(orc_system* s, int kind, const int* faces, int nfaces, int nthreads);
+/* Face mass of IntegratedDirichletModel: xf nNf*dim face nodes (face-node
+ * order), M nNf*nNf. */
+int orc_face_mass(int dim, int p, const double* xf, double* M);
 /* Cell-local vectors [u | q | lambda] (nEl x n) of the fields u (nEl*nN),
  * q (nEl*nN*dim) and lambda (nFaces*nNf), the trace in each cell's face-node
  * order (Solver::prepareLocalFieldMap, Solver.cpp:14-57). */

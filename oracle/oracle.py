@@ -42,6 +42,8 @@ def _load():
         'orc_apply_rk': ([I, I, D, I, P, I, P, P, P, P], I),
         'orc_system_set_time_scheme': ([P, I, D, I, P, I, P, P, P, I], I),
         'orc_system_local_vectors': ([P] * 5, I),
+        'orc_system_set_boundary_model': ([P, I, P, I, I], I),
+        'orc_face_mass': ([I, I, P, P], I),
         'orc_system_build': ([I, I, Model, I, P, I, P, I, P, P, P, P, P, P, I, P, P, P, I], P),
         'orc_system_free': ([P], None),
         'orc_system_sizes': ([P, P], I),
@@ -121,6 +123,15 @@ def apply_euler_explicit(dim, p, dt, u_old, K, F):
     K2, F2 = np.array(K, dtype=np.float64, order='C'), np.array(F, dtype=np.float64)
     _chk(lib.orc_apply_euler_explicit(dim, p, dt, _p(np.ascontiguousarray(u_old, dtype=np.float64)), _p(K2), _p(F2)))
     return K2, F2
+
+
+def face_mass(dim, p, xf):
+    """Face mass of IntegratedDirichletModel (Mass on the face element with
+    the face's own nodes xf, nNf x dim, IntegratedDirichletModel.cpp:35-44)."""
+    n = sizes(dim, p)['nNf']
+    M = np.zeros((n, n))
+    _chk(lib.orc_face_mass(dim, p, _p(np.ascontiguousarray(xf, dtype=np.float64)), _p(M)))
+    return M
 
 
 def apply_rk(dim, p, dt, a_row, stage, ut, kst, K, F):
@@ -293,6 +304,13 @@ class System:
         a, uo, ol, sl = self._ts
         nst = 0 if a is None else a.size
         _chk(lib.orc_system_set_time_scheme(self._h, kind, dt, nst, _p(a), stage, _p(uo), _p(ol), _p(sl), nthreads))
+
+    def set_boundary_model(self, kind, faces=None, nthreads=0):
+        """Boundary model of the listed boundary faces (None: all), then
+        reassemble: 0 DirichletModel, 1 IntegratedDirichletModel."""
+        self._bf = None if faces is None else np.ascontiguousarray(faces, dtype=np.int32)
+        n = 0 if self._bf is None else self._bf.size
+        _chk(lib.orc_system_set_boundary_model(self._h, kind, _p(self._bf), n, nthreads))
 
     def local_vectors(self, u, q, lam):
         """Cell-local [u | q | lambda] vectors (nCells x n) of whole-mesh fields."""
