@@ -286,6 +286,7 @@ extern "C" int hfx_hdg_set_mesh(hfx_hdg* h, const hfx_mesh* mm) {
   h->rk_stage = 0;
   h->bc_kind.clear();     // boundary models are per face of the mesh
   h->n_bc_int = 0;
+  h->n_bc_hdg = 0;
   h->mesh = mm->m;
   h->mesh.order = h->order;
   h->affine = mesh_is_affine(*h->re, h->mesh);
@@ -736,12 +737,16 @@ extern "C" int hfx_hdg_set_boundary_model(hfx_hdg* h, int kind, const int* faces
   if (h->bc_kind.size() != (size_t)m.nFaces) h->bc_kind.assign(m.nFaces, HFX_BC_DIRICHLET);
   if (!faces) {
     for (int F = 0; F < m.nFaces; ++F)
-      if (bnd[F]) h->bc_kind[F] = (char)kind;
+      if (bnd[F] && h->bc_kind[F] != 2) h->bc_kind[F] = (char)kind;
   } else {
     require(n >= 0, "hfx_hdg_set_boundary_model: negative face count");
     for (int i = 0; i < n; ++i)
       require(faces[i] >= 0 && faces[i] < m.nFaces && bnd[faces[i]],
               "hfx_hdg_set_boundary_model: not a boundary face");
+    for (int i = 0; i < n; ++i)
+      require(h->bc_kind[faces[i]] != 2,
+              "hfx_hdg_set_boundary_model: the face has an HDG-type boundary model (clear it with "
+              "hfx_hdg_set_boundary_hdg, n = 0)");
     for (int i = 0; i < n; ++i) h->bc_kind[faces[i]] = (char)kind;
   }
   std::vector<int> lst;
@@ -751,6 +756,46 @@ extern "C" int hfx_hdg_set_boundary_model(hfx_hdg* h, int kind, const int* faces
   h->bc_int_faces.upload(lst, h->stream);
   HFX_HIP(hipStreamSynchronize(h->stream));
   h->n_bc_int = (int)lst.size();
+  h->assembled = h->solved = false;
+  ++h->args_version;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_boundary_hdg(hfx_hdg* h, const int* faces, int n, const double* B, const double* r,
+                                        int matrix_set, int rhs_set) {
+  HFX_TRY
+  require(h && h->has_mesh, "hfx_hdg_set_boundary_hdg: set the mesh first");
+  require(n >= 0 && (n == 0 || (faces && B && r)), "hfx_hdg_set_boundary_hdg: null argument");
+  const hfx::Mesh& m = h->mesh;
+  const int nNf = m.nNf, nc = nNf * (2 + h->dim);
+  std::vector<char> bnd(m.nFaces, 0);
+  for (int f : m.boundary) bnd[f] = 1;
+  std::vector<char> seen(m.nFaces, 0);
+  for (int i = 0; i < n; ++i) {
+    require(faces[i] >= 0 && faces[i] < m.nFaces && bnd[faces[i]], "hfx_hdg_set_boundary_hdg: not a boundary face");
+    require(!seen[faces[i]], "hfx_hdg_set_boundary_hdg: a face is listed twice");
+    seen[faces[i]] = 1;
+  }
+  for (size_t i = 0; i < (size_t)n * nNf * nc; ++i)
+    require(std::isfinite(B[i]), "hfx_hdg_set_boundary_hdg: local matrices must be finite");
+  if (h->bc_kind.size() != (size_t)m.nFaces) h->bc_kind.assign(m.nFaces, HFX_BC_DIRICHLET);
+  for (int F = 0; F < m.nFaces; ++F)
+    if (h->bc_kind[F] == 2) h->bc_kind[F] = HFX_BC_DIRICHLET;  // the new set replaces the old one
+  for (int i = 0; i < n; ++i) h->bc_kind[faces[i]] = 2;
+  h->set_device();
+  h->bc_hdg_faces.upload(std::vector<int>(faces, faces + n), h->stream);
+  h->bc_hdg_B.upload(std::vector<double>(B, B + (size_t)n * nNf * nc), h->stream);
+  h->bc_hdg_r.upload(std::vector<double>(r, r + (size_t)n * nNf), h->stream);
+  h->n_bc_hdg = n;
+  h->bc_hdg_mset = matrix_set != 0;
+  h->bc_hdg_rset = rhs_set != 0;
+  // the integrated-Dirichlet list no longer holds these faces
+  std::vector<int> lst;
+  for (int F = 0; F < m.nFaces; ++F)
+    if (h->bc_kind[F] == HFX_BC_INTEGRATED_DIRICHLET) lst.push_back(F);
+  h->bc_int_faces.upload(lst, h->stream);
+  h->n_bc_int = (int)lst.size();
+  h->upload_boundary_flags();
   h->assembled = h->solved = false;
   ++h->args_version;
   HFX_END

@@ -82,6 +82,11 @@ struct hfx_hdg {
   std::vector<char> bc_kind;
   hfx::DevBuf<int> bc_int_faces;
   int n_bc_int = 0;
+  // HDG-type boundary model (hfx_hdg_set_boundary_hdg): faces, local
+  // matrices, rhs, Set / Add
+  hfx::DevBuf<int> bc_hdg_faces;
+  hfx::DevBuf<double> bc_hdg_B, bc_hdg_r;
+  int n_bc_hdg = 0, bc_hdg_mset = 1, bc_hdg_rset = 1;
   // HDGSolverOpts::type (hfx_hdg_set_solver_type); the explicit modes read the
   // problem's "Solution" u and "Flux" q fields at assembly
   int hdg_type = HFX_HDG_IMPLICIT;
@@ -256,6 +261,17 @@ struct hfx_hdg {
                                        cells.p, coords.p, tabs.p + t_fphi, tabs.p + t_fdphi, tabs.p + t_fw, g.p,
                                        rhs.p, s);
     }
+    if (n_bc_hdg > 0)
+      hfx::launch_hdg_boundary(A, n_bc_hdg, bc_hdg_faces.p, dim, re->nN, mesh.nFpc, faceNodes.p, perm.p, invperm.p,
+                               bpos.p, U.p, Q.p, U0.p, Q0.p, bc_hdg_B.p, bc_hdg_r.p, bc_hdg_mset, bc_hdg_rset, rhs.p, s);
+  }
+  // Dirichlet flags of the boundary faces: every boundary face but those of
+  // an HDG-type model (their rows are scattered as the cell's own)
+  void upload_boundary_flags() {
+    std::vector<uint8_t> dir(mesh.nFaces, 0);
+    for (int f : mesh.boundary) dir[f] = (bc_kind.empty() || bc_kind[f] != 2) ? 1 : 0;
+    is_dir.upload(dir, stream);
+    HFX_HIP(hipStreamSynchronize(stream));
   }
   void run_scatter() { scatter_on(stream, last_fused); }
 };

@@ -59,6 +59,16 @@ void launch_integrated_dirichlet(const BsrDev& A, int nbf, const int* bfaces, in
                                  const double* fphi, const double* fdphi, const double* fw, const double* g,
                                  double* rhs, hipStream_t s);
 
+// HDG-type boundary model on the boundary faces hfaces[nhf] (not flagged
+// Dirichlet, so their rows hold the cell's own S and S0), after either
+// scatter: B (nhf x bs x bs(2+dim)) = [B_u | B_q | B_lambda] in face-node
+// order and r (nhf x bs); the rows are set (matrix_set / rhs_set) or added.
+// bpos: cell_block_positions of the pattern.
+void launch_hdg_boundary(const BsrDev& A, int nhf, const int* hfaces, int dim, int nN, int nFpc, const int* fnodes,
+                         const int8_t* face_perm, const int8_t* invperm, const int* bpos, const double* U,
+                         const double* Q, const double* U0, const double* Q0, const double* B, const double* r,
+                         int matrix_set, int rhs_set, double* rhs, hipStream_t s);
+
 // BSR block index of every (face f, face g) pair of every cell:
 // out[(e*nFpc + f)*nFpc + g] (the diagonal block when f == g).
 std::vector<int> cell_block_positions(const BsrPattern& P, int nCells, int nFpc, const int* cell2face);

@@ -195,6 +195,64 @@ __global__ void integrated_dirichlet_rhs_kernel(int nbf, const int* __restrict__
   rhs[(size_t)F * nNf + r] = v;
 }
 
+// HDG-type boundary model rows (HDGSolver.cpp:503-518) of boundary face
+// F = hfaces[ib], whose rows the scatter left as the cell's own: row pa (face
+// order) over every cell-local trace column c becomes (Set) or gains (Add)
+// sum_j B_u[pa][j] U[i_j][c] + sum_(j,k) B_q[pa][j,k] Q[i_j d + k][c]
+// (+ B_lambda[pa][pc] on the face's own block), i_j the cell node at face
+// position j.  One thread per (face, row, column).
+__global__ void hdg_boundary_kernel(int nhf, const int* __restrict__ hfaces, int dim, int nN, int nNf, int nFpc,
+                                    const int* __restrict__ face_cells, const int* __restrict__ fnodes,
+                                    const int8_t* __restrict__ face_perm, const int8_t* __restrict__ invperm,
+                                    const int* __restrict__ bpos, const double* __restrict__ U,
+                                    const double* __restrict__ Q, const double* __restrict__ B, int mset,
+                                    double* __restrict__ vals) {
+  const int NL = nFpc * nNf, nc = nNf * (2 + dim);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nhf * nNf * NL) return;
+  const int ib = (int)(t / ((long long)nNf * NL)), pa = (int)((t / NL) % nNf), c = (int)(t % NL);
+  const int F = hfaces[ib];
+  const int e = face_cells[4 * (size_t)F], lf = face_cells[4 * (size_t)F + 1];
+  const int8_t* ipf = invperm + ((size_t)e * nFpc + lf) * nNf;
+  const double* Bi = B + ((size_t)ib * nNf + pa) * nc;
+  const int nQ = nN * dim;
+  double v = 0.0;
+  for (int pj = 0; pj < nNf; ++pj) {
+    const int i = fnodes[lf * nNf + ipf[pj]];
+    v += Bi[pj] * U[((size_t)e * nN + i) * NL + c];
+    for (int k = 0; k < dim; ++k) v += Bi[nNf + pj * dim + k] * Q[((size_t)e * nQ + i * dim + k) * NL + c];
+  }
+  const int G = c / nNf, b = c % nNf;
+  const int pc = face_perm[((size_t)e * nFpc + G) * nNf + b];
+  if (G == lf) v += Bi[nNf * (1 + dim) + pc];
+  double* dst = vals + (size_t)bpos[((size_t)e * nFpc + lf) * nFpc + G] * nNf * nNf + pa * nNf + pc;
+  *dst = mset ? v : *dst + v;
+}
+
+// rhs of those rows: r - B_u U0_F - B_q Q0_F, set or added
+__global__ void hdg_boundary_rhs_kernel(int nhf, const int* __restrict__ hfaces, int dim, int nN, int nNf, int nFpc,
+                                        const int* __restrict__ face_cells, const int* __restrict__ fnodes,
+                                        const int8_t* __restrict__ invperm, const double* __restrict__ U0,
+                                        const double* __restrict__ Q0, const double* __restrict__ B,
+                                        const double* __restrict__ r, int rset, double* __restrict__ rhs) {
+  const int nc = nNf * (2 + dim);
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nhf * nNf) return;
+  const int ib = (int)(t / nNf), pa = (int)(t % nNf);
+  const int F = hfaces[ib];
+  const int e = face_cells[4 * (size_t)F], lf = face_cells[4 * (size_t)F + 1];
+  const int8_t* ipf = invperm + ((size_t)e * nFpc + lf) * nNf;
+  const double* Bi = B + ((size_t)ib * nNf + pa) * nc;
+  double v = r[(size_t)ib * nNf + pa];
+  for (int pj = 0; pj < nNf; ++pj) {
+    const int i = fnodes[lf * nNf + ipf[pj]];
+    v -= Bi[pj] * U0[(size_t)e * nN + i];
+    for (int k = 0; k < dim; ++k) v -= Bi[nNf + pj * dim + k] * Q0[((size_t)e * nN + i) * dim + k];
+  }
+  double* dst = rhs + (size_t)F * nNf + pa;
+  *dst = rset ? v : *dst + v;
+}
+
 __global__ void reconstruct_kernel(int nEl, int nN, int nQ, int nFpc, int bs, const double* __restrict__ U,
                                    const double* __restrict__ U0, const double* __restrict__ Q,
                                    const double* __restrict__ Q0, const int* __restrict__ cell2face,
@@ -281,6 +339,21 @@ void launch_integrated_dirichlet(const BsrDev& A, int nbf, const int* bfaces, in
   HFX_HIP(hipGetLastError());
   hipLaunchKernelGGL(integrated_dirichlet_rhs_kernel, dim3(ceil_div((long long)nbf * bs, nt)), dim3(nt), 0, s, nbf,
                      bfaces, bs, A.diag.p, A.vals.p, g, rhs);
+  HFX_HIP(hipGetLastError());
+}
+
+void launch_hdg_boundary(const BsrDev& A, int nhf, const int* hfaces, int dim, int nN, int nFpc, const int* fnodes,
+                         const int8_t* face_perm, const int8_t* invperm, const int* bpos, const double* U,
+                         const double* Q, const double* U0, const double* Q0, const double* B, const double* r,
+                         int matrix_set, int rhs_set, double* rhs, hipStream_t s) {
+  if (nhf <= 0) return;
+  const int nt = 256, bs = A.bs;
+  const long long n = (long long)nhf * bs * nFpc * bs;
+  hipLaunchKernelGGL(hdg_boundary_kernel, dim3(ceil_div(n, nt)), dim3(nt), 0, s, nhf, hfaces, dim, nN, bs, nFpc,
+                     A.face_cells.p, fnodes, face_perm, invperm, bpos, U, Q, B, matrix_set, A.vals.p);
+  HFX_HIP(hipGetLastError());
+  hipLaunchKernelGGL(hdg_boundary_rhs_kernel, dim3(ceil_div((long long)nhf * bs, nt)), dim3(nt), 0, s, nhf, hfaces,
+                     dim, nN, bs, nFpc, A.face_cells.p, fnodes, invperm, U0, Q0, B, r, rhs_set, rhs);
   HFX_HIP(hipGetLastError());
 }
 

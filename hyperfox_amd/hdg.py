@@ -249,6 +249,16 @@ class HDGProblem:
             f = np.ascontiguousarray(faces, dtype=np.int32)
             check(lib.hfx_hdg_set_boundary_model(self._h, kind, _ptr(f), f.size))
 
+    def set_boundary_hdg(self, faces, B, r, matrix_set=True, rhs_set=True):
+        """HDG-type boundary model on the listed boundary faces: B (n, nNf,
+        nNf (2 + dim)) = [B_u | B_q | B_lambda] in face-node order, r (n,
+        nNf), Set or Add assembly; an empty list clears it."""
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        B = _f64(B)
+        r = _f64(r)
+        check(lib.hfx_hdg_set_boundary_hdg(self._h, _ptr(f), f.size, _ptr(B), _ptr(r), int(bool(matrix_set)),
+                                           int(bool(rhs_set))))
+
     def set_trace(self, lam):
         """Set the trace field (nFaces x nNf)."""
         lam = _f64(lam)

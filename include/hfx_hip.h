@@ -212,6 +212,16 @@ int hfx_hdg_rk_stages(hfx_hdg* h, int* out);
  * Both Set the face's rows and RHS.  A non-boundary face is an error. */
 enum { HFX_BC_DIRICHLET = 0, HFX_BC_INTEGRATED_DIRICHLET = 1 };
 int hfx_hdg_set_boundary_model(hfx_hdg* h, int kind, const int* faces, int n);
+/* HDG-type boundary model (BoundaryModelType HDGType, applied by
+ * HDGSolver::applyBoundaryConditions, HDGSolver.cpp:503-518) on the boundary
+ * faces `faces` (n; n = 0 clears; replaces any earlier HDG-type set): B holds
+ * n local matrices of nNf x nNf(2 + dim), [B_u | B_q (node-major, dim per
+ * node) | B_lambda] in face-node order, as a boundary model's compute()
+ * returns them, r n right-hand sides of nNf; matrix_set / rhs_set choose Set
+ * (nonzero) or Add assembly.  Each face's rows become (or gain)
+ * B_u U_F + B_q Q_F + B_lambda and r - B_u U0_F - B_q Q0_F. */
+int hfx_hdg_set_boundary_hdg(hfx_hdg* h, const int* faces, int n, const double* B, const double* r, int matrix_set,
+                             int rhs_set);
 /* Set the "Trace" field lambda (nFaces x nNf, face-node order), e.g. an
  * initial condition; hfx_hdg_solve overwrites it. */
 int hfx_hdg_set_trace(hfx_hdg* h, const double* lambda);

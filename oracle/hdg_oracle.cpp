@@ -916,7 +916,12 @@ struct orc_system {
   std::vector<int> cells, faces, c2f, f2c;
   std::vector<int> perm;              // nEl*nF*nNf : face2CellMap
   std::vector<char> is_bnd;           // nFaces
-  std::vector<char> bc_kind;          // per face: 0 DirichletModel, 1 IntegratedDirichletModel
+  std::vector<char> bc_kind;          // per face: 0 DirichletModel, 1 IntegratedDirichletModel, 2 HDG type
+  // HDG-type boundary model per face: index into hb/hr, local matrix
+  // [B_u | B_q | B_lambda] (nNf x nNf (2 + d)) and rhs (nNf), assembly types
+  std::vector<int> hdg_idx;
+  std::vector<double> hb, hr;
+  int hdg_mset = 1, hdg_rset = 1;
   std::vector<double> S, S0, U, Q, U0, Q0;
   double ts_dt = 0.0;          // time step (0: no time scheme)
   int ts_kind = 0;             // 1 implicit Euler, 2 explicit Euler, 3 Runge-Kutta stage
@@ -979,6 +984,41 @@ void system_assemble(orc_system* s, int nEl, int nthreads) {
       for (int f = 0; f < nF; ++f) {
         int fc = s->c2f[(size_t)e * nF + f];
         if (!s->is_bnd[fc]) continue;
+        if (!s->bc_kind.empty() && s->bc_kind[fc] == 2) {
+          // HDG-type boundary model (HDGSolver.cpp:489-518): Set zeroes the
+          // face's rows of S / S0; then S_f += B_u U_F + B_q Q_F,
+          // S_ff += B_lambda, S0_f += r - B_u U0_F - B_q Q0_F, with U_F, Q_F the
+          // rows of U, Q at the face's nodes (face order = this cell's order)
+          const int ib = s->hdg_idx[fc];
+          const int nc = nNf * (2 + d);
+          const double* B = &s->hb[(size_t)ib * nNf * nc];
+          const double* r = &s->hr[(size_t)ib * nNf];
+          const double* Ue = &s->U[(size_t)e * lu * ll];
+          const double* Qe = &s->Q[(size_t)e * lq * ll];
+          const double* U0e = &s->U0[(size_t)e * lu];
+          const double* Q0e = &s->Q0[(size_t)e * lq];
+          for (int a = 0; a < nNf; ++a) {
+            double* row = Se + (size_t)(f * nNf + a) * ll;
+            if (s->hdg_mset)
+              for (int j = 0; j < ll; ++j) row[j] = 0.0;
+            if (s->hdg_rset) S0e[f * nNf + a] = 0.0;
+            double rr = r[a];
+            for (int jn = 0; jn < nNf; ++jn) {
+              const int i = R.fmap[f][jn];
+              const double bu = B[a * nc + jn];
+              for (int c = 0; c < ll; ++c) row[c] += bu * Ue[(size_t)i * ll + c];
+              rr -= bu * U0e[i];
+              for (int k = 0; k < d; ++k) {
+                const double bq = B[a * nc + nNf + jn * d + k];
+                for (int c = 0; c < ll; ++c) row[c] += bq * Qe[(size_t)(i * d + k) * ll + c];
+                rr -= bq * Q0e[i * d + k];
+              }
+            }
+            for (int b = 0; b < nNf; ++b) row[f * nNf + b] += B[a * nc + nNf * (1 + d) + b];
+            S0e[f * nNf + a] += rr;
+          }
+          continue;
+        }
         const bool integ = !s->bc_kind.empty() && s->bc_kind[fc] == 1;
         std::vector<double> xf, M;
         if (integ) {
@@ -1254,6 +1294,28 @@ extern "C" int orc_system_set_boundary_model(orc_system* s, int kind, const int*
       s->bc_kind[faces[i]] = (char)kind;
     }
   }
+  system_assemble(s, s->nEl, nthreads > 0 ? nthreads : omp_get_max_threads());
+  ORC_END
+}
+
+extern "C" int orc_system_set_boundary_hdg(orc_system* s, const int* faces, int nfaces, const double* B,
+                                           const double* r, int matrix_set, int rhs_set, int nthreads) {
+  ORC_TRY
+  const int nNf = s->R->nNf, nc = nNf * (2 + s->dim);
+  if (s->bc_kind.empty()) s->bc_kind.assign(s->nFaces, 0);
+  s->hdg_idx.assign(s->nFaces, -1);
+  for (int F = 0; F < s->nFaces; ++F)
+    if (s->bc_kind[F] == 2) s->bc_kind[F] = 0;
+  for (int i = 0; i < nfaces; ++i) {
+    if (faces[i] < 0 || faces[i] >= s->nFaces || !s->is_bnd[faces[i]])
+      throw std::runtime_error("boundary model: not a boundary face");
+    s->bc_kind[faces[i]] = 2;
+    s->hdg_idx[faces[i]] = i;
+  }
+  s->hb.assign(B, B + (size_t)nfaces * nNf * nc);
+  s->hr.assign(r, r + (size_t)nfaces * nNf);
+  s->hdg_mset = matrix_set != 0;
+  s->hdg_rset = rhs_set != 0;
   system_assemble(s, s->nEl, nthreads > 0 ? nthreads : omp_get_max_threads());
   ORC_END
 }

@@ -120,6 +120,13 @@ int orc_system_set_time_scheme(orc_system* s, int kind, double dt, int nst, cons
  * rhs g), 1 IntegratedDirichletModel (Set/Set, the face mass M and rhs M g,
  * IntegratedDirichletModel.cpp:35-56). */
 int orc_system_set_boundary_model(orc_system* s, int kind, const int* faces, int nfaces, int nthreads);
+/* HDG-type boundary model (HDGSolver.cpp:489-518) on the boundary faces
+ * `faces` (replacing any earlier HDG-type set): B nfaces x nNf x nNf(2+dim),
+ * the model's local matrix [B_u | B_q (node-major, dim per node) | B_lambda]
+ * in face-node order, r nfaces x nNf its rhs; matrix_set / rhs_set: Set (1)
+ * or Add (0) assembly.  Then reassemble. */
+int orc_system_set_boundary_hdg(orc_system* s, const int* faces, int nfaces, const double* B, const double* r,
+                                int matrix_set, int rhs_set, int nthreads);
 /* Face mass of IntegratedDirichletModel: xf nNf*dim face nodes (face-node
  * order), M nNf*nNf. */
 int orc_face_mass(int dim, int p, const double* xf, double* M);

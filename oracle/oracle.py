@@ -44,6 +44,7 @@ def _load():
         'orc_system_local_vectors': ([P] * 5, I),
         'orc_system_set_boundary_model': ([P, I, P, I, I], I),
         'orc_face_mass': ([I, I, P, P], I),
+        'orc_system_set_boundary_hdg': ([P, P, I, P, P, I, I, I], I),
         'orc_system_build': ([I, I, Model, I, P, I, P, I, P, P, P, P, P, P, I, P, P, P, I], P),
         'orc_system_free': ([P], None),
         'orc_system_sizes': ([P, P], I),
@@ -311,6 +312,16 @@ class System:
         self._bf = None if faces is None else np.ascontiguousarray(faces, dtype=np.int32)
         n = 0 if self._bf is None else self._bf.size
         _chk(lib.orc_system_set_boundary_model(self._h, kind, _p(self._bf), n, nthreads))
+
+    def set_boundary_hdg(self, faces, B, r, matrix_set=True, rhs_set=True, nthreads=0):
+        """HDG-type boundary model on the listed boundary faces: B (n, nNf,
+        nNf (2 + dim)) = [B_u | B_q | B_lambda] in face-node order, r (n, nNf),
+        Set or Add assembly; then reassemble."""
+        self._hf = np.ascontiguousarray(faces, dtype=np.int32)
+        self._hb = np.ascontiguousarray(B, dtype=np.float64)
+        self._hr = np.ascontiguousarray(r, dtype=np.float64)
+        _chk(lib.orc_system_set_boundary_hdg(self._h, _p(self._hf), self._hf.size, _p(self._hb), _p(self._hr),
+                                             int(matrix_set), int(rhs_set), nthreads))
 
     def local_vectors(self, u, q, lam):
         """Cell-local [u | q | lambda] vectors (nCells x n) of whole-mesh fields."""

@@ -100,3 +100,50 @@ def test_boundary_model_arguments_are_checked():
     rp, cols, vals, rhs = prob.trace_csr()
     s = oracle_system(w, f)
     assert rel(vals, s.csr()[2]) < 1e-12
+
+
+# ---------------------------------------------------------------- HDG type
+
+
+@pytest.mark.parametrize('mode', ['set', 'add'])
+@pytest.mark.parametrize('kernel', [0, 1])
+@pytest.mark.parametrize('dim,p', [(2, 1), (2, 3), (2, 5), (3, 2), (3, 3)])
+def test_hdg_type_boundary_matches_oracle(dim, p, kernel, mode):
+    from test_boundary_models_oracle import hdg_B
+    w = small_workload(dim, p, 2)
+    prob, f = run_gpu(w)
+    prob.set_kernel(kernel)
+    bf = boundary_faces(w)[::2]
+    B, r = hdg_B(dim, p, bf.size, 13)
+    st = mode == 'set'
+    prob.set_boundary_model(hf.BC_INTEGRATED_DIRICHLET, boundary_faces(w)[1::4])  # with a mix of the others
+    prob.set_boundary_hdg(bf, B, r, matrix_set=st, rhs_set=st)
+    prob.assemble()
+    rp, cols, vals, rhs = prob.trace_csr()
+    s = oracle_system(w, f)
+    s.set_boundary_model(1, boundary_faces(w)[1::4])
+    s.set_boundary_hdg(bf, B, r, matrix_set=st, rhs_set=st)
+    rpo, colso, valso, rhso = s.csr()
+    assert np.array_equal(cols, colso)
+    assert rel(vals, valso) < 1e-12, rel(vals, valso)
+    assert rel(rhs, rhso) < 1e-12, rel(rhs, rhso)
+
+
+def test_hdg_type_arguments_are_checked_and_clearing_restores_dirichlet():
+    from test_boundary_models_oracle import hdg_B
+    w = small_workload(2, 2, 1)
+    prob, f = run_gpu(w)
+    bf = boundary_faces(w)
+    B, r = hdg_B(2, 2, 2, 1)
+    interior = np.nonzero(np.asarray(w.mesh.arrays()['face2cell'])[:, 1] >= 0)[0]
+    with pytest.raises(hf.HfxError, match='not a boundary face'):
+        prob.set_boundary_hdg([bf[0], interior[0]], B, r)
+    with pytest.raises(hf.HfxError, match='twice'):
+        prob.set_boundary_hdg([bf[0], bf[0]], B, r)
+    prob.set_boundary_hdg(bf[:2], B, r)
+    with pytest.raises(hf.HfxError, match='HDG-type'):
+        prob.set_boundary_model(hf.BC_INTEGRATED_DIRICHLET, bf[:1])
+    prob.set_boundary_hdg([], np.zeros((0, 3, 12)), np.zeros((0, 3)))
+    prob.assemble()
+    vals = prob.trace_csr()[2]
+    assert rel(vals, oracle_system(w, f).csr()[2]) < 1e-12

@@ -119,3 +119,76 @@ def test_boundary_model_per_face_and_checks():
         s.set_boundary_model(1, interior[:1])
     with pytest.raises(orc.OracleError, match='unknown kind'):
         s.set_boundary_model(3)
+
+
+# ---------------------------------------------------------------- HDG type
+# The HDG-type branch of applyBoundaryConditions (HDGSolver.cpp:503-518): a
+# boundary model's local matrix [B_u | B_q | B_lambda] and rhs r act on the
+# face's rows through the cell's U, Q, U0, Q0.
+
+
+def hdg_B(dim, p, nfaces, seed, scale=0.3):
+    rng = np.random.default_rng(seed)
+    nNf = orc.sizes(dim, p)['nNf']
+    B = scale * rng.normal(size=(nfaces, nNf, nNf * (2 + dim)))
+    B[:, :, nNf * (1 + dim):] += 2.0 * np.eye(nNf)     # keep the face rows well posed
+    return B, rng.normal(size=(nfaces, nNf))
+
+
+def test_hdg_type_model_can_state_dirichlet():
+    m, a, g, mk = system(2, 2)
+    bf = np.nonzero(np.asarray(a['face2cell'])[:, 1] < 0)[0]
+    nNf = m.sizes['nNf']
+    B = np.zeros((bf.size, nNf, nNf * 4))
+    B[:, :, 3 * nNf:] = np.eye(nNf)
+    s1 = mk()
+    s1.set_boundary_hdg(bf, B, g[bf])
+    v0, r0 = mk().csr()[2:]
+    v1, r1 = s1.csr()[2:]
+    assert np.array_equal(v1, v0) and np.array_equal(r1, r0)
+
+
+@pytest.mark.parametrize('dim,p', [(2, 1), (2, 3), (3, 2)])
+def test_hdg_type_rows_impose_their_constraint(dim, p):
+    """After the solve, B_u u_F + B_q q_F + B_lambda lambda_F = r on every
+    such face, with u, q the reconstructed fields."""
+    m, a, g, mk = system(dim, p)
+    bf = np.nonzero(np.asarray(a['face2cell'])[:, 1] < 0)[0][::2]
+    B, r = hdg_B(dim, p, bf.size, 7)
+    s = mk()
+    s.set_boundary_hdg(bf, B, r)
+    lam = s.solve(method=1)[0]
+    u, q = s.reconstruct(lam)
+    nNf = m.sizes['nNf']
+    fn = orc.refel(dim, p)['face_nodes']
+    for i, F in enumerate(bf):
+        e = a['face2cell'][F, 0]
+        lf = list(a['cell2face'][e]).index(F)
+        nodes = fn[lf]
+        uF = u[e, nodes]
+        qF = q[e].reshape(-1, dim)[nodes].reshape(-1)
+        lF = lam[F * nNf:(F + 1) * nNf]
+        res = B[i, :, :nNf] @ uF + B[i, :, nNf:nNf * (1 + dim)] @ qF + B[i, :, nNf * (1 + dim):] @ lF - r[i]
+        assert np.abs(res).max() < 1e-9 * max(1.0, np.abs(r[i]).max()), (F, np.abs(res).max())
+
+
+def test_hdg_type_add_is_set_plus_the_natural_rows():
+    m, a, g, mk = system(2, 3)
+    bf = np.nonzero(np.asarray(a['face2cell'])[:, 1] < 0)[0][:5]
+    B, r = hdg_B(2, 3, bf.size, 9)
+    out = {}
+    for name, BB, rr, st in (('set', B, r, True), ('add', B, r, False), ('nat', 0 * B, 0 * r, False)):
+        s = mk()
+        s.set_boundary_hdg(bf, BB, rr, matrix_set=st, rhs_set=st)
+        out[name] = s.csr()
+    rp, cols = out['set'][:2]
+    nNf = m.sizes['nNf']
+    rows = np.concatenate([np.arange(F * nNf, (F + 1) * nNf) for F in bf])
+    for rw in rows:
+        sl = slice(rp[rw], rp[rw + 1])
+        assert np.allclose(out['add'][2][sl], out['set'][2][sl] + out['nat'][2][sl], rtol=1e-13, atol=1e-14)
+        assert np.isclose(out['add'][3][rw], out['set'][3][rw] + out['nat'][3][rw], rtol=1e-13, atol=1e-14)
+    # the natural rows are the cell's own flux rows: no identity block left
+    F = bf[0]
+    sl = slice(rp[F * nNf], rp[F * nNf + 1])
+    assert not np.allclose(out['nat'][2][sl][cols[sl] == F * nNf], 1.0)
