@@ -30,6 +30,8 @@ _SIGS = {
     'hfx_cubature': ([_I, _I, _P, _P], _I),
     'hfx_mesh_structured': ([_I, _I, _I, _D, ctypes.c_ulonglong, _P], _I),
     'hfx_mesh_from_cells': ([_I, _I, _I, _P, _I, _P, _P], _I),
+    'hfx_mesh_structured_gpu': ([_I, _I, _I, _I, _D, ctypes.c_ulonglong, _P], _I),
+    'hfx_mesh_from_cells_gpu': ([_I, _I, _I, _I, _P, _I, _P, _P], _I),
     'hfx_mesh_sizes': ([_P, _P], _I),
     'hfx_mesh_get': ([_P] * 8, _I),
     'hfx_mesh_free': ([_P], None),

@@ -17,6 +17,7 @@
 #include "krylov.h"
 #include "mesh.h"
 #include "refel.h"
+#include "topology.h"
 #include "trace_system.h"
 #include "capi_internal.h"
 
@@ -203,6 +204,28 @@ extern "C" int hfx_mesh_from_cells(int dim, int order, int nNodes, const double*
   auto m = std::unique_ptr<hfx_mesh>(new hfx_mesh());
   m->re.reset(new hfx::RefEl(dim, order));
   m->m = hfx::mesh_from_cells(*m->re, nNodes, coords, nCells, cells);
+  *out = m.release();
+  HFX_END
+}
+
+extern "C" int hfx_mesh_structured_gpu(int device, int dim, int order, int n, double jitter, unsigned long long seed,
+                                       hfx_mesh** out) {
+  HFX_TRY
+  auto m = std::unique_ptr<hfx_mesh>(new hfx_mesh());
+  m->re.reset(new hfx::RefEl(dim, order));
+  m->m = hfx::structured_mesh(*m->re, n, jitter, seed, false);
+  hfx::build_topology_gpu(*m->re, m->m, device);
+  *out = m.release();
+  HFX_END
+}
+
+extern "C" int hfx_mesh_from_cells_gpu(int device, int dim, int order, int nNodes, const double* coords, int nCells,
+                                       const int* cells, hfx_mesh** out) {
+  HFX_TRY
+  auto m = std::unique_ptr<hfx_mesh>(new hfx_mesh());
+  m->re.reset(new hfx::RefEl(dim, order));
+  m->m = hfx::mesh_from_cells(*m->re, nNodes, coords, nCells, cells, false);
+  hfx::build_topology_gpu(*m->re, m->m, device);
   *out = m.release();
   HFX_END
 }

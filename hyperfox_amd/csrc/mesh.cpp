@@ -92,7 +92,7 @@ void build_topology(const RefEl& re, Mesh& m) {
   }
 }
 
-Mesh mesh_from_cells(const RefEl& re, int nNodes, const double* coords, int nCells, const int* cells) {
+Mesh mesh_from_cells(const RefEl& re, int nNodes, const double* coords, int nCells, const int* cells, bool topology) {
   Mesh m;
   m.dim = re.dim;
   m.nN = re.nN;
@@ -102,11 +102,11 @@ Mesh mesh_from_cells(const RefEl& re, int nNodes, const double* coords, int nCel
   m.cells.assign(cells, cells + (size_t)nCells * re.nN);
   for (int v : m.cells)
     if (v < 0 || v >= nNodes) throw std::runtime_error("mesh: cell node index out of range");
-  build_topology(re, m);
+  if (topology) build_topology(re, m);
   return m;
 }
 
-Mesh structured_mesh(const RefEl& re, int n, double jit, uint64_t seed) {
+Mesh structured_mesh(const RefEl& re, int n, double jit, uint64_t seed, bool topology) {
   const int d = re.dim;
   if (d != 2 && d != 3) throw std::runtime_error("structured_mesh: dimension must be 2 or 3");
   if (n < 1) throw std::runtime_error("structured_mesh: need n >= 1");
@@ -219,7 +219,7 @@ Mesh structured_mesh(const RefEl& re, int n, double jit, uint64_t seed) {
   }
   m.coords.swap(coords);
   m.nNodes = (int)(m.coords.size() / d);
-  build_topology(re, m);
+  if (topology) build_topology(re, m);
   return m;
 }
 

@@ -41,10 +41,12 @@ void build_topology(const RefEl& re, Mesh& m);
 // dim 3: n^3 cubes, six tetrahedra around the (i,j,k)-(i+1,j+1,k+1) diagonal.
 // High-order nodes are placed by the affine map of the reference nodes
 // (tools/convertGmsh2H5HO.cpp:367-397) and shared between cells.
-Mesh structured_mesh(const RefEl& re, int n, double jit, uint64_t seed);
+// topology = false leaves the face arrays for build_topology_gpu.
+Mesh structured_mesh(const RefEl& re, int n, double jit, uint64_t seed, bool topology = true);
 
 // Mesh from caller-supplied high-order cells and node coordinates.
-Mesh mesh_from_cells(const RefEl& re, int nNodes, const double* coords, int nCells, const int* cells);
+Mesh mesh_from_cells(const RefEl& re, int nNodes, const double* coords, int nCells, const int* cells,
+                     bool topology = true);
 
 uint64_t splitmix64(uint64_t& state);
 double splitmix_uniform(uint64_t& state);  // U[0,1)

@@ -73,18 +73,27 @@ class Mesh:
         self._arrays = None
 
     @classmethod
-    def structured(cls, dim, order, n, jitter=0.0, seed=0):
+    def structured(cls, dim, order, n, jitter=0.0, seed=0, device=None):
+        """device: build the topology on that GPU (same arrays as the host build)."""
         h = ctypes.c_void_p()
-        check(lib.hfx_mesh_structured(dim, order, n, float(jitter), ctypes.c_ulonglong(seed), ctypes.byref(h)))
+        if device is None:
+            check(lib.hfx_mesh_structured(dim, order, n, float(jitter), ctypes.c_ulonglong(seed), ctypes.byref(h)))
+        else:
+            check(lib.hfx_mesh_structured_gpu(int(device), dim, order, n, float(jitter), ctypes.c_ulonglong(seed),
+                                              ctypes.byref(h)))
         return cls(h, dim, order)
 
     @classmethod
-    def from_cells(cls, dim, order, nodes, cells):
+    def from_cells(cls, dim, order, nodes, cells, device=None):
         nodes = _f64(nodes)
         cells = _i32(cells)
         h = ctypes.c_void_p()
-        check(lib.hfx_mesh_from_cells(dim, order, nodes.shape[0], _ptr(nodes), cells.shape[0], _ptr(cells),
-                                      ctypes.byref(h)))
+        if device is None:
+            check(lib.hfx_mesh_from_cells(dim, order, nodes.shape[0], _ptr(nodes), cells.shape[0], _ptr(cells),
+                                          ctypes.byref(h)))
+        else:
+            check(lib.hfx_mesh_from_cells_gpu(int(device), dim, order, nodes.shape[0], _ptr(nodes), cells.shape[0],
+                                              _ptr(cells), ctypes.byref(h)))
         return cls(h, dim, order)
 
     def arrays(self):

@@ -63,6 +63,16 @@ int hfx_mesh_from_cells(int dim, int order, int nNodes, const double* coords, in
                         hfx_mesh** out);
 
 /* out[0..4] = nNodes, nCells, nFaces, nBoundaryFaces, trace DOFs. */
+/* The same meshes with the topology (faces, cell2face, face2cell, boundary,
+ * face-node permutations; the reference's MOAB computeFaces,
+ * src/mesh/Mesh.cpp:183-537) built on GPU `device` by a key sort instead of
+ * the host hash pass; identical arrays.  3-D meshes need fewer than 2^21
+ * vertices. */
+int hfx_mesh_structured_gpu(int device, int dim, int order, int n, double jitter, unsigned long long seed,
+                            hfx_mesh** out);
+int hfx_mesh_from_cells_gpu(int device, int dim, int order, int nNodes, const double* coords, int nCells,
+                            const int* cells, hfx_mesh** out);
+
 int hfx_mesh_sizes(const hfx_mesh* m, long long* out);
 
 /* Copies of the topology (Mesh getters, Mesh.cpp:145-161, 292-295, 435-448,
