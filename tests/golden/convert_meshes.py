@@ -37,7 +37,9 @@ FILES = [
 def dump(h5dump, path, dset, dtype):
     with tempfile.TemporaryDirectory() as td:
         out = os.path.join(td, 'o.txt')
-        subprocess.run([h5dump, '-d', dset, '-y', '-w', '0', '-o', out, path], check=True,
+        # %.17g: every double printed with enough digits to read back exactly
+        fmt = ['-m', '%.17g'] if dtype == np.float64 else []
+        subprocess.run([h5dump, '-d', dset, '-y', '-w', '0'] + fmt + ['-o', out, path], check=True,
                        stdout=subprocess.DEVNULL)
         hdr = subprocess.run([h5dump, '-H', '-d', dset, path], check=True,
                              capture_output=True, text=True).stdout
