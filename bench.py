@@ -91,7 +91,7 @@ def cpu_baseline(n_cpu, p, threads, n_cpu1=8):
             'host_cpus': os.cpu_count()}
 
 
-PC_NAMES = {'jacobi': 1, 'block-jacobi': 2, 'learned': 3, 'ilu0': 4, 'ilu0-natural': 5}
+PC_NAMES = {'jacobi': 1, 'block-jacobi': 2, 'learned': 3, 'ilu0': 4, 'ilu0-natural': 5, 'ilu0-f32': 6}
 
 
 def solver_bytes_per_iter(n, nb, bs, nnzb, method, pc='block-jacobi'):
@@ -101,10 +101,12 @@ def solver_bytes_per_iter(n, nb, bs, nnzb, method, pc='block-jacobi'):
     vector passes (two Gram-Schmidt sweeps).  Applies: point Jacobi 3 vector
     passes; block Jacobi the inverse diagonal blocks + 2 passes; ILU(0) every
     factor block (FP64) with its column id + the inverse pivot blocks + 4
-    passes; learned the FP32 BSR matrix + column ids + 2 passes."""
+    passes (ilu0-f32: the factor blocks in FP32); learned the FP32 BSR matrix
+    + column ids + 2 passes."""
     spmv = nnzb * bs * bs * 8 + nnzb * 4 + 2 * n * 8
     apply = {'jacobi': 3 * n * 8, 'block-jacobi': nb * bs * bs * 8 + 2 * n * 8,
              'ilu0': nnzb * bs * bs * 8 + nnzb * 8 + nb * bs * bs * 8 + 4 * n * 8,
+             'ilu0-f32': nnzb * bs * bs * 4 + nnzb * 8 + nb * bs * bs * 4 + 4 * n * 8,
              'learned': nnzb * bs * bs * 4 + nnzb * 4 + 2 * n * 8}[pc.replace('-natural', '')]
     if method == 'bicgstab':
         return 2 * spmv + 2 * apply + 21 * n * 8
@@ -134,8 +136,8 @@ def trace_solves(hf, pr, prob, w):
     bs = prob.sizes['nNf']
     nb = n_dofs // bs
     out = {}
-    for method, pc in (('bicgstab', 'block-jacobi'), ('bicgstab', 'ilu0'), ('fgmres30', 'jacobi'),
-                       ('fgmres30', 'block-jacobi'), ('fgmres30', 'ilu0')):
+    for method, pc in (('bicgstab', 'block-jacobi'), ('bicgstab', 'ilu0'), ('bicgstab', 'ilu0-f32'),
+                       ('fgmres30', 'jacobi'), ('fgmres30', 'block-jacobi'), ('fgmres30', 'ilu0')):
         r = timed_solve(hf, prob, method, pc, 20000)
         by = solver_bytes_per_iter(n_dofs, nb, bs, nnzb, method, pc) * r['iters']
         r.update(algorithmic_GB=by / 1e9, achieved_GBs=by / (r['ms'] * 1e-3) / 1e9)

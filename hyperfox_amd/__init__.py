@@ -13,7 +13,7 @@ from .linalg import HipLinAlgebra, HipOpts  # noqa: F401
 MODEL_LAPLACE = 0
 MODEL_DIFFUSION_SOURCE = 1
 MODEL_CDR_SOURCE = 2
-PC_NONE, PC_JACOBI, PC_BJACOBI, PC_LEARNED, PC_ILU0, PC_ILU0_NATURAL = 0, 1, 2, 3, 4, 5
+PC_NONE, PC_JACOBI, PC_BJACOBI, PC_LEARNED, PC_ILU0, PC_ILU0_NATURAL, PC_ILU0_F32 = 0, 1, 2, 3, 4, 5, 6
 ILU_MULTICOLOUR, ILU_NATURAL = 0, 1
 KRYLOV_FGMRES, KRYLOV_BICGSTAB = 0, 1
 TS_NONE, TS_BEULER, TS_FEULER, TS_RK = 0, 1, 2, 3

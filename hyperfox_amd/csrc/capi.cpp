@@ -546,12 +546,13 @@ std::function<void(const double*, double*)> make_pc(hfx_hdg* h, int pc) {
     hfx::lpc_prepare(h);
     const hfx::LpcInfer* st = h->lpc_state.get();
     M = [st, s](const double* in, double* out) { hfx::bsr_spmv_f32vals(st->g, st->mvals.p, in, out, s); };
-  } else if (pc == HFX_PC_ILU0 || pc == HFX_PC_ILU0_NATURAL) {
-    const int ord = pc == HFX_PC_ILU0 ? hfx::ILU_MULTICOLOUR : hfx::ILU_NATURAL;
+  } else if (pc == HFX_PC_ILU0 || pc == HFX_PC_ILU0_NATURAL || pc == HFX_PC_ILU0_F32) {
+    const int ord = pc == HFX_PC_ILU0_NATURAL ? hfx::ILU_NATURAL : hfx::ILU_MULTICOLOUR;
     if (!h->ilu || h->ilu->ordering != ord) {
       h->ilu.reset(new hfx::Ilu());
       h->ilu->setup(h->A.nb, h->A.bs, h->pat.rowptr.data(), h->pat.col.data(), ord, s);
     }
+    h->ilu->low = pc == HFX_PC_ILU0_F32;
     h->ilu->factor(h->A, s);
     hfx::Ilu* f = h->ilu.get();
     M = [f, s](const double* in, double* out) { f->apply(in, out, s); };

@@ -21,6 +21,10 @@ struct Ilu {
   DevBuf<int> rowptr, fwd_rows, bwd_rows, lptr, lpos, lcol, uptr, upos, ucol, dslot, pptr, pslot, ppos, err;
   DevBuf<signed char> kind;
   DevBuf<double> lu, dinv, tmp;
+  // low-precision sweeps: float copies of the factors, made after each
+  // factorisation (the sweeps widen them and sum in double)
+  bool low = false;
+  DevBuf<float> lu32, dinv32;
   // Analyse the pattern (host) and upload the schedule.
   void setup(int nb, int bs, const int* rowptr_host, const int* col_host, int ordering, hipStream_t s);
   // Numeric factorisation of A (same pattern); throws on a singular pivot block.

@@ -6,6 +6,8 @@
 #include "bsr_row.h"
 #include "ilu.h"
 
+#include <algorithm>
+
 namespace hfx {
 
 namespace {
@@ -134,45 +136,56 @@ __global__ __launch_bounds__(FT) void ilu_factor_kernel(int bs, const int* __res
 }
 
 // Forward sweep over one level: y_i = x_i - sum_k L_ik y_k.
-template <int BS>
+template <int BS, typename V>
 __global__ __launch_bounds__(AT) void ilu_fwd_kernel(int nrows, const int* __restrict__ rows,
                                                      const int* __restrict__ lptr, const int* __restrict__ lpos,
-                                                     const int* __restrict__ lcol, const double* __restrict__ lu,
+                                                     const int* __restrict__ lcol, const V* __restrict__ lu,
                                                      const double* __restrict__ x, double* __restrict__ y) {
   const int t = blockIdx.x * AT + threadIdx.x;
   if (t >= nrows * BS) return;
   const int i = rows[t / BS], r = t % BS;
-  y[(size_t)i * BS + r] = x[(size_t)i * BS + r] - bsr_row_sum<BS>(lptr[i], lptr[i + 1], r, lpos, lcol, lu, y);
+  y[(size_t)i * BS + r] = x[(size_t)i * BS + r] - bsr_row_sum<BS, V>(lptr[i], lptr[i + 1], r, lpos, lcol, lu, y);
 }
 
 // Backward sweep over one level: z_i = Dinv_i y_i - sum_j Uhat_ij z_j.
-template <int BS>
+template <int BS, typename V>
 __global__ __launch_bounds__(AT) void ilu_bwd_kernel(int nrows, const int* __restrict__ rows,
                                                      const int* __restrict__ uptr, const int* __restrict__ upos,
-                                                     const int* __restrict__ ucol, const double* __restrict__ lu,
-                                                     const double* __restrict__ dinv, const double* __restrict__ y,
+                                                     const int* __restrict__ ucol, const V* __restrict__ lu,
+                                                     const V* __restrict__ dinv, const double* __restrict__ y,
                                                      double* __restrict__ z) {
   const int t = blockIdx.x * AT + threadIdx.x;
   if (t >= nrows * BS) return;
   const int i = rows[t / BS], r = t % BS;
-  z[(size_t)i * BS + r] = bsr_row_dot<BS>(dinv + ((size_t)i * BS + r) * BS, y + (size_t)i * BS) -
-                          bsr_row_sum<BS>(uptr[i], uptr[i + 1], r, upos, ucol, lu, z);
+  z[(size_t)i * BS + r] = bsr_row_dot<BS, V>(dinv + ((size_t)i * BS + r) * BS, y + (size_t)i * BS) -
+                          bsr_row_sum<BS, V>(uptr[i], uptr[i + 1], r, upos, ucol, lu, z);
+}
+
+__global__ void to_float_kernel(size_t n, const double* __restrict__ a, float* __restrict__ b) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
+template <int BS, typename V>
+void sweeps_v(const Ilu& f, const V* lu, const V* dinv, const double* x, double* y, double* z, hipStream_t s) {
+  const IluPlan& P = f.plan;
+  for (int l = 0; l < P.nlev_f; ++l) {
+    const int n = P.fwd_off[l + 1] - P.fwd_off[l];
+    hipLaunchKernelGGL((ilu_fwd_kernel<BS, V>), dim3(ceil_div((long long)n * BS, AT)), dim3(AT), 0, s, n,
+                       f.fwd_rows.p + P.fwd_off[l], f.lptr.p, f.lpos.p, f.lcol.p, lu, x, y);
+  }
+  for (int l = 0; l < P.nlev_b; ++l) {
+    const int n = P.bwd_off[l + 1] - P.bwd_off[l];
+    hipLaunchKernelGGL((ilu_bwd_kernel<BS, V>), dim3(ceil_div((long long)n * BS, AT)), dim3(AT), 0, s, n,
+                       f.bwd_rows.p + P.bwd_off[l], f.uptr.p, f.upos.p, f.ucol.p, lu, dinv, y, z);
+  }
+  HFX_HIP(hipGetLastError());
 }
 
 template <int BS>
 void sweeps(const Ilu& f, const double* x, double* y, double* z, hipStream_t s) {
-  const IluPlan& P = f.plan;
-  for (int l = 0; l < P.nlev_f; ++l) {
-    const int n = P.fwd_off[l + 1] - P.fwd_off[l];
-    hipLaunchKernelGGL((ilu_fwd_kernel<BS>), dim3(ceil_div((long long)n * BS, AT)), dim3(AT), 0, s, n,
-                       f.fwd_rows.p + P.fwd_off[l], f.lptr.p, f.lpos.p, f.lcol.p, f.lu.p, x, y);
-  }
-  for (int l = 0; l < P.nlev_b; ++l) {
-    const int n = P.bwd_off[l + 1] - P.bwd_off[l];
-    hipLaunchKernelGGL((ilu_bwd_kernel<BS>), dim3(ceil_div((long long)n * BS, AT)), dim3(AT), 0, s, n,
-                       f.bwd_rows.p + P.bwd_off[l], f.uptr.p, f.upos.p, f.ucol.p, f.lu.p, f.dinv.p, y, z);
-  }
-  HFX_HIP(hipGetLastError());
+  if (f.low) sweeps_v<BS, float>(f, f.lu32.p, f.dinv32.p, x, y, z, s);
+  else sweeps_v<BS, double>(f, f.lu.p, f.dinv.p, x, y, z, s);
 }
 
 }  // namespace
@@ -222,6 +235,15 @@ void Ilu::factor(const BsrDev& A, hipStream_t s) {
   HFX_HIP(hipMemcpyAsync(&e, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HFX_HIP(hipStreamSynchronize(s));
   if (e) throw std::runtime_error("ILU(0): singular pivot block in block row " + std::to_string(e - 1));
+  if (low) {
+    lu32.alloc(lu.n);
+    dinv32.alloc(dinv.n);
+    hipLaunchKernelGGL(to_float_kernel, dim3(std::min<long long>(ceil_div((long long)lu.n, 256), 8192)), dim3(256), 0,
+                       s, lu.n, lu.p, lu32.p);
+    hipLaunchKernelGGL(to_float_kernel, dim3(std::min<long long>(ceil_div((long long)dinv.n, 256), 8192)), dim3(256),
+                       0, s, dinv.n, dinv.p, dinv32.p);
+    HFX_HIP(hipGetLastError());
+  }
 }
 
 void Ilu::apply(const double* x, double* y, hipStream_t s) {

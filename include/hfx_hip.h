@@ -105,7 +105,12 @@ enum {
   HFX_PC_BJACOBI = 2,
   HFX_PC_LEARNED = 3,
   HFX_PC_ILU0 = 4,
-  HFX_PC_ILU0_NATURAL = 5
+  HFX_PC_ILU0_NATURAL = 5,
+  /* HFX_PC_ILU0 (multicolour) factorised in FP64 and applied from FP32
+   * copies of the factors, summed in FP64: half the bytes per sweep.  The
+   * solve still stops on the FP64 true residual; only the iteration count
+   * can differ.  No counterpart in the reference (PCILU is FP64). */
+  HFX_PC_ILU0_F32 = 6
 };
 
 /* Create a problem on GPU `device` for simplices of `dim` and `order`

@@ -82,6 +82,38 @@ def test_apply_matches_oracle(case, pc):
     assert np.abs(y - z).max() <= 1e-11 * np.abs(z).max()
 
 
+def test_float_factor_apply_is_the_oracle_apply_on_float_factors(case):
+    """HFX_PC_ILU0_F32 reads the factors rounded to float and sums in double:
+    it equals the FP64 oracle's apply on those rounded factors up to the
+    summation order, and differs from the FP64 apply by the rounding only."""
+    prob = case['prob']
+    nb, b, rowptr, col, vals, _ = case['sys']
+    x = np.random.default_rng(13).standard_normal(nb * b)
+    y32 = prob.pc_apply(hf.PC_ILU0_F32, x)
+    y64 = prob.pc_apply(hf.PC_ILU0, x)
+    rank = io.ranks(nb, rowptr, col, 'multicolour')
+    F, Dinv = io.factor(nb, b, rowptr, col, vals, rank)
+    z = io.apply(nb, b, rowptr, col, F.astype(np.float32).astype(np.float64),
+                 Dinv.astype(np.float32).astype(np.float64), rank, x)
+    assert np.abs(y32 - z).max() <= 1e-11 * np.abs(z).max()
+    d = np.abs(y32 - y64).max() / np.abs(y64).max()
+    assert 0 < d < 1e-4, d
+
+
+@pytest.mark.parametrize('krylov', [hf.KRYLOV_FGMRES, hf.KRYLOV_BICGSTAB], ids=['fgmres', 'bicgstab'])
+def test_float_factor_solve_reaches_the_fp64_residual(krylov):
+    w, prob = assembled(2, 3, 16)
+    prob.set_krylov(krylov)
+    it64, r64, _ = prob.solve(pc=hf.PC_ILU0, rtol=1e-12, maxits=20000)
+    lam64 = prob.solution()[2].copy()
+    it32, r32, rel32 = prob.solve(pc=hf.PC_ILU0_F32, rtol=1e-12, maxits=20000)
+    lam32 = prob.solution()[2]
+    print('krylov %d: ILU(0) %d its, FP32 factors %d its' % (krylov, it64, it32))
+    assert r64 == 2 and r32 == 2 and rel32 <= 1e-12
+    assert np.abs(lam32 - lam64).max() <= 1e-9 * np.abs(lam64).max()
+    assert it32 <= 1.2 * it64 + 2
+
+
 @pytest.mark.parametrize('krylov', [hf.KRYLOV_FGMRES, hf.KRYLOV_BICGSTAB], ids=['fgmres', 'bicgstab'])
 @pytest.mark.parametrize('pc', [hf.PC_ILU0, hf.PC_ILU0_NATURAL], ids=['multicolour', 'natural'])
 def test_ilu_solve_matches_block_jacobi(krylov, pc):
