@@ -64,6 +64,7 @@ _SIGS = {
     'hfx_hdg_set_trace': ([_P, _P], _I),
     'hfx_hdg_set_boundary_model': ([_P, _I, _P, _I], _I),
     'hfx_hdg_set_boundary_hdg': ([_P, _P, _I, _P, _P, _I, _I], _I),
+    'hfx_hdg_set_solver_graphs': ([_P, _I], _I),
     'hfx_hdg_ilu_factors': ([_P, _I, _P, _P, _P], _I),
     'hfx_ilu_analyse': ([_I, _P, _P, _I, _P, _P, _P, _P, _P], _I),
     'hfx_hdg_get_solution': ([_P] * 4, _I),

@@ -593,6 +593,7 @@ void do_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, hfx::Kry
   o.rtol = rtol;
   o.maxits = maxits;
   o.restart = restart;
+  o.graph = h->solver_graphs;
   if (h->krylov == HFX_KRYLOV_BICGSTAB) hfx::bicgstab(h->A, h->rhs.p, h->lambda.p, M, o, info, s);
   else hfx::fgmres(h->A, h->rhs.p, h->lambda.p, M, o, info, s);
 }
@@ -871,6 +872,13 @@ extern "C" int hfx_hdg_set_fused_scatter(hfx_hdg* h, int on) {
   require(h != nullptr, "hfx_hdg_set_fused_scatter: null handle");
   h->fuse_scatter = on != 0;
   ++h->args_version;
+  HFX_END
+}
+
+extern "C" int hfx_hdg_set_solver_graphs(hfx_hdg* h, int on) {
+  HFX_TRY
+  require(h != nullptr, "hfx_hdg_set_solver_graphs: null handle");
+  h->solver_graphs = on != 0;
   HFX_END
 }
 

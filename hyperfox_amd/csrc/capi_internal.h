@@ -103,6 +103,7 @@ struct hfx_hdg {
   double last_pc_setup_ms = 0.0;
   std::unique_ptr<hfx::Ilu> ilu;  // ILU(0) schedule + factors for the current pattern
   int krylov = HFX_KRYLOV_FGMRES;
+  bool solver_graphs = false;  // hfx_hdg_set_solver_graphs
   // last solve options
   int last_pc = HFX_PC_BJACOBI, last_maxits = 10000, last_restart = 30;
   double last_rtol = 1e-10;

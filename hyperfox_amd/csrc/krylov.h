@@ -24,6 +24,12 @@ struct KrylovOpts {
   double rtol = 1e-10;   // on the true residual ||b - A x|| / ||b||
   int maxits = 10000;
   int restart = 30;
+  // BiCGStab: run the steady iterations between two residual checks as one
+  // captured HIP graph (the same kernels in the same order, so the same
+  // results), when a preconditioner is given.  Off by default: measured no
+  // faster on gfx950 (the launches are already asynchronous), and each solve
+  // pays the capture
+  bool graph = false;
 };
 
 struct KrylovInfo {

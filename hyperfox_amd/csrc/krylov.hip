@@ -568,9 +568,67 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
     HFX_HIP(hipGetLastError());
   };
   const int check_every = 8;
+  // one steady iteration (after the first of a restart): p, M p, A p^ with
+  // (rhat, v), alpha, s, M s, A s^ with (s, t) and (t, t), omega, x, r with
+  // (rhat, r) and (r, r)
+  auto steady = [&]() {
+    hipLaunchKernelGGL(bicg_p_kernel, dim3(nblk), dim3(256), 0, s, n, 0, sc.p, r.p, p.p, v.p);
+    HFX_HIP(hipGetLastError());
+    apply_M(p.p, ph.p);
+    spmv_dot(ph.p, v.p, rh.p, false);
+    scalars(1, 1);
+    hipLaunchKernelGGL(bicg_s_kernel, dim3(nblk), dim3(256), 0, s, n, sc.p, r.p, v.p, sv.p);
+    HFX_HIP(hipGetLastError());
+    apply_M(sv.p, sh.p);
+    spmv_dot(sh.p, tv.p, sv.p, true);
+    scalars(2, 2);
+    hipLaunchKernelGGL(bicg_xr_dot_kernel, dim3(nblk), dim3(RT), 0, s, n, sc.p, ph.p, sh.p, sv.p, tv.p, rh.p, x, r.p,
+                       fpart.p);
+    HFX_HIP(hipGetLastError());
+    scalars(2, 0);
+  };
+  // check_every steady iterations as one graph, captured at first use; it
+  // holds kernels only (no memset or copy nodes), and every buffer it names
+  // lives until the end of this call
+  struct GraphHold {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    ~GraphHold() {
+      if (e) (void)hipGraphExecDestroy(e);
+      if (g) (void)hipGraphDestroy(g);
+    }
+  } gh;
+  const bool use_graph = o.graph && static_cast<bool>(precond);
+  auto graph_block = [&]() {
+    if (!gh.e) {
+      HFX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        for (int k = 0; k < check_every; ++k) steady();
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HFX_HIP(hipStreamEndCapture(s, &gh.g));
+      HFX_HIP(hipGraphInstantiate(&gh.e, gh.g, nullptr, nullptr, 0));
+    }
+    HFX_HIP(hipGraphLaunch(gh.e, s));
+  };
   int it = 0;
   bool restart = true;
   while (true) {
+    if (!restart && use_graph && it % check_every == 0 && it + check_every <= o.maxits) {
+      graph_block();
+      it += check_every;
+      double hs[SC_N];
+      HFX_HIP(hipMemcpyAsync(hs, sc.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+      HFX_HIP(hipStreamSynchronize(s));
+      const double rr = std::sqrt(std::fabs(hs[SC_RR]));
+      if (!std::isfinite(hs[SC_RR])) { info.reason = DIVERGED_NANORINF; break; }
+      if (rr <= target || hs[SC_BRK] != 0.0 || it >= o.maxits) restart = true;
+      continue;
+    }
     if (restart) {
       // r = b - A x (true FP64 residual), rhat = r, rho = (r, r)
       bsr_spmv(A, x, r.p, b, s);
@@ -586,22 +644,22 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
       ++info.restarts;
       restart = false;
       hipLaunchKernelGGL(bicg_p_kernel, dim3(nblk), dim3(256), 0, s, n, 1, sc.p, r.p, p.p, v.p);
+      HFX_HIP(hipGetLastError());
+      apply_M(p.p, ph.p);
+      spmv_dot(ph.p, v.p, rh.p, false);          // v = A p^, (rhat, v)
+      scalars(1, 1);
+      hipLaunchKernelGGL(bicg_s_kernel, dim3(nblk), dim3(256), 0, s, n, sc.p, r.p, v.p, sv.p);
+      HFX_HIP(hipGetLastError());
+      apply_M(sv.p, sh.p);
+      spmv_dot(sh.p, tv.p, sv.p, true);          // t = A s^, (s, t), (t, t)
+      scalars(2, 2);
+      hipLaunchKernelGGL(bicg_xr_dot_kernel, dim3(nblk), dim3(RT), 0, s, n, sc.p, ph.p, sh.p, sv.p, tv.p, rh.p, x,
+                         r.p, fpart.p);
+      HFX_HIP(hipGetLastError());
+      scalars(2, 0);                             // (rhat, r), (r, r)
     } else {
-      hipLaunchKernelGGL(bicg_p_kernel, dim3(nblk), dim3(256), 0, s, n, 0, sc.p, r.p, p.p, v.p);
+      steady();
     }
-    HFX_HIP(hipGetLastError());
-    apply_M(p.p, ph.p);
-    spmv_dot(ph.p, v.p, rh.p, false);            // v = A p^, (rhat, v)
-    scalars(1, 1);
-    hipLaunchKernelGGL(bicg_s_kernel, dim3(nblk), dim3(256), 0, s, n, sc.p, r.p, v.p, sv.p);
-    HFX_HIP(hipGetLastError());
-    apply_M(sv.p, sh.p);
-    spmv_dot(sh.p, tv.p, sv.p, true);            // t = A s^, (s, t), (t, t)
-    scalars(2, 2);
-    hipLaunchKernelGGL(bicg_xr_dot_kernel, dim3(nblk), dim3(RT), 0, s, n, sc.p, ph.p, sh.p, sv.p, tv.p, rh.p, x, r.p,
-                       fpart.p);
-    HFX_HIP(hipGetLastError());
-    scalars(2, 0);                               // (rhat, r), (r, r)
     ++it;
     if (it % check_every == 0 || it >= o.maxits) {
       double hs[SC_N];

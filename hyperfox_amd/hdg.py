@@ -292,6 +292,11 @@ class HDGProblem:
         element kernel writes the trace matrix in place instead of storing S."""
         check(lib.hfx_hdg_set_fused_scatter(self._h, 1 if on else 0))
 
+    def set_solver_graphs(self, on):
+        """BiCGStab steady iterations as one captured HIP graph (default off:
+        no faster on MI355X)."""
+        check(lib.hfx_hdg_set_solver_graphs(self._h, 1 if on else 0))
+
     def set_krylov(self, method):
         """0 restarted FGMRES (default), 1 BiCGStab."""
         check(lib.hfx_hdg_set_krylov(self._h, method))

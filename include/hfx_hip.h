@@ -270,6 +270,12 @@ int hfx_hdg_set_fused_scatter(hfx_hdg* h, int on);
  * KSPGMRES) or BiCGStab with device-resident scalars (KSPBCGS). */
 enum { HFX_KRYLOV_FGMRES = 0, HFX_KRYLOV_BICGSTAB = 1 };
 int hfx_hdg_set_krylov(hfx_hdg* h, int method);
+/* BiCGStab with a preconditioner runs the steady iterations between two
+ * residual checks (8) as one captured HIP graph (on = 1) or as separate
+ * launches (on = 0, the default: graphs measured no faster on MI355X, and
+ * each solve pays the capture); the kernels and their order are the same, so
+ * are the results.  No counterpart in the reference. */
+int hfx_hdg_set_solver_graphs(hfx_hdg* h, int on);
 
 /* Trace solve (LinAlgebraInterface::solve, HDGSolver.cpp:699-708): FP64
  * Krylov method of hfx_hdg_set_krylov with preconditioner `pc` (restart

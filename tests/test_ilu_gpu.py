@@ -151,3 +151,23 @@ def test_point_ilu_behind_the_linalg_interface(pc):
     assert np.abs(x - ref).max() <= 1e-9 * np.abs(ref).max()
     its, reason, rel = la.last_info
     assert reason == 2 and rel <= 1e-12
+
+
+@pytest.mark.parametrize('pc', [hf.PC_JACOBI, hf.PC_BJACOBI, hf.PC_ILU0, hf.PC_ILU0_F32],
+                         ids=['jacobi', 'block-jacobi', 'ilu0', 'ilu0-f32'])
+def test_bicgstab_graph_replay_gives_the_launch_by_launch_result(pc):
+    """The captured graph replays the same kernels in the same order: the same
+    iteration count and the same solution bit for bit, over several replays
+    and a restart."""
+    w, prob = assembled(2, 3, 16)
+    prob.set_krylov(hf.KRYLOV_BICGSTAB)
+    prob.set_solver_graphs(False)
+    it0, r0, rel0 = prob.solve(pc=pc, rtol=1e-12, maxits=20000)
+    lam0 = prob.solution()[2].copy()
+    prob.set_solver_graphs(True)
+    for rep in range(2):
+        it1, r1, rel1 = prob.solve(pc=pc, rtol=1e-12, maxits=20000)
+        lam1 = prob.solution()[2]
+        assert (it1, r1) == (it0, r0) and rel1 == rel0, (rep, it0, it1)
+        assert np.array_equal(lam1, lam0), rep
+    assert r0 == 2 and it0 > 16
