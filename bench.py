@@ -127,6 +127,24 @@ def timed_solve(hf, prob, method, pc, maxits, warm=True):
             'true_relres': relres, 'ms': ms, 'pc_setup_ms': pc_setup_ms(prob)}
 
 
+def topology_times(hf, w, reps=3):
+    """Mesh topology (faces, cell2face, face2cell, boundary, face-node
+    permutations) of the benchmark mesh from its cells, on the host and on the
+    GPU (identical arrays), best of `reps`, wall ms including the cell copy."""
+    a = w.mesh.arrays()
+    out = {}
+    for name, dev in (('host_ms', None), ('gpu_ms', 0)):
+        best = 1e30
+        for _ in range(reps):
+            t = time.perf_counter()
+            m = hf.Mesh.from_cells(3, w.p, a['coords'], a['cells'], device=dev)
+            best = min(best, (time.perf_counter() - t) * 1e3)
+        out[name] = best
+        out['faces'] = m.n_faces
+    print('# topology: host %.1f ms, GPU %.1f ms' % (out['host_ms'], out['gpu_ms']), file=sys.stderr, flush=True)
+    return out
+
+
 def trace_solves(hf, pr, prob, w):
     """Trace solve of the benchmark problem to a true FP64 relative residual
     of 1e-10: BiCGStab with block Jacobi and block ILU(0) (multicolour order),
@@ -410,6 +428,7 @@ def main():
         res['transient'] = transient(hf, pr, prob, w)
         if not args.no_learned:
             res['learned_vs_plain'] = learned_vs_plain(hf, pr, args.learned_n)
+    res['topology'] = topology_times(hf, w)
     if not args.no_batch:
         res['batched'] = batched(hf, pr)
     if not args.no_cpu:
