@@ -399,7 +399,7 @@ void geometry(const Ref& R, const double* x, Geo& g) {
         for (int a = 0; a < fd; ++a)
           for (int b = 0; b < d; ++b) J[a * d + b] += R.fdphi[(q * R.nNf + i) * fd + a] * xi[b];
       }
-      double JJt[4];
+      double JJt[9];  // fd x fd with fd <= 2; det() is sized for 3 x 3
       for (int a = 0; a < fd; ++a)
         for (int c = 0; c < fd; ++c) {
           double s = 0;
@@ -816,7 +816,7 @@ void face_mass(const Ref& R, const double* xf, double* M) {
     for (int i = 0; i < nNf; ++i)
       for (int a = 0; a < fd; ++a)
         for (int b = 0; b < d; ++b) J[a * d + b] += R.fdphi[(q * nNf + i) * fd + a] * xf[i * d + b];
-    double JJt[4];
+    double JJt[9];  // fd x fd with fd <= 2; det() is sized for 3 x 3
     for (int a = 0; a < fd; ++a)
       for (int c = 0; c < fd; ++c) {
         double s = 0;
