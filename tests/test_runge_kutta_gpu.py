@@ -204,3 +204,70 @@ def test_time_scheme_arguments_are_checked():
         prob.assemble()
     prob.rk_compute_solution()
     assert prob.rk_stages() == (2, 0)
+
+
+@pytest.mark.parametrize('hdg_type', [1, 2], ids=['wexplicit', 'sexplicit'])
+@pytest.mark.parametrize('kernel', [1, 0])
+@pytest.mark.parametrize('dim,p', [(2, 3), (3, 2)])
+def test_runge_kutta_stage_with_the_explicit_hdg_modes(dim, p, kernel, hdg_type):
+    """The reference runs its Runge-Kutta schemes with the explicit HDG modes
+    (tests/parallel/TestParHDGConvectionDiffusionReactionSource.cpp:72-78,
+    109, 226): the u-row transform and S = Sll, S0 from the Solution and
+    Flux fields compose."""
+    w = small_workload(dim, p, 2)
+    prob, f = run_gpu(w)
+    prob.set_kernel(kernel)
+    u, q, lam = random_fields(w, 21)
+    set_fields(prob, u, q, lam)
+    prob.set_solver_type(hdg_type)
+    prob.set_time_scheme(hf.TS_RK, DT)
+    prob.set_butcher('ALX2')
+    prob.rk_start_step()
+    prob.assemble()
+    g = prob.condensed()
+    ut = oracle_system(w, f).local_vectors(u, q, lam)
+    t = np.asarray(orc.RK_TABLES['ALX2'])
+    kst = np.zeros((t.shape[0] - 1, ut.shape[1]))
+    for e in range(w.mesh.n_cells):
+        Ko, Fo = orc.apply_rk(dim, p, DT, t[0, 1:], 0, ut[e], kst, *oracle_local(w, f, e))
+        c = orc.condense(dim, p, Ko, Fo)
+        c['S'], c['S0'] = orc.explicit_blocks(dim, p, Ko, Fo, u[e], q[e])
+        check_condensed(g, c, e)
+
+
+@pytest.mark.parametrize('name', ['RK43', 'SSPRK3'])
+@pytest.mark.parametrize('dim,p', [(2, 3), (3, 2)])
+def test_runge_kutta_stage_on_curved_cells(dim, p, name):
+    from test_boundary_models_gpu import curved_workload
+    w = curved_workload(dim, p)
+    prob, f = run_gpu(w)
+    assert prob.kernel_info()['kernel'] == 'general'
+    u, q, lam = random_fields(w, 23)
+    set_fields(prob, u, q, lam)
+    prob.set_time_scheme(hf.TS_RK, DT)
+    prob.set_butcher(name)
+    prob.rk_start_step()
+    prob.assemble()
+    g = prob.condensed()
+    ut = oracle_system(w, f).local_vectors(u, q, lam)
+    t = np.asarray(orc.RK_TABLES[name])
+    kst = np.zeros((t.shape[0] - 1, ut.shape[1]))
+    for e in range(w.mesh.n_cells):
+        Ko, Fo = orc.apply_rk(dim, p, DT, t[0, 1:], 0, ut[e], kst, *oracle_local(w, f, e))
+        check_condensed(g, orc.condense(dim, p, Ko, Fo), e)
+
+
+def test_batches_take_implicit_euler_but_reject_the_general_transform():
+    import hyperfox_amd.problems as pr
+    w = pr.diffusion_source_2d(4, 2)          # straight cells, identity diffusion: batchable
+    prob, f = run_gpu(w)
+    b = hf.Batch()
+    b.add(prob)
+    prob.set_solution(random_fields(w, 3)[0])
+    prob.set_time_scheme(hf.TS_BEULER, DT)
+    b.assemble(use_graph=False)
+    prob.set_time_scheme(hf.TS_RK, DT)
+    prob.set_butcher('ALX2')
+    prob.rk_start_step()
+    with pytest.raises(hf.HfxError, match='implicit Euler only'):
+        b.assemble(use_graph=False)
