@@ -253,25 +253,49 @@ __global__ void hdg_boundary_rhs_kernel(int nhf, const int* __restrict__ hfaces,
   *dst = rset ? v : *dst + v;
 }
 
-__global__ void reconstruct_kernel(int nEl, int nN, int nQ, int nFpc, int bs, const double* __restrict__ U,
-                                   const double* __restrict__ U0, const double* __restrict__ Q,
-                                   const double* __restrict__ Q0, const int* __restrict__ cell2face,
-                                   const int8_t* __restrict__ face_perm, const double* __restrict__ lambda,
-                                   double* __restrict__ u, double* __restrict__ q) {
-  const int rows = nN + nQ;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)nEl * rows) return;
-  const int e = (int)(t / rows), i = (int)(t % rows);
-  const int NL = nFpc * bs;
-  const double* M = i < nN ? U + ((size_t)e * nN + i) * NL : Q + ((size_t)e * nQ + (i - nN)) * NL;
-  double acc = i < nN ? U0[(size_t)e * nN + i] : Q0[(size_t)e * nQ + (i - nN)];
-  for (int f = 0; f < nFpc; ++f) {
-    const int F = cell2face[(size_t)e * nFpc + f];
-    const int8_t* pp = face_perm + ((size_t)e * nFpc + f) * bs;
-    for (int j = 0; j < bs; ++j) acc += M[f * bs + j] * lambda[(size_t)F * bs + pp[j]];
+// u = U lambda_e + U0, q = Q lambda_e + Q0 for one cell per wavefront: the
+// cell's lambda is gathered into LDS, then up to 64 rows of its U and Q blocks
+// at a time are staged in LDS with coalesced loads (rows padded to NL + 1
+// doubles against bank conflicts) and lane r sums row r, from U0 / Q0 and over
+// the columns in order.
+__global__ __launch_bounds__(64) void reconstruct_kernel(int nEl, int nN, int nQ, int nFpc, int bs,
+                                                         const double* __restrict__ U, const double* __restrict__ U0,
+                                                         const double* __restrict__ Q, const double* __restrict__ Q0,
+                                                         const int* __restrict__ cell2face,
+                                                         const int8_t* __restrict__ face_perm,
+                                                         const double* __restrict__ lambda, double* __restrict__ u,
+                                                         double* __restrict__ q) {
+  extern __shared__ double rsm[];
+  const int NL = nFpc * bs, rows = nN + nQ, ld = NL + 1;
+  double* L = rsm;
+  double* T = rsm + NL;
+  const int lane = threadIdx.x;
+  for (int e = blockIdx.x; e < nEl; e += gridDim.x) {
+    __syncthreads();  // the previous cell is done with L and T
+    for (int t = lane; t < NL; t += 64) {
+      const int f = t / bs, j = t % bs;
+      L[t] = lambda[(size_t)cell2face[(size_t)e * nFpc + f] * bs + face_perm[((size_t)e * nFpc + f) * bs + j]];
+    }
+    const double* Ue = U + (size_t)e * nN * NL;
+    const double* Qe = Q + (size_t)e * nQ * NL;
+    for (int r0 = 0; r0 < rows; r0 += 64) {
+      const int nr = min(64, rows - r0);
+      if (r0 > 0) __syncthreads();  // the rows of the previous pass are consumed
+      for (int k = lane; k < nr * NL; k += 64) {
+        const int rr = r0 + k / NL, c = k % NL;
+        T[(k / NL) * ld + c] = rr < nN ? Ue[(size_t)rr * NL + c] : Qe[(size_t)(rr - nN) * NL + c];
+      }
+      __syncthreads();
+      if (lane < nr) {
+        const int r = r0 + lane;
+        double acc = r < nN ? U0[(size_t)e * nN + r] : Q0[(size_t)e * nQ + (r - nN)];
+        const double* row = T + lane * ld;
+        for (int c = 0; c < NL; ++c) acc += row[c] * L[c];
+        if (r < nN) u[(size_t)e * nN + r] = acc;
+        else q[(size_t)e * nQ + (r - nN)] = acc;
+      }
+    }
   }
-  if (i < nN) u[(size_t)e * nN + i] = acc;
-  else q[(size_t)e * nQ + (i - nN)] = acc;
 }
 
 }  // namespace
@@ -377,10 +401,13 @@ std::vector<int> cell_block_positions(const BsrPattern& P, int nCells, int nFpc,
 void launch_reconstruct(int nEl, int nN, int nQ, int nFpc, int bs, const double* U, const double* U0, const double* Q,
                         const double* Q0, const int* cell2face, const int8_t* face_perm, const double* lambda, double* u,
                         double* q, hipStream_t s) {
-  const long long n = (long long)nEl * (nN + nQ);
-  if (n <= 0) return;
-  hipLaunchKernelGGL(reconstruct_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, nEl, nN, nQ, nFpc, bs, U, U0, Q, Q0,
-                     cell2face, face_perm, lambda, u, q);
+  if (nEl <= 0) return;
+  const int NL = nFpc * bs;
+  const size_t lds = ((size_t)NL + 64 * ((size_t)NL + 1)) * sizeof(double);
+  if (lds > 64 * 1024) throw std::runtime_error("reconstruct: trace block too long");
+  const int grid = std::min(nEl, 65536);
+  hipLaunchKernelGGL(reconstruct_kernel, dim3(grid), dim3(64), lds, s, nEl, nN, nQ, nFpc, bs, U, U0, Q, Q0, cell2face,
+                     face_perm, lambda, u, q);
   HFX_HIP(hipGetLastError());
 }
 
