@@ -1,0 +1,190 @@
+"""CPU oracle of the Burgers path: HDGBurgersModel on HDGSolver (IMPLICIT)
+under NonLinearWrapper, with m = dim unknowns per node.
+
+TEST INFRASTRUCTURE ONLY: loaded by tests/, never by the product
+(hyperfox_amd).  The local systems come from liboracle_hdg.so
+(orc_burgers_local, HDGBurgersModel.cpp:86-132; orc_condense_sizes,
+HDGSolver.cpp:331-348); the trace system is assembled here with numpy and
+solved with a sparse direct solver:
+
+* local trace index (f nNf + j) m + a -> global (F nNf + p) m + a, p the
+  position of the cell's face node j in the face's node list
+  (HDGSolver.cpp:531-675);
+* boundary faces: IntegratedDirichletModel rows M (x) I_m and M g
+  (IntegratedDirichletModel.cpp:35-56, the Mass operator with m unknowns per
+  node) or DirichletModel rows I and g (DirichletModel.cpp);
+* u = U lambda_e + U0, q = Q lambda_e + Q0 (HDGSolver.cpp:741-775);
+* NonLinearWrapper::solve (NonLinearWrapper.cpp:41-79) with the vanilla
+  residual ||u - u_buf|| / ||u_buf|| (||u - u_buf|| when u_buf = 0) and the
+  damped update u = u_buf = (1 - d) u + d u_buf; the next linearisation
+  reads the last solved trace.
+"""
+import ctypes
+
+import numpy as np
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import _chk, _p, face_mass, lib, refel, sizes
+
+lib.orc_burgers_local.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 8
+lib.orc_burgers_local.restype = ctypes.c_int
+lib.orc_condense_sizes.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 8
+lib.orc_condense_sizes.restype = ctypes.c_int
+
+
+def _c(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def burgers_local(dim, p, x, Dn, tau, sol, trace, f_ip=None):
+    """(K, F) of one cell; see orc_burgers_local for the layouts."""
+    s = sizes(dim, p)
+    m = dim
+    n = s['nN'] * m * (dim + 1) + s['nF'] * s['nNf'] * m
+    K = np.zeros((n, n))
+    F = np.zeros(n)
+    args = [_c(v) for v in (x, Dn, tau, sol, trace, f_ip)]
+    _chk(lib.orc_burgers_local(dim, p, *[_p(v) for v in args], _p(K), _p(F)))
+    return K, F
+
+
+def condense_sizes(lu, lq, ll, K, F):
+    """(S, S0, U, Q, U0, Q0) of HDGSolver's static condensation."""
+    K = _c(K)
+    F = _c(F)
+    S, S0 = np.zeros((ll, ll)), np.zeros(ll)
+    U, Q, U0, Q0 = np.zeros((lu, ll)), np.zeros((lq, ll)), np.zeros(lu), np.zeros(lq)
+    _chk(lib.orc_condense_sizes(lu, lq, ll, _p(K), _p(F), _p(S), _p(S0), _p(U), _p(Q), _p(U0), _p(Q0)))
+    return S, S0, U, Q, U0, Q0
+
+
+def face_positions(dim, p, cells, faces, cell2face):
+    """pos[e, f, j]: position of cell e's local face f node j in the node list
+    of its face (the orientation HDGSolver reads from the mesh)."""
+    fmap = refel(dim, p)['face_nodes']
+    nC, nF = cell2face.shape
+    pos = np.zeros((nC, nF, fmap.shape[1]), dtype=np.int64)
+    for e in range(nC):
+        for f in range(nF):
+            row = list(faces[cell2face[e, f]])
+            for j, ln in enumerate(fmap[f]):
+                pos[e, f, j] = row.index(cells[e, ln])
+    return pos
+
+
+class BurgersSystem:
+    """HDGSolver over HDGBurgersModel on a whole mesh (numpy arrays of the
+    mesh: coords, cells, faces, cell2face, face2cell).
+
+    tau: (nFaces, nNf, sides, m, m) with tau[F, p, s][a, b] = tau_ab of the
+    face's node p on the side of cell face2cell[F, s]; Dn: nNodes scalars or
+    None; f_ip: (nCells, nIP, m) or None; dirichlet: (nFaces, nNf, m);
+    boundary: 'integrated' or 'dirichlet'."""
+
+    def __init__(self, dim, p, mesh, tau, Dn=None, f_ip=None, dirichlet=None, boundary='integrated'):
+        self.dim, self.p, self.m = dim, p, dim
+        self.s = sizes(dim, p)
+        self.mesh = {k: np.asarray(mesh[k]) for k in ('coords', 'cells', 'faces', 'cell2face', 'face2cell')}
+        M = self.mesh
+        self.nC, self.nFaces = M['cells'].shape[0], M['faces'].shape[0]
+        self.pos = face_positions(dim, p, M['cells'], M['faces'], M['cell2face'])
+        self.tau = np.asarray(tau, dtype=np.float64)
+        self.Dn = None if Dn is None else np.asarray(Dn, dtype=np.float64)
+        self.f_ip = None if f_ip is None else np.asarray(f_ip, dtype=np.float64)
+        nNf, m = self.s['nNf'], self.m
+        self.g = np.zeros((self.nFaces, nNf, m)) if dirichlet is None else np.asarray(dirichlet, dtype=np.float64)
+        self.bfaces = np.nonzero(M['face2cell'][:, 1] < 0)[0]
+        self.boundary = boundary
+        self.nL = self.s['nF'] * nNf * m
+        self.ndofs = self.nFaces * nNf * m
+        # global trace index of every local trace entry of every cell
+        f = np.arange(self.s['nF'])[None, :, None, None]
+        a = np.arange(m)[None, None, None, :]
+        F = M['cell2face'][:, :, None, None]
+        self.gidx = ((F * nNf + self.pos[:, :, :, None]) * m + a).reshape(self.nC, -1)
+        del f
+
+    def local_fields(self, e, sol, trace):
+        """Cell e's tau (side of e, local node order), BufferSolution and trace."""
+        M, s, m = self.mesh, self.s, self.m
+        nNf = s['nNf']
+        taus = np.zeros((s['nF'], nNf, m * m))
+        for f in range(s['nF']):
+            F = M['cell2face'][e, f]
+            side = 0 if (self.tau.shape[2] == 1 or M['face2cell'][F, 0] == e) else 1
+            for j in range(nNf):
+                taus[f, j] = self.tau[F, self.pos[e, f, j], side].T.reshape(-1)  # column-major m x m
+        lam = np.asarray(trace).reshape(-1)[self.gidx[e]]
+        return taus.reshape(-1), np.asarray(sol).reshape(self.nC, -1)[e], lam
+
+    def assemble(self, sol, trace):
+        """Condensed blocks of every cell and the global trace system."""
+        M, s, m, dim = self.mesh, self.s, self.m, self.dim
+        nN, nNf = s['nN'], s['nNf']
+        lu, lq, ll = nN * m, nN * dim * m, self.nL
+        self.U = np.zeros((self.nC, lu, ll))
+        self.Q = np.zeros((self.nC, lq, ll))
+        self.U0 = np.zeros((self.nC, lu))
+        self.Q0 = np.zeros((self.nC, lq))
+        self.S = np.zeros((self.nC, ll, ll))
+        self.S0 = np.zeros((self.nC, ll))
+        for e in range(self.nC):
+            x = M['coords'][M['cells'][e]]
+            taus, so, lam = self.local_fields(e, sol, trace)
+            Dn = None if self.Dn is None else self.Dn[M['cells'][e]]
+            fi = None if self.f_ip is None else self.f_ip[e]
+            K, F = burgers_local(dim, self.p, x, Dn, taus, so, lam, fi)
+            (self.S[e], self.S0[e], self.U[e], self.Q[e], self.U0[e], self.Q0[e]) = condense_sizes(lu, lq, ll, K, F)
+        rows = np.repeat(self.gidx, ll, axis=1).reshape(-1)
+        cols = np.tile(self.gidx, (1, ll)).reshape(-1)
+        A = sp.coo_matrix((self.S.reshape(-1), (rows, cols)), shape=(self.ndofs, self.ndofs)).tocsr()
+        b = np.zeros(self.ndofs)
+        np.add.at(b, self.gidx.reshape(-1), self.S0.reshape(-1))
+        # boundary rows
+        A = A.tolil()
+        bs = nNf * m
+        for F in self.bfaces:
+            r0 = F * bs
+            A[r0:r0 + bs, :] = 0.0
+            gF = self.g[F].reshape(-1)
+            if self.boundary == 'integrated':
+                Mf = face_mass(dim, self.p, M['coords'][M['faces'][F]])
+                blk = np.kron(Mf, np.eye(m))
+                A[r0:r0 + bs, r0:r0 + bs] = blk
+                b[r0:r0 + bs] = blk @ gF
+            else:
+                A[r0:r0 + bs, r0:r0 + bs] = np.eye(bs)
+                b[r0:r0 + bs] = gF
+        self.A = A.tocsr()
+        self.b = b
+        return self.A, self.b
+
+    def solve(self):
+        return spla.spsolve(self.A.tocsc(), self.b)
+
+    def reconstruct(self, lam):
+        L = np.asarray(lam).reshape(-1)[self.gidx]
+        u = np.einsum('eij,ej->ei', self.U, L) + self.U0
+        q = np.einsum('eij,ej->ei', self.Q, L) + self.Q0
+        return u, q
+
+    def newton(self, sol, buf, trace, max_iters=20, tol=1e-6, damp=0.0):
+        """NonLinearWrapper::solve: returns (sol, buf, flux, trace, iterations, residual)."""
+        sol = np.array(sol, dtype=np.float64).reshape(self.nC, -1)
+        buf = np.array(buf, dtype=np.float64).reshape(self.nC, -1)
+        trace = np.array(trace, dtype=np.float64).reshape(-1)
+        flux = None
+        res = np.inf
+        it = 0
+        for it in range(1, max_iters + 1):
+            self.assemble(buf, trace)
+            trace = self.solve()
+            sol, flux = self.reconstruct(trace)
+            dn, rn = np.sum((sol - buf) ** 2), np.sum(buf ** 2)
+            res = np.sqrt(dn / rn) if rn != 0 else np.sqrt(dn)
+            if res < tol:
+                break
+            sol = (1.0 - damp) * sol + damp * buf
+            buf = sol.copy()
+        return sol, buf, flux, trace, it, res

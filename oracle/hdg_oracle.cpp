@@ -583,10 +583,11 @@ void local_system(const Ref& R, orc_model mdl, const double* x, const double* D,
   }
 }
 
-// Static condensation (HDGSolver.cpp:331-348).  Row-major outputs.
-void condense(const Ref& R, const double* K, const double* F, double* S, double* S0, double* U, double* Q, double* U0,
-              double* Q0) {
-  const int n = R.n(), lu = R.lenU(), lq = R.lenQ(), ll = R.lenL();
+// Static condensation (HDGSolver.cpp:331-348) of a local system with u, q
+// and lambda blocks of lu, lq and ll rows.  Row-major outputs.
+void condense_sizes(int lu, int lq, int ll, const double* K, const double* F, double* S, double* S0, double* U,
+                    double* Q, double* U0, double* Q0) {
+  const int n = lu + lq + ll;
   const int ou = 0, oq = lu, ol = lu + lq;
   auto blk = [&](int r0, int nr, int c0, int nc) {
     vec o((size_t)nr * nc);
@@ -643,6 +644,11 @@ void condense(const Ref& R, const double* K, const double* F, double* S, double*
     for (int k = 0; k < lq; ++k) s -= K[(size_t)(ol + i) * n + oq + k] * Q0[k];
     S0[i] = s;
   }
+}
+
+void condense(const Ref& R, const double* K, const double* F, double* S, double* S0, double* U, double* Q, double* U0,
+              double* Q0) {
+  condense_sizes(R.lenU(), R.lenQ(), R.lenL(), K, F, S, S0, U, Q, U0, Q0);
 }
 
 // Explicit HDG modes (HDGSolver.cpp:346-353): S = Sll and
@@ -835,7 +841,191 @@ void check_rk(double dt, int nst, int stage) {
   if (nst < 1 || stage < 0 || stage >= nst) throw std::runtime_error("RungeKutta: stage out of range");
 }
 
+
+// HDGBurgersModel::computeLocalMatrix / computeLocalRHS
+// (HDGBurgersModel.cpp:86-132) with m = d unknowns per node: HDGBase with an
+// m x m tau per face node (HDGBase.cpp:18-32, 67-158), HDGDiffusion with a
+// scalar coefficient per node (HDGDiffusion.cpp:31-145, Dn null: no
+// diffusion operator, HDGBurgersModel.cpp:99-104), HDGUNabU
+// (HDGUNabU.cpp:86-190) and one Source per component (Source.cpp:5-48).
+// Local order: u (i m + a), q ((i d + k) m + a), lambda ((f nNf + j) m + a).
+// tau: (f nNf + j) m^2 + (b m + a) for tau_ab (column-major per node);
+// sol: i m + a (the "BufferSolution"); trace: lambda in the cell's local
+// face-node order; f_ip: q m + a (null: no source).
+void burgers_local(const Ref& R, const double* x, const double* Dn, const double* tau, const double* sol,
+                   const double* trace, const double* f_ip, double* K, double* F) {
+  const int d = R.d, m = d, nN = R.nN, nNf = R.nNf, nF = R.nF;
+  const int lu = nN * m, lq = nN * d * m, ll = nF * nNf * m, n = lu + lq + ll;
+  const int oq = lu, ol = lu + lq;
+  Geo g; geometry(R, x, g);
+  std::fill(K, K + (size_t)n * n, 0.0);
+  std::fill(F, F + n, 0.0);
+  auto at = [&](int r, int c) -> double& { return K[(size_t)r * n + c]; };
+  auto grad = [&](int q, int i, double* gr) {
+    const double* Ji = &g.Jinv[q * d * d];
+    for (int k = 0; k < d; ++k) {
+      double s = 0;
+      for (int a = 0; a < d; ++a) s += Ji[k * d + a] * R.dphi[(q * nN + i) * d + a];
+      gr[k] = s;
+    }
+  };
+  auto L = [&](int f, int j, int a) { return ol + (f * nNf + j) * m + a; };
+  // --- HDGBase
+  for (int f = 0; f < nF; ++f)
+    for (int q = 0; q < R.nipf; ++q) {
+      const int o = f * R.nipf + q;
+      const double dv = g.dV[R.nip + o];
+      const double* ps = &R.fphi[q * nNf];
+      double T[9] = {0};
+      for (int t = 0; t < m * m; ++t)
+        for (int j = 0; j < nNf; ++j) T[t] += tau[(f * nNf + j) * m * m + t] * ps[j];
+      for (int a = 0; a < nNf; ++a)
+        for (int c = 0; c < nNf; ++c) {
+          const int ia = R.fmap[f][a], ic = R.fmap[f][c];
+          const double w = dv * ps[a] * ps[c];
+          for (int na = 0; na < m; ++na)
+            for (int mb = 0; mb < m; ++mb) {
+              const double val = T[mb * m + na] * w;
+              at(L(f, a, na), L(f, c, mb)) -= val;
+              at(L(f, a, na), ic * m + mb) += val;
+              at(ia * m + na, ic * m + mb) += val;
+              at(ia * m + na, L(f, c, mb)) -= val;
+            }
+          for (int na = 0; na < m; ++na)
+            for (int k = 0; k < d; ++k) at(oq + (ia * d + k) * m + na, L(f, c, na)) -= w * g.nrm[o * d + k];
+        }
+    }
+  for (int q = 0; q < R.nip; ++q) {
+    const double dv = g.dV[q];
+    const double* ph = &R.phi[q * nN];
+    for (int i = 0; i < nN; ++i) {
+      double gr[3];
+      grad(q, i, gr);
+      for (int k = 0; k < d; ++k)
+        for (int a = 0; a < m; ++a)
+          for (int j = 0; j < nN; ++j) {
+            at(oq + (i * d + k) * m + a, j * m + a) += dv * gr[k] * ph[j];
+            at(oq + (i * d + k) * m + a, oq + (j * d + k) * m + a) += dv * ph[i] * ph[j];
+          }
+    }
+  }
+  // --- HDGUNabU, in its own matrix for the right-hand side
+  std::vector<double> op((size_t)n * n, 0.0);
+  auto opa = [&](int r, int c) -> double& { return op[(size_t)r * n + c]; };
+  for (int f = 0; f < nF; ++f)
+    for (int q = 0; q < R.nipf; ++q) {
+      const int o = f * R.nipf + q;
+      const double dv = g.dV[R.nip + o];
+      const double* ps = &R.fphi[q * nNf];
+      const double* nv = &g.nrm[o * d];
+      double lam[3] = {0}, us[3] = {0};
+      for (int j = 0; j < nNf; ++j)
+        for (int a = 0; a < m; ++a) {
+          lam[a] += ps[j] * trace[(f * nNf + j) * m + a];
+          us[a] += ps[j] * sol[R.fmap[f][j] * m + a];
+        }
+      double ln = 0;
+      for (int k = 0; k < d; ++k) ln += lam[k] * nv[k];
+      for (int i = 0; i < nNf; ++i)
+        for (int a = 0; a < m; ++a)
+          for (int j = 0; j < nNf; ++j) {
+            opa(L(f, i, a), L(f, j, a)) += ln * dv * ps[j] * ps[i];
+            for (int b = 0; b < m; ++b) opa(L(f, i, a), L(f, j, b)) += us[a] * ps[j] * ps[i] * nv[b] * dv;
+          }
+    }
+  for (int f = 0; f < nF; ++f)
+    for (int i = 0; i < nNf; ++i)
+      for (int a = 0; a < m; ++a)
+        for (int c = 0; c < ll; ++c) opa(R.fmap[f][i] * m + a, ol + c) += opa(L(f, i, a), ol + c);
+  for (int q = 0; q < R.nip; ++q) {
+    const double dv = g.dV[q];
+    const double* ph = &R.phi[q * nN];
+    std::vector<double> gm((size_t)nN * d);
+    for (int i = 0; i < nN; ++i) grad(q, i, &gm[(size_t)i * d]);
+    double gs[9] = {0}, uq[3] = {0};  // gs[k][a] = d_k u_a
+    for (int i = 0; i < nN; ++i)
+      for (int a = 0; a < m; ++a) {
+        uq[a] += ph[i] * sol[i * m + a];
+        for (int k = 0; k < d; ++k) gs[k * m + a] += gm[(size_t)i * d + k] * sol[i * m + a];
+      }
+    double div = 0;
+    for (int k = 0; k < d; ++k) div += gs[k * m + k];
+    for (int i = 0; i < nN; ++i) {
+      double ug = 0;
+      for (int k = 0; k < d; ++k) ug += uq[k] * gm[(size_t)i * d + k];
+      for (int a = 0; a < m; ++a)
+        for (int j = 0; j < nN; ++j) {
+          opa(i * m + a, j * m + a) -= (div * ph[i] + ug) * dv * ph[j];
+          for (int b = 0; b < m; ++b)
+            opa(i * m + a, j * m + b) -= uq[a] * (gm[(size_t)j * d + b] * ph[i] + gm[(size_t)i * d + b] * ph[j]) * dv;
+        }
+    }
+  }
+  // rhs = op [sol/2 ; trace/2] on the u and lambda rows (HDGUNabU.cpp:180-189)
+  for (int r = 0; r < n; ++r) {
+    if (r >= oq && r < ol) continue;
+    double s = 0;
+    for (int c = 0; c < lu; ++c) s += opa(r, c) * (sol[c] / 2.0);
+    for (int c = 0; c < ll; ++c) s += opa(r, ol + c) * (trace[c] / 2.0);
+    F[r] = s;
+  }
+  for (size_t t = 0; t < (size_t)n * n; ++t) K[t] += op[t];
+  // --- HDGDiffusion with D I at each node
+  if (Dn) {
+    for (int f = 0; f < nF; ++f)
+      for (int q = 0; q < R.nipf; ++q) {
+        const int o = f * R.nipf + q;
+        const double dv = g.dV[R.nip + o];
+        const double* ps = &R.fphi[q * nNf];
+        double Dq = 0;
+        for (int a = 0; a < nNf; ++a) Dq += ps[a] * Dn[R.fmap[f][a]];
+        for (int a = 0; a < nNf; ++a)
+          for (int c = 0; c < nNf; ++c)
+            for (int k = 0; k < d; ++k) {
+              const double val = Dq * g.nrm[o * d + k] * dv * ps[a] * ps[c];
+              for (int na = 0; na < m; ++na) {
+                at(L(f, a, na), oq + (R.fmap[f][c] * d + k) * m + na) -= val;
+                at(R.fmap[f][a] * m + na, oq + (R.fmap[f][c] * d + k) * m + na) -= val;
+              }
+            }
+      }
+    for (int q = 0; q < R.nip; ++q) {
+      const double dv = g.dV[q];
+      const double* ph = &R.phi[q * nN];
+      double Dq = 0;
+      for (int i = 0; i < nN; ++i) Dq += ph[i] * Dn[i];
+      for (int i = 0; i < nN; ++i) {
+        double gr[3];
+        grad(q, i, gr);
+        for (int a = 0; a < m; ++a)
+          for (int j = 0; j < nN; ++j)
+            for (int k = 0; k < d; ++k) at(i * m + a, oq + (j * d + k) * m + a) += Dq * gr[k] * dv * ph[j];
+      }
+    }
+  }
+  // --- Source_a on the u rows of component a (HDGBurgersModel.cpp:118-128)
+  if (f_ip)
+    for (int q = 0; q < R.nip; ++q)
+      for (int i = 0; i < nN; ++i)
+        for (int a = 0; a < m; ++a) F[i * m + a] += g.dV[q] * f_ip[q * m + a] * R.phi[q * nN + i];
+}
 }  // namespace
+
+extern "C" int orc_burgers_local(int dim, int p, const double* x, const double* Dn, const double* tau,
+                                 const double* sol, const double* trace, const double* f_ip, double* K, double* F) {
+  ORC_TRY
+  check_dim(dim, p);
+  burgers_local(ref(dim, p), x, Dn, tau, sol, trace, f_ip, K, F);
+  ORC_END
+}
+
+extern "C" int orc_condense_sizes(int lu, int lq, int ll, const double* K, const double* F, double* S, double* S0,
+                                  double* U, double* Q, double* U0, double* Q0) {
+  ORC_TRY
+  if (lu <= 0 || lq <= 0 || ll <= 0) throw std::runtime_error("condense: empty blocks");
+  condense_sizes(lu, lq, ll, K, F, S, S0, U, Q, U0, Q0);
+  ORC_END
+}
 
 extern "C" int orc_apply_euler_explicit(int dim, int p, double dt, const double* u_old, double* K, double* F) {
   ORC_TRY

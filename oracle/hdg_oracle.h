@@ -134,6 +134,19 @@ int orc_face_mass(int dim, int p, const double* xf, double* M);
  * q (nEl*nN*dim) and lambda (nFaces*nNf), the trace in each cell's face-node
  * order (Solver::prepareLocalFieldMap, Solver.cpp:14-57). */
 int orc_system_local_vectors(const orc_system* s, const double* u, const double* q, const double* lambda, double* out);
+/* HDGBurgersModel local system (HDGBurgersModel.cpp:86-132), m = dim
+ * unknowns per node, n = nN m (dim + 1) + nF nNf m rows in the order u
+ * (i m + a), q ((i dim + k) m + a), lambda ((f nNf + j) m + a).  Dn: nN
+ * scalar diffusion coefficients (NULL: no HDGDiffusion); tau: nF nNf m^2,
+ * tau_ab of each cell-local face node at (b m + a); sol: nN m (the
+ * BufferSolution); trace: nF nNf m in the cell's face-node order; f_ip:
+ * nIP m source values (NULL: none).  K n x n row-major, F n. */
+int orc_burgers_local(int dim, int p, const double* x, const double* Dn, const double* tau, const double* sol,
+                      const double* trace, const double* f_ip, double* K, double* F);
+/* Static condensation (HDGSolver.cpp:331-348) of a local system with lu, lq
+ * and ll rows of u, q and lambda. */
+int orc_condense_sizes(int lu, int lq, int ll, const double* K, const double* F, double* S, double* S0, double* U,
+                       double* Q, double* U0, double* Q0);
 /* sizes: out[0]=ndofs, out[1]=nnz */
 int orc_system_sizes(const orc_system* s, long long* out);
 int orc_system_csr(const orc_system* s, long long* rowptr, int* cols, double* vals, double* rhs);
