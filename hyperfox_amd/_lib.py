@@ -118,6 +118,20 @@ _SIGS = {
     'hfx_lpc_outputs': ([_P, _I, _I, _P, _P], _I),
     'hfx_hdg_set_lpc': ([_P, _P, _I], _I),
     'hfx_hdg_pc_setup_ms': ([_P, _P], _I),
+    'hfx_burgers_create': ([_I, _I, _I, _P], _I),
+    'hfx_burgers_destroy': ([_P], None),
+    'hfx_burgers_set_mesh': ([_P, _P], _I),
+    'hfx_burgers_set_fields': ([_P, _P, _I, _P, _P, _P], _I),
+    'hfx_burgers_set_boundary_model': ([_P, _I], _I),
+    'hfx_burgers_set_krylov': ([_P, _I], _I),
+    'hfx_burgers_set_state': ([_P, _P, _P, _P], _I),
+    'hfx_burgers_get_state': ([_P, _P, _P, _P, _P], _I),
+    'hfx_burgers_sizes': ([_P, _P], _I),
+    'hfx_burgers_assemble': ([_P], _I),
+    'hfx_burgers_condensed': ([_P] * 7, _I),
+    'hfx_burgers_trace_system': ([_P] * 5, _I),
+    'hfx_burgers_solve': ([_P, _I, _D, _I, _I, _P, _P, _P], _I),
+    'hfx_burgers_nonlinear_solve': ([_P, _I, _D, _D, _I, _D, _I, _I, _P, _P, _P], _I),
 }
 
 for _name, (_args, _res) in _SIGS.items():

@@ -1,0 +1,113 @@
+"""The Burgers path on the GPU: HDGBurgersModel (src/model/HDGBurgersModel.cpp)
+on HDGSolver's IMPLICIT assembly with m = dim unknowns per node, and
+NonLinearWrapper's loop (src/solver/NonLinearWrapper.cpp), through the
+hfx_burgers_* entry points of libhfx_hip.so.
+
+Field shapes follow the reference's Field values: tau (nFaces, nNf, sides,
+m, m) with tau[F, p, s][a, b] = tau_ab of the face's node p seen from the
+cell face2cell[F, s]; D (nNodes,) or None; dirichlet (nFaces, nNf, m);
+source values at the cell IPs (nCells, nIP, m); Solution and
+BufferSolution (nCells, nN, m); Flux (nCells, nN, dim, m); Trace
+(nFaces, nNf, m)."""
+import ctypes
+
+import numpy as np
+
+from ._lib import check, lib
+from .hdg import _f64, _ptr
+
+BC_DIRICHLET, BC_INTEGRATED_DIRICHLET = 0, 1
+
+
+class BurgersProblem:
+    """HDGSolver + HDGBurgersModel resident on one GPU."""
+
+    def __init__(self, dim, order, device=0):
+        h = ctypes.c_void_p()
+        check(lib.hfx_burgers_create(int(device), int(dim), int(order), ctypes.byref(h)))
+        self._h = h
+        self.dim, self.order, self.m = dim, order, dim
+        self.mesh = None
+
+    def __del__(self):
+        if getattr(self, '_h', None):
+            lib.hfx_burgers_destroy(self._h)
+            self._h = None
+
+    def set_mesh(self, mesh):
+        check(lib.hfx_burgers_set_mesh(self._h, mesh._h))
+        self.mesh = mesh
+        s = mesh.sizes
+        self.nN, self.nNf = s['nN'], s['nNf']
+        self.nIP = s['nIP']
+        out = np.zeros(5, dtype=np.int64)
+        check(lib.hfx_burgers_sizes(self._h, _ptr(out)))
+        self.nb, self.bs, self.nnzb, self.lu, self.ll = (int(v) for v in out)
+
+    def set_fields(self, tau, D=None, dirichlet=None, source=None):
+        """tau (nFaces, nNf, sides, m, m); the other fields as the module says."""
+        t = np.asarray(tau, dtype=np.float64)
+        if t.ndim != 5 or t.shape[3:] != (self.m, self.m) or t.shape[2] not in (1, 2):
+            raise ValueError('tau must be (nFaces, nNf, sides, m, m) with sides 1 or 2')
+        tcol = _f64(np.swapaxes(t, 3, 4))  # column-major m x m per node
+        Dd = None if D is None else _f64(D)
+        g = None if dirichlet is None else _f64(dirichlet)
+        f = None if source is None else _f64(source)
+        self._keep = (tcol, Dd, g, f)
+        check(lib.hfx_burgers_set_fields(self._h, _ptr(tcol), t.shape[2], _ptr(Dd), _ptr(g), _ptr(f)))
+
+    def set_boundary_model(self, kind):
+        check(lib.hfx_burgers_set_boundary_model(self._h, int(kind)))
+
+    def set_krylov(self, krylov):
+        check(lib.hfx_burgers_set_krylov(self._h, int(krylov)))
+
+    def set_state(self, sol=None, buffer=None, trace=None):
+        a = [None if v is None else _f64(v) for v in (sol, buffer, trace)]
+        check(lib.hfx_burgers_set_state(self._h, *[_ptr(v) for v in a]))
+
+    def state(self):
+        """(Solution, BufferSolution, Flux, Trace) copied to the host."""
+        nC, nF = self.mesh.n_cells, self.mesh.n_faces
+        sol = np.zeros((nC, self.nN, self.m))
+        buf = np.zeros_like(sol)
+        flux = np.zeros((nC, self.nN, self.dim, self.m))
+        tr = np.zeros((nF, self.nNf, self.m))
+        check(lib.hfx_burgers_get_state(self._h, _ptr(sol), _ptr(buf), _ptr(flux), _ptr(tr)))
+        return sol, buf, flux, tr
+
+    def assemble(self):
+        check(lib.hfx_burgers_assemble(self._h))
+
+    def condensed(self):
+        """(S, S0, U, Q, U0, Q0) of every cell, the trace in cell-local order."""
+        nC, lu, ll = self.mesh.n_cells, self.lu, self.ll
+        S, S0 = np.zeros((nC, ll, ll)), np.zeros((nC, ll))
+        U, Q = np.zeros((nC, lu, ll)), np.zeros((nC, lu * self.dim, ll))
+        U0, Q0 = np.zeros((nC, lu)), np.zeros((nC, lu * self.dim))
+        check(lib.hfx_burgers_condensed(self._h, *[_ptr(v) for v in (S, S0, U, Q, U0, Q0)]))
+        return S, S0, U, Q, U0, Q0
+
+    def trace_system(self):
+        """(rowptr, col, vals (nnzb, bs, bs), rhs) of the assembled trace system."""
+        rowptr = np.zeros(self.nb + 1, dtype=np.int32)
+        col = np.zeros(self.nnzb, dtype=np.int32)
+        vals = np.zeros((self.nnzb, self.bs, self.bs))
+        rhs = np.zeros(self.nb * self.bs)
+        check(lib.hfx_burgers_trace_system(self._h, _ptr(rowptr), _ptr(col), _ptr(vals), _ptr(rhs)))
+        return rowptr, col, vals, rhs
+
+    def solve(self, pc=2, rtol=1e-12, maxits=10000, restart=30):
+        """Trace solve + reconstruction; returns (iterations, reason, true relative residual)."""
+        it, reason, rr = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        check(lib.hfx_burgers_solve(self._h, int(pc), float(rtol), int(maxits), int(restart), ctypes.byref(it),
+                                    ctypes.byref(reason), ctypes.byref(rr)))
+        return it.value, reason.value, rr.value
+
+    def nonlinear_solve(self, max_iters=20, res_tol=1e-6, damp=0.0, pc=2, rtol=1e-12, maxits=10000, restart=30):
+        """NonLinearWrapper::solve; returns (iterations, residual, total linear iterations)."""
+        it, res, lin = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        check(lib.hfx_burgers_nonlinear_solve(self._h, int(max_iters), float(res_tol), float(damp), int(pc),
+                                              float(rtol), int(maxits), int(restart), ctypes.byref(it),
+                                              ctypes.byref(res), ctypes.byref(lin)))
+        return it.value, res.value, lin.value

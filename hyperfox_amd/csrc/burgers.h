@@ -1,0 +1,59 @@
+// Burgers cell kernel: the HDGBurgersModel local system (m = dim unknowns
+// per node) assembled and condensed onto the face traces on the GPU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hfx {
+
+// Arguments of one Burgers cell-kernel launch (device pointers).
+//   tabs       packed reference tables (ESz layout)
+//   tau        nFaces x nNf x sides x m^2 (tau_ab at b m + a, side s of the
+//              face's cell face2cell[2F + s])
+//   Dn         nNodes scalar diffusion coefficients, null: no HDGDiffusion
+//   sol        nEl x nN x m, the linearisation state ("BufferSolution")
+//   trace      nFaces x nNf x m, the linearisation trace ("Trace")
+//   f_ip       nEl x nIP x m source values at the cell IPs, null: none
+// Outputs, element-major and row-major, lambda in the cell's local
+// face-node order ((f nNf + j) m + a): S (mL x mL), S0 (mL), U (mN x mL),
+// Q (d mN x mL, row (i d + k) m + a), U0, Q0.
+struct BurgersArgs {
+  int nEl = 0;
+  int tau_sides = 1;
+  const double* tabs = nullptr;
+  const int* faceNodes = nullptr;
+  const double* coords = nullptr;
+  const int* cells = nullptr;
+  const int* cell2face = nullptr;
+  const int* face2cell = nullptr;
+  const int8_t* face_perm = nullptr;
+  const double* Dn = nullptr;
+  const double* tau = nullptr;
+  const double* sol = nullptr;
+  const double* trace = nullptr;
+  const double* f_ip = nullptr;
+  double *S = nullptr, *S0 = nullptr, *U = nullptr, *Q = nullptr, *U0 = nullptr, *Q0 = nullptr;
+};
+
+// True when (dim, order) has a Burgers cell kernel: 2-D orders 1..5 and
+// 3-D order 1 (m nN <= 64 rows for the workgroup Gauss-Jordan, trace blocks
+// of the solver's block sizes).
+bool burgers_supported(int dim, int order);
+
+// Launch over all a.nEl cells (one workgroup of 256 threads per cell,
+// grid-stride); throws on error.
+void launch_burgers(int dim, int order, const BurgersArgs& a, hipStream_t s);
+
+// m-fold face permutations: out[(e nF + f) nNf m + j m + a] = perm[...j] m + a.
+void launch_expand_perm(long long nblk, int nNf, int m, const int8_t* perm, int8_t* out, hipStream_t s);
+
+// NonLinearWrapper residual partials and update (NonLinearWrapper.cpp:12-79):
+// sums of (cur - prev)^2 and prev^2 into out2[0..1] (deterministic two-stage
+// reduction), and cur = prev = (1 - damp) cur + damp prev.
+void burgers_residual(long long n, const double* cur, const double* prev, double* partial, double* out2,
+                      hipStream_t s);
+void burgers_damp(long long n, double damp, double* cur, double* prev, hipStream_t s);
+
+}  // namespace hfx

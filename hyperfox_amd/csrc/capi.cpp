@@ -39,8 +39,9 @@ void set_last_error(const std::string& msg) { g_err = msg; }
   }                                   \
   return 0;
 
-namespace {
-
+namespace hfx {
+// Reference tables of the general element kernels (ESz's packed layout);
+// shared with the Burgers entry points (capi_burgers.cpp).
 void pack_tables(const hfx::RefEl& re, std::vector<double>& t) {
   const int d = re.dim;
   const hfx::RefEl& fe = *re.face;
@@ -53,6 +54,10 @@ void pack_tables(const hfx::RefEl& re, std::vector<double>& t) {
   t.insert(t.end(), fe.cub.w.begin(), fe.cub.w.end());
   (void)d;
 }
+}  // namespace hfx
+
+namespace {
+using hfx::pack_tables;
 
 // Reference tables of the affine element kernel (order of ALay in
 // hdg_affine.hip): PHI, DPHI, FPHI, W, FW, Ahat_a = Mhat^-1 Chat_a,

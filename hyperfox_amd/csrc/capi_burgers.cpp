@@ -1,0 +1,405 @@
+// C ABI of the Burgers path: HDGBurgersModel (m = dim unknowns per node) on
+// HDGSolver's IMPLICIT assembly, the trace solve and NonLinearWrapper's
+// loop, resident on one GPU.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hfx_hip.h"
+#include "burgers.h"
+#include "capi_internal.h"
+
+namespace hfx {
+void set_last_error(const std::string& msg);
+}
+
+#define HFX_TRY try {
+#define HFX_END                         \
+  }                                     \
+  catch (const std::exception& e) {     \
+    hfx::set_last_error(e.what());      \
+    return -1;                          \
+  }                                     \
+  return 0;
+
+struct hfx_burgers {
+  int device = 0, dim = 0, order = 0, m = 0;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<hfx::RefEl> re;
+  hfx::Mesh mesh;
+  bool has_mesh = false, has_fields = false, assembled = false;
+  int tau_sides = 1, bc_kind = HFX_BC_INTEGRATED_DIRICHLET, krylov = HFX_KRYLOV_FGMRES;
+  hfx::DevBuf<double> tabs, coords, Dn, tau, f_ip, g;
+  hfx::DevBuf<double> sol, buf, flux, trace;  // Solution, BufferSolution, Flux, Trace
+  hfx::DevBuf<double> S, S0, U, Q, U0, Q0, rhs, dinv, red;
+  hfx::DevBuf<int> faceNodes, cells, c2f, f2c, bfaces;
+  hfx::DevBuf<int8_t> perm, perm_m, invperm_m;
+  hfx::DevBuf<uint8_t> is_dir;
+  hfx::BsrPattern pat;
+  hfx::BsrDev A;
+  std::unique_ptr<hfx::Ilu> ilu;
+  void set_device() const { HFX_HIP(hipSetDevice(device)); }
+  int bs() const { return mesh.nNf * m; }
+  long long n_trace() const { return (long long)mesh.nFaces * bs(); }
+  long long n_sol() const { return (long long)mesh.nCells * mesh.nN * m; }
+};
+
+namespace {
+
+void need(bool c, const char* msg) {
+  if (!c) throw std::runtime_error(msg);
+}
+
+void assemble(hfx_burgers* h) {
+  need(h->has_mesh && h->has_fields, "hfx_burgers_assemble: set the mesh and the fields first");
+  const hfx::Mesh& m = h->mesh;
+  hipStream_t s = h->stream;
+  hfx::BurgersArgs a;
+  a.nEl = m.nCells;
+  a.tau_sides = h->tau_sides;
+  a.tabs = h->tabs.p;
+  a.faceNodes = h->faceNodes.p;
+  a.coords = h->coords.p;
+  a.cells = h->cells.p;
+  a.cell2face = h->c2f.p;
+  a.face2cell = h->f2c.p;
+  a.face_perm = h->perm.p;
+  a.Dn = h->Dn.p;
+  a.tau = h->tau.p;
+  a.sol = h->buf.p;
+  a.trace = h->trace.p;
+  a.f_ip = h->f_ip.p;
+  a.S = h->S.p; a.S0 = h->S0.p; a.U = h->U.p; a.Q = h->Q.p; a.U0 = h->U0.p; a.Q0 = h->Q0.p;
+  hfx::launch_burgers(h->dim, h->order, a, s);
+  // trace system: HDGSolver's scatter (Dirichlet rows identity / g), then the
+  // IntegratedDirichletModel rows M (x) I and (M (x) I) g
+  hfx::launch_scatter(h->A, m.nFpc, h->S.p, h->S0.p, h->invperm_m.p, h->is_dir.p, h->g.p, h->rhs.p, s);
+  if (h->bc_kind == HFX_BC_INTEGRATED_DIRICHLET && !m.boundary.empty()) {
+    const hfx::RefEl& re = *h->re;
+    const int nN = re.nN, nNf = re.nNf(), nip = re.nip(), nipf = re.face->nip(), d = h->dim;
+    const size_t t_fphi = (size_t)nip * nN * (1 + d), t_fdphi = t_fphi + (size_t)nipf * nNf;
+    const size_t t_fw = t_fdphi + (size_t)nipf * nNf * (d - 1) + nip;
+    hfx::launch_integrated_dirichlet(h->A, (int)m.boundary.size(), h->bfaces.p, d, nN, nipf, m.nFpc, h->faceNodes.p,
+                                     h->perm.p, h->cells.p, h->coords.p, h->tabs.p + t_fphi, h->tabs.p + t_fdphi,
+                                     h->tabs.p + t_fw, h->g.p, h->rhs.p, s, h->m);
+  }
+  h->assembled = true;
+}
+
+std::function<void(const double*, double*)> make_pc(hfx_burgers* h, int pc) {
+  const long long n = h->A.nrows();
+  hipStream_t s = h->stream;
+  if (pc == HFX_PC_NONE) return {};
+  if (pc == HFX_PC_JACOBI) {
+    h->dinv.alloc(n);
+    hfx::bsr_inv_point_diag(h->A, h->dinv.p, s);
+    return [h, n, s](const double* in, double* out) { hfx::apply_point_diag(n, h->dinv.p, in, out, s); };
+  }
+  if (pc == HFX_PC_BJACOBI) {
+    h->dinv.alloc((size_t)h->A.nb * h->A.bs * h->A.bs);
+    hfx::bsr_invert_diag(h->A, h->dinv.p, s);
+    return [h, s](const double* in, double* out) { hfx::apply_block_diag(h->A.nb, h->A.bs, h->dinv.p, in, out, s); };
+  }
+  if (pc == HFX_PC_ILU0 || pc == HFX_PC_ILU0_F32) {
+    if (!h->ilu) {
+      h->ilu.reset(new hfx::Ilu());
+      h->ilu->setup(h->A.nb, h->A.bs, h->pat.rowptr.data(), h->pat.col.data(), hfx::ILU_MULTICOLOUR, s);
+    }
+    h->ilu->low = pc == HFX_PC_ILU0_F32;
+    h->ilu->factor(h->A, s);
+    hfx::Ilu* f = h->ilu.get();
+    return [f, s](const double* in, double* out) { f->apply(in, out, s); };
+  }
+  throw std::runtime_error("hfx_burgers_solve: unsupported preconditioner");
+}
+
+void solve(hfx_burgers* h, int pc, double rtol, int maxits, int restart, hfx::KrylovInfo& info) {
+  need(h->assembled, "hfx_burgers_solve: assemble first");
+  hipStream_t s = h->stream;
+  const hfx::Mesh& m = h->mesh;
+  auto M = make_pc(h, pc);
+  HFX_HIP(hipMemsetAsync(h->trace.p, 0, h->n_trace() * sizeof(double), s));
+  hfx::KrylovOpts o;
+  o.rtol = rtol;
+  o.maxits = maxits;
+  o.restart = restart;
+  if (h->krylov == HFX_KRYLOV_BICGSTAB) hfx::bicgstab(h->A, h->rhs.p, h->trace.p, M, o, info, s);
+  else hfx::fgmres(h->A, h->rhs.p, h->trace.p, M, o, info, s);
+  const int mu = m.nN * h->m;
+  hfx::launch_reconstruct(m.nCells, mu, mu * h->dim, m.nFpc, h->bs(), h->U.p, h->U0.p, h->Q.p, h->Q0.p, h->c2f.p,
+                          h->perm_m.p, h->trace.p, h->sol.p, h->flux.p, s);
+  HFX_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" int hfx_burgers_create(int device, int dim, int order, hfx_burgers** out) {
+  HFX_TRY
+  need(out != nullptr, "hfx_burgers_create: null output");
+  int n = 0;
+  HFX_HIP(hipGetDeviceCount(&n));
+  need(device >= 0 && device < n, "hfx_burgers_create: no such GPU");
+  need(hfx::burgers_supported(dim, order),
+       "hfx_burgers_create: the Burgers kernel covers 2-D orders 1-5 and 3-D order 1");
+  auto h = std::unique_ptr<hfx_burgers>(new hfx_burgers());
+  h->device = device;
+  h->dim = dim;
+  h->order = order;
+  h->m = dim;
+  h->re.reset(new hfx::RefEl(dim, order));
+  h->set_device();
+  HFX_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  std::vector<double> t;
+  hfx::pack_tables(*h->re, t);
+  h->tabs.upload(t, h->stream);
+  std::vector<int> fnodes;
+  for (auto& f : h->re->faceNodes) fnodes.insert(fnodes.end(), f.begin(), f.end());
+  h->faceNodes.upload(fnodes, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  *out = h.release();
+  HFX_END
+}
+
+extern "C" void hfx_burgers_destroy(hfx_burgers* h) {
+  if (!h) return;
+  try {
+    h->set_device();
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+  } catch (...) {
+  }
+  hipStream_t s = h->stream;
+  delete h;
+  if (s) (void)hipStreamDestroy(s);
+}
+
+extern "C" int hfx_burgers_set_mesh(hfx_burgers* h, const hfx_mesh* mm) {
+  HFX_TRY
+  need(h && mm, "hfx_burgers_set_mesh: null handle");
+  need(mm->m.dim == h->dim && mm->re->order == h->order, "hfx_burgers_set_mesh: mesh dimension/order mismatch");
+  h->set_device();
+  h->mesh = mm->m;
+  h->ilu.reset();
+  h->has_fields = h->assembled = false;
+  const hfx::Mesh& m = h->mesh;
+  hipStream_t s = h->stream;
+  const int md = h->m, bs = m.nNf * md;
+  h->coords.upload(m.coords, s);
+  h->cells.upload(m.cells, s);
+  h->c2f.upload(m.cell2face, s);
+  h->f2c.upload(m.face2cell, s);
+  h->perm.upload(m.face_perm, s);
+  h->bfaces.upload(m.boundary, s);
+  std::vector<int8_t> inv(m.face_perm.size());
+  for (size_t blk = 0; blk < m.face_perm.size() / m.nNf; ++blk)
+    for (int j = 0; j < m.nNf; ++j) inv[blk * m.nNf + m.face_perm[blk * m.nNf + j]] = (int8_t)j;
+  hfx::DevBuf<int8_t> inv_d;
+  inv_d.upload(inv, s);
+  const long long nblk = (long long)m.nCells * m.nFpc;
+  h->perm_m.alloc((size_t)nblk * bs);
+  h->invperm_m.alloc((size_t)nblk * bs);
+  hfx::launch_expand_perm(nblk, m.nNf, md, h->perm.p, h->perm_m.p, s);
+  hfx::launch_expand_perm(nblk, m.nNf, md, inv_d.p, h->invperm_m.p, s);
+  h->pat = hfx::build_bsr_pattern(m.nCells, m.nFaces, m.nFpc, bs, m.cell2face.data(), m.face2cell.data());
+  h->A.nb = h->pat.nb;
+  h->A.bs = h->pat.bs;
+  h->A.nnzb = (int)h->pat.col.size();
+  h->A.rowptr.upload(h->pat.rowptr, s);
+  h->A.col.upload(h->pat.col, s);
+  h->A.rowf.upload(h->pat.rowf, s);
+  h->A.diag.upload(h->pat.diag, s);
+  h->A.contrib.upload(h->pat.contrib, s);
+  h->A.face_cells.upload(h->pat.face_cells, s);
+  h->A.vals.alloc((size_t)h->A.nnzb * bs * bs);
+  std::vector<uint8_t> dir(m.nFaces, 0);
+  for (int f : m.boundary) dir[f] = 1;
+  h->is_dir.upload(dir, s);
+  const size_t mu = (size_t)m.nN * md, ml = (size_t)m.nFpc * bs;
+  h->S.alloc(m.nCells * ml * ml);
+  h->S0.alloc(m.nCells * ml);
+  h->U.alloc(m.nCells * mu * ml);
+  h->Q.alloc(m.nCells * mu * h->dim * ml);
+  h->U0.alloc(m.nCells * mu);
+  h->Q0.alloc(m.nCells * mu * h->dim);
+  h->rhs.alloc(h->n_trace());
+  h->sol.alloc(h->n_sol());
+  h->buf.alloc(h->n_sol());
+  h->flux.alloc(h->n_sol() * h->dim);
+  h->trace.alloc(h->n_trace());
+  h->red.alloc(2 * 256 + 2);
+  HFX_HIP(hipMemsetAsync(h->sol.p, 0, h->sol.n * sizeof(double), s));
+  HFX_HIP(hipMemsetAsync(h->buf.p, 0, h->buf.n * sizeof(double), s));
+  HFX_HIP(hipMemsetAsync(h->flux.p, 0, h->flux.n * sizeof(double), s));
+  HFX_HIP(hipMemsetAsync(h->trace.p, 0, h->trace.n * sizeof(double), s));
+  HFX_HIP(hipStreamSynchronize(s));
+  h->has_mesh = true;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_set_fields(hfx_burgers* h, const double* tau, int tau_sides, const double* D,
+                                      const double* dirichlet, const double* f_ip) {
+  HFX_TRY
+  need(h && h->has_mesh, "hfx_burgers_set_fields: set the mesh first");
+  need(tau != nullptr, "hfx_burgers_set_fields: the Tau field is required");
+  need(tau_sides == 1 || tau_sides == 2,
+       "hfx_burgers_set_fields: the Tau field must have the same or twice the number of values per object as the "
+       "Solution field");
+  h->set_device();
+  const hfx::Mesh& m = h->mesh;
+  hipStream_t s = h->stream;
+  const size_t mm = (size_t)h->m * h->m;
+  h->tau_sides = tau_sides;
+  h->tau.upload(tau, (size_t)m.nFaces * m.nNf * tau_sides * mm, s);
+  if (D) h->Dn.upload(D, (size_t)m.nNodes, s);
+  else h->Dn.release();
+  h->g.alloc(h->n_trace());
+  if (dirichlet) HFX_HIP(hipMemcpyAsync(h->g.p, dirichlet, h->n_trace() * sizeof(double), hipMemcpyHostToDevice, s));
+  else HFX_HIP(hipMemsetAsync(h->g.p, 0, h->n_trace() * sizeof(double), s));
+  if (f_ip) h->f_ip.upload(f_ip, (size_t)m.nCells * h->re->nip() * h->m, s);
+  else h->f_ip.release();
+  HFX_HIP(hipStreamSynchronize(s));
+  h->has_fields = true;
+  h->assembled = false;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_set_boundary_model(hfx_burgers* h, int kind) {
+  HFX_TRY
+  need(h != nullptr, "hfx_burgers_set_boundary_model: null handle");
+  need(kind == HFX_BC_DIRICHLET || kind == HFX_BC_INTEGRATED_DIRICHLET,
+       "hfx_burgers_set_boundary_model: unknown boundary model");
+  h->bc_kind = kind;
+  h->assembled = false;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_set_krylov(hfx_burgers* h, int krylov) {
+  HFX_TRY
+  need(h != nullptr, "hfx_burgers_set_krylov: null handle");
+  need(krylov == HFX_KRYLOV_FGMRES || krylov == HFX_KRYLOV_BICGSTAB, "hfx_burgers_set_krylov: unknown method");
+  h->krylov = krylov;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_set_state(hfx_burgers* h, const double* sol, const double* buffer, const double* trace) {
+  HFX_TRY
+  need(h && h->has_mesh, "hfx_burgers_set_state: set the mesh first");
+  h->set_device();
+  hipStream_t s = h->stream;
+  if (sol) HFX_HIP(hipMemcpyAsync(h->sol.p, sol, h->sol.n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (buffer) HFX_HIP(hipMemcpyAsync(h->buf.p, buffer, h->buf.n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (trace) HFX_HIP(hipMemcpyAsync(h->trace.p, trace, h->trace.n * sizeof(double), hipMemcpyHostToDevice, s));
+  HFX_HIP(hipStreamSynchronize(s));
+  h->assembled = false;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_get_state(hfx_burgers* h, double* sol, double* buffer, double* flux, double* trace) {
+  HFX_TRY
+  need(h && h->has_mesh, "hfx_burgers_get_state: set the mesh first");
+  h->set_device();
+  hipStream_t s = h->stream;
+  if (sol) h->sol.download(sol, h->sol.n, s);
+  if (buffer) h->buf.download(buffer, h->buf.n, s);
+  if (flux) h->flux.download(flux, h->flux.n, s);
+  if (trace) h->trace.download(trace, h->trace.n, s);
+  HFX_HIP(hipStreamSynchronize(s));
+  HFX_END
+}
+
+extern "C" int hfx_burgers_sizes(const hfx_burgers* h, long long* out) {
+  HFX_TRY
+  need(h && h->has_mesh && out, "hfx_burgers_sizes: set the mesh first");
+  out[0] = h->A.nb;
+  out[1] = h->A.bs;
+  out[2] = h->A.nnzb;
+  out[3] = (long long)h->mesh.nN * h->m;                // u rows per cell
+  out[4] = (long long)h->mesh.nFpc * h->A.bs;           // trace entries per cell
+  HFX_END
+}
+
+extern "C" int hfx_burgers_assemble(hfx_burgers* h) {
+  HFX_TRY
+  need(h != nullptr, "hfx_burgers_assemble: null handle");
+  h->set_device();
+  assemble(h);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  HFX_END
+}
+
+extern "C" int hfx_burgers_condensed(hfx_burgers* h, double* S, double* S0, double* U, double* Q, double* U0,
+                                     double* Q0) {
+  HFX_TRY
+  need(h && h->assembled, "hfx_burgers_condensed: assemble first");
+  h->set_device();
+  hipStream_t s = h->stream;
+  if (S) h->S.download(S, h->S.n, s);
+  if (S0) h->S0.download(S0, h->S0.n, s);
+  if (U) h->U.download(U, h->U.n, s);
+  if (Q) h->Q.download(Q, h->Q.n, s);
+  if (U0) h->U0.download(U0, h->U0.n, s);
+  if (Q0) h->Q0.download(Q0, h->Q0.n, s);
+  HFX_HIP(hipStreamSynchronize(s));
+  HFX_END
+}
+
+extern "C" int hfx_burgers_trace_system(hfx_burgers* h, int* rowptr, int* col, double* vals, double* rhs) {
+  HFX_TRY
+  need(h && h->assembled, "hfx_burgers_trace_system: assemble first");
+  h->set_device();
+  if (rowptr) std::copy(h->pat.rowptr.begin(), h->pat.rowptr.end(), rowptr);
+  if (col) std::copy(h->pat.col.begin(), h->pat.col.end(), col);
+  if (vals) h->A.vals.download(vals, h->A.vals.n, h->stream);
+  if (rhs) h->rhs.download(rhs, h->rhs.n, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
+  HFX_END
+}
+
+extern "C" int hfx_burgers_solve(hfx_burgers* h, int pc, double rtol, int maxits, int restart, int* iters, int* reason,
+                                 double* relres) {
+  HFX_TRY
+  need(h != nullptr, "hfx_burgers_solve: null handle");
+  h->set_device();
+  hfx::KrylovInfo info;
+  solve(h, pc, rtol, maxits, restart, info);
+  if (iters) *iters = info.iters;
+  if (reason) *reason = info.reason;
+  if (relres) *relres = info.true_relres;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_nonlinear_solve(hfx_burgers* h, int max_iters, double res_tol, double damp, int pc,
+                                           double rtol, int maxits, int restart, int* iters, double* residual,
+                                           int* linear_iters) {
+  HFX_TRY
+  need(h != nullptr, "hfx_burgers_nonlinear_solve: null handle");
+  need(max_iters >= 1, "hfx_burgers_nonlinear_solve: at least one iteration");
+  h->set_device();
+  hipStream_t s = h->stream;
+  double res = 0.0;
+  int it = 0, lin = 0;
+  for (it = 1; it <= max_iters; ++it) {
+    assemble(h);
+    hfx::KrylovInfo info;
+    solve(h, pc, rtol, maxits, restart, info);
+    lin += info.iters;
+    if (info.reason < 0) throw std::runtime_error("hfx_burgers_nonlinear_solve: the linear solve did not converge");
+    hfx::burgers_residual(h->n_sol(), h->sol.p, h->buf.p, h->red.p, h->red.p + 512, s);
+    double sums[2];
+    HFX_HIP(hipMemcpyAsync(sums, h->red.p + 512, sizeof(sums), hipMemcpyDeviceToHost, s));
+    HFX_HIP(hipStreamSynchronize(s));
+    res = sums[1] != 0.0 ? std::sqrt(sums[0] / sums[1]) : std::sqrt(sums[0]);
+    if (res < res_tol) break;
+    hfx::burgers_damp(h->n_sol(), damp, h->sol.p, h->buf.p, s);
+  }
+  if (it > max_iters) it = max_iters;
+  HFX_HIP(hipStreamSynchronize(s));
+  if (iters) *iters = it;
+  if (residual) *residual = res;
+  if (linear_iters) *linear_iters = lin;
+  HFX_END
+}

@@ -32,6 +32,8 @@ struct LpcInfer {
 
 struct hfx_hdg;
 namespace hfx {
+// Packed reference tables of the general element kernels (ESz layout).
+void pack_tables(const RefEl& re, std::vector<double>& t);
 // Build the learned preconditioner matrix for the problem's current trace
 // matrix (network attached with hfx_hdg_set_lpc); defined in capi_lpc.cpp.
 void lpc_prepare(hfx_hdg* h);

@@ -254,7 +254,10 @@ void Ilu::apply(const double* x, double* y, hipStream_t s) {
     case 4: sweeps<4>(*this, x, tmp.p, y, s); break;
     case 5: sweeps<5>(*this, x, tmp.p, y, s); break;
     case 6: sweeps<6>(*this, x, tmp.p, y, s); break;
+    case 8: sweeps<8>(*this, x, tmp.p, y, s); break;
+    case 9: sweeps<9>(*this, x, tmp.p, y, s); break;
     case 10: sweeps<10>(*this, x, tmp.p, y, s); break;
+    case 12: sweeps<12>(*this, x, tmp.p, y, s); break;
     default: throw std::runtime_error("ILU(0): unsupported block size for the sweeps");
   }
 }

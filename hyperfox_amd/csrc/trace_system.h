@@ -52,12 +52,14 @@ void launch_scatter_diag(const BsrDev& A, int nFpc, const double* dside, const d
 
 // IntegratedDirichletModel on the boundary faces bfaces[nbf], after either
 // scatter: their diagonal blocks become the face mass of the face's nodes
-// (first cell's nodes through fnodes / face_perm; face tables fphi
-// [nipf][bs], fdphi [nipf][bs][dim-1], fw [nipf]) and their RHS rows M g.
+// (first cell's nodes through fnodes / face_perm, the node-level nF x nNf
+// permutation; face tables fphi [nipf][nNf], fdphi [nipf][nNf][dim-1],
+// fw [nipf]) and their RHS rows M g.  With m unknowns per node (block size
+// nNf m) the block is M (x) I_m (the Mass operator with m DOFs per node).
 void launch_integrated_dirichlet(const BsrDev& A, int nbf, const int* bfaces, int dim, int nN, int nipf, int nFpc,
                                  const int* fnodes, const int8_t* face_perm, const int* cells, const double* coords,
                                  const double* fphi, const double* fdphi, const double* fw, const double* g,
-                                 double* rhs, hipStream_t s);
+                                 double* rhs, hipStream_t s, int m = 1);
 
 // HDG-type boundary model on the boundary faces hfaces[nhf] (not flagged
 // Dirichlet, so their rows hold the cell's own S and S0), after either

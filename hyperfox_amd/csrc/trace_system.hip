@@ -146,7 +146,7 @@ __global__ void integrated_dirichlet_kernel(int nbf, const int* __restrict__ bfa
                                             const int* __restrict__ fnodes, const int8_t* __restrict__ face_perm,
                                             const int* __restrict__ cells, const double* __restrict__ coords,
                                             const double* __restrict__ fphi, const double* __restrict__ fdphi,
-                                            const double* __restrict__ fw, const int* __restrict__ diag,
+                                            const double* __restrict__ fw, const int* __restrict__ diag, int m,
                                             double* __restrict__ vals) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nbf * nNf * nNf) return;
@@ -155,7 +155,7 @@ __global__ void integrated_dirichlet_kernel(int nbf, const int* __restrict__ bfa
   const int e = face_cells[4 * (size_t)F], lf = face_cells[4 * (size_t)F + 1];
   const int* cn = cells + (size_t)e * nN;
   const int fd = dim - 1;
-  double m = 0.0;
+  double mass = 0.0;
   for (int q = 0; q < nipf; ++q) {
     double J[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // fd x dim
     for (int j = 0; j < nNf; ++j) {
@@ -174,10 +174,11 @@ __global__ void integrated_dirichlet_kernel(int nbf, const int* __restrict__ bfa
       const double g11 = J[3] * J[3] + J[4] * J[4] + J[5] * J[5];
       meas = sqrt(g00 * g11 - g01 * g01);
     }
-    m += fphi[q * nNf + a] * fphi[q * nNf + b] * (fw[q] * meas);
+    mass += fphi[q * nNf + a] * fphi[q * nNf + b] * (fw[q] * meas);
   }
   const int8_t* pp = face_perm + ((size_t)e * nFpc + lf) * nNf;
-  vals[(size_t)diag[F] * nNf * nNf + pp[a] * nNf + pp[b]] = m;
+  const int bs = nNf * m;
+  for (int c = 0; c < m; ++c) vals[(size_t)diag[F] * bs * bs + (pp[a] * m + c) * bs + pp[b] * m + c] = mass;
 }
 
 // rhs of those rows: M g (g in face-node order; null: zero)
@@ -311,7 +312,7 @@ void launch_scatter(const BsrDev& A, int nFpc, const double* S, const double* S0
     break;
   if (n > 0) {
     switch (A.bs) {
-      HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(10)
+      HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(12)
       default: throw std::runtime_error("scatter: unsupported block size");
     }
     HFX_HIP(hipGetLastError());
@@ -353,12 +354,13 @@ void launch_scatter_diag(const BsrDev& A, int nFpc, const double* dside, const d
 void launch_integrated_dirichlet(const BsrDev& A, int nbf, const int* bfaces, int dim, int nN, int nipf, int nFpc,
                                  const int* fnodes, const int8_t* face_perm, const int* cells, const double* coords,
                                  const double* fphi, const double* fdphi, const double* fw, const double* g,
-                                 double* rhs, hipStream_t s) {
+                                 double* rhs, hipStream_t s, int m) {
   if (nbf <= 0) return;
-  const int nt = 256, bs = A.bs;
-  const long long n = (long long)nbf * bs * bs;
-  hipLaunchKernelGGL(integrated_dirichlet_kernel, dim3(ceil_div(n, nt)), dim3(nt), 0, s, nbf, bfaces, dim, nN, bs,
-                     nipf, nFpc, A.face_cells.p, fnodes, face_perm, cells, coords, fphi, fdphi, fw, A.diag.p,
+  const int nt = 256, bs = A.bs, nNf = bs / m;
+  if (nNf * m != bs) throw std::runtime_error("integrated Dirichlet: block size is not a multiple of m");
+  const long long n = (long long)nbf * nNf * nNf;
+  hipLaunchKernelGGL(integrated_dirichlet_kernel, dim3(ceil_div(n, nt)), dim3(nt), 0, s, nbf, bfaces, dim, nN, nNf,
+                     nipf, nFpc, A.face_cells.p, fnodes, face_perm, cells, coords, fphi, fdphi, fw, A.diag.p, m,
                      A.vals.p);
   HFX_HIP(hipGetLastError());
   hipLaunchKernelGGL(integrated_dirichlet_rhs_kernel, dim3(ceil_div((long long)nbf * bs, nt)), dim3(nt), 0, s, nbf,

@@ -497,6 +497,61 @@ int hfx_hdg_set_lpc(hfx_hdg* h, hfx_lpc* lpc, int bf16);
  * pattern + factorisation; Jacobi: inverse diagonal). */
 int hfx_hdg_pc_setup_ms(hfx_hdg* h, double* ms);
 
+/* ------------------------------------------------------------- Burgers path
+ * HDGBurgersModel (src/model/HDGBurgersModel.cpp:34-132) on HDGSolver's
+ * IMPLICIT assembly with m = dim unknowns per node, the trace solve, and
+ * NonLinearWrapper (src/solver/NonLinearWrapper.cpp:41-79).  Fields, as the
+ * reference's Field values: Tau nFaces x nNf x sides x m^2 (tau_ab at
+ * b m + a, column-major per node; side s of the face's cell face2cell[2F+s]),
+ * DiffusionTensor nNodes scalars (NULL: no diffusion operator,
+ * HDGBurgersModel.cpp:99-104), Dirichlet nFaces x nNf x m, source values
+ * nCells x nIP x m at the cell IPs (NULL: no source; one Source per component,
+ * HDGBurgersModel.cpp:118-128).  State: Solution and BufferSolution
+ * nCells x nN x m, Flux nCells x nN x dim x m, Trace nFaces x nNf x m; the
+ * model linearises about BufferSolution and Trace.  2-D orders 1-5 and 3-D
+ * order 1. */
+typedef struct hfx_burgers hfx_burgers;
+int hfx_burgers_create(int device, int dim, int order, hfx_burgers** out);
+void hfx_burgers_destroy(hfx_burgers* h);
+/* HDGSolver::setMesh + allocate (HDGSolver.cpp:5-164, block size nNf m);
+ * the state fields start at zero. */
+int hfx_burgers_set_mesh(hfx_burgers* h, const hfx_mesh* m);
+/* HDGBurgersModel::setFieldMap (HDGBurgersModel.cpp:5-32) and the solver's
+ * field checks (HDGSolver.cpp:30-78). */
+int hfx_burgers_set_fields(hfx_burgers* h, const double* tau, int tau_sides, const double* D, const double* dirichlet,
+                           const double* f_ip);
+/* HFX_BC_DIRICHLET (DirichletModel) or HFX_BC_INTEGRATED_DIRICHLET
+ * (IntegratedDirichletModel, the default; rows (M (x) I_m) and (M (x) I_m) g)
+ * on every boundary face. */
+int hfx_burgers_set_boundary_model(hfx_burgers* h, int kind);
+int hfx_burgers_set_krylov(hfx_burgers* h, int krylov);
+/* Host copies in and out of the state fields (NULL: left as they are). */
+int hfx_burgers_set_state(hfx_burgers* h, const double* sol, const double* buffer, const double* trace);
+int hfx_burgers_get_state(hfx_burgers* h, double* sol, double* buffer, double* flux, double* trace);
+/* out[0..4] = block rows, block size, stored blocks, u rows per cell,
+ * trace entries per cell. */
+int hfx_burgers_sizes(const hfx_burgers* h, long long* out);
+/* HDGSolver::assemble (HDGSolver.cpp:166-675): the cell kernel (local
+ * system + condensation) and the trace system with the boundary rows. */
+int hfx_burgers_assemble(hfx_burgers* h);
+/* Condensed blocks of every cell, row-major, the trace in each cell's
+ * face-node order: S, S0, U, Q, U0, Q0 (any may be NULL). */
+int hfx_burgers_condensed(hfx_burgers* h, double* S, double* S0, double* U, double* Q, double* U0, double* Q0);
+/* The assembled BSR trace matrix (row-major blocks) and right-hand side. */
+int hfx_burgers_trace_system(hfx_burgers* h, int* rowptr, int* col, double* vals, double* rhs);
+/* HDGSolver::solve (HDGSolver.cpp:677-779): Krylov solve of the trace system
+ * from a zero guess into Trace (pc HFX_PC_NONE, _JACOBI, _BJACOBI, _ILU0 or
+ * _ILU0_F32), then Solution and Flux reconstructed. */
+int hfx_burgers_solve(hfx_burgers* h, int pc, double rtol, int maxits, int restart, int* iters, int* reason,
+                      double* relres);
+/* NonLinearWrapper::solve: up to max_iters of assemble + solve; residual
+ * ||Solution - BufferSolution|| / ||BufferSolution|| (the absolute norm when
+ * BufferSolution is zero); below res_tol it stops, else Solution =
+ * BufferSolution = (1 - damp) Solution + damp BufferSolution.  Fails when a
+ * linear solve does not converge. */
+int hfx_burgers_nonlinear_solve(hfx_burgers* h, int max_iters, double res_tol, double damp, int pc, double rtol,
+                                int maxits, int restart, int* iters, double* residual, int* linear_iters);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,166 @@
+"""The Burgers path on the GPU (HDGBurgersModel, HDGBurgersModel.cpp:34-132,
+with m = dim unknowns per node, on HDGSolver + NonLinearWrapper) against the
+Burgers oracle (pinned in test_burgers_oracle.py): the condensed blocks of
+every cell on straight and curved meshes with two-sided matrix tau, nodal
+diffusion, a linearisation state and a source; the trace system with both
+boundary models; the solve and reconstruction; the nonlinear loop on the
+reference's stationary regression case, undamped and damped; the checks."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import hyperfox_amd as hf
+from hyperfox_amd.burgers import BC_DIRICHLET, BC_INTEGRATED_DIRICHLET
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, '..', 'oracle'))
+import burgers_oracle as bo  # noqa: E402
+from test_burgers_oracle import cole_hopf, regression_case  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1)]
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+def workload(dim, p, seed=3, curved=False, n=2):
+    """A jittered (or curved) structured mesh with random fields and state."""
+    rng = np.random.default_rng(seed)
+    mesh = hf.Mesh.structured(dim, p, n, 0.15, seed)
+    if curved:
+        a = mesh.arrays()
+        x = a['coords'].copy()
+        ho = np.setdiff1d(np.arange(x.shape[0]), np.unique(a['cells'][:, :dim + 1]))
+        for k in range(dim):
+            x[ho, k] += 0.03 * np.sin(np.pi * x[ho, (k + 1) % dim]) * np.sin(np.pi * x[ho, k])
+        mesh = hf.Mesh.from_cells(dim, p, x, a['cells'])
+    s, m = mesh.sizes, dim
+    C = np.eye(m) + 0.2 * rng.standard_normal((m, m))
+    tau = np.einsum('fns,ab->fnsab', 1.0 + rng.random((mesh.n_faces, s['nNf'], 2)), C)
+    fields = dict(tau=tau, D=0.5 + rng.random(mesh.n_nodes),
+                  dirichlet=rng.standard_normal((mesh.n_faces, s['nNf'], m)),
+                  source=rng.standard_normal((mesh.n_cells, s['nIP'], m)))
+    state = dict(buffer=0.5 * rng.standard_normal((mesh.n_cells, s['nN'], m)),
+                 trace=0.5 * rng.standard_normal((mesh.n_faces, s['nNf'], m)))
+    return mesh, fields, state
+
+
+def gpu_problem(dim, p, mesh, fields, state, bc=BC_INTEGRATED_DIRICHLET):
+    prob = hf.BurgersProblem(dim, p)
+    prob.set_mesh(mesh)
+    prob.set_fields(**fields)
+    prob.set_boundary_model(bc)
+    prob.set_state(buffer=state['buffer'], trace=state['trace'])
+    return prob
+
+
+def oracle_problem(dim, p, mesh, fields, bc='integrated'):
+    return bo.BurgersSystem(dim, p, mesh.arrays(), fields['tau'], Dn=fields['D'],
+                            f_ip=fields['source'], dirichlet=fields['dirichlet'], boundary=bc)
+
+
+@pytest.mark.parametrize('curved', [False, True])
+@pytest.mark.parametrize('dim,p', CASES)
+def test_condensed_blocks_match_oracle(dim, p, curved):
+    print('burgers condensed', dim, p, curved, flush=True)
+    mesh, fields, state = workload(dim, p, curved=curved)
+    prob = gpu_problem(dim, p, mesh, fields, state)
+    prob.assemble()
+    got = prob.condensed()
+    orc = oracle_problem(dim, p, mesh, fields)
+    orc.assemble(state['buffer'].reshape(mesh.n_cells, -1), state['trace'])
+    for name, g, o in zip(('S', 'S0', 'U', 'Q', 'U0', 'Q0'), got, (orc.S, orc.S0, orc.U, orc.Q, orc.U0, orc.Q0)):
+        assert rel(g, o) < 1e-10, name
+
+
+@pytest.mark.parametrize('bc', [BC_INTEGRATED_DIRICHLET, BC_DIRICHLET])
+@pytest.mark.parametrize('dim,p', [(2, 2), (2, 4), (3, 1)])
+def test_trace_system_matches_oracle(dim, p, bc):
+    print('burgers trace system', dim, p, bc, flush=True)
+    mesh, fields, state = workload(dim, p)
+    prob = gpu_problem(dim, p, mesh, fields, state, bc)
+    prob.assemble()
+    rowptr, col, vals, rhs = prob.trace_system()
+    bs = prob.bs
+    A = np.zeros((prob.nb * bs, prob.nb * bs))
+    for F in range(prob.nb):
+        for k in range(rowptr[F], rowptr[F + 1]):
+            A[F * bs:(F + 1) * bs, col[k] * bs:(col[k] + 1) * bs] = vals[k]
+    orc = oracle_problem(dim, p, mesh, fields, 'integrated' if bc == BC_INTEGRATED_DIRICHLET else 'dirichlet')
+    Ao, bo_ = orc.assemble(state['buffer'].reshape(mesh.n_cells, -1), state['trace'])
+    assert rel(A, Ao.toarray()) < 1e-10
+    assert rel(rhs, bo_) < 1e-10
+
+
+@pytest.mark.parametrize('pc', [hf.PC_BJACOBI, hf.PC_ILU0, hf.PC_JACOBI])
+@pytest.mark.parametrize('dim,p', [(2, 3), (3, 1)])
+def test_solve_and_reconstruction_match_oracle(dim, p, pc):
+    print('burgers solve', dim, p, pc, flush=True)
+    mesh, fields, state = workload(dim, p)
+    prob = gpu_problem(dim, p, mesh, fields, state)
+    prob.assemble()
+    it, reason, rr = prob.solve(pc=pc, rtol=1e-13, maxits=5000)
+    assert reason > 0 and rr <= 1e-13
+    sol, buf, flux, tr = prob.state()
+    orc = oracle_problem(dim, p, mesh, fields)
+    orc.assemble(state['buffer'].reshape(mesh.n_cells, -1), state['trace'])
+    lam = orc.solve()
+    u, q = orc.reconstruct(lam)
+    assert rel(tr.reshape(-1), lam) < 1e-9
+    assert rel(sol.reshape(mesh.n_cells, -1), u) < 1e-9
+    assert rel(flux.reshape(mesh.n_cells, -1), q) < 1e-9
+    assert np.array_equal(buf, state['buffer'])
+
+
+@pytest.mark.parametrize('damp', [0.0, 0.3])
+@pytest.mark.parametrize('p', [1, 2, 3])
+def test_stationary_regression_case_matches_oracle(p, damp):
+    """TestHDGBurgersStat.cpp on regression_dim-2_h-2e-1: the reference's
+    settings (zero initial guess, at most 10 iterations, tolerance 1e-6;
+    GMRES + block Jacobi) with and without damping."""
+    print('burgers regression', p, damp, flush=True)
+    mesh_a, orc = regression_case('2e-1', p)
+    mesh = hf.Mesh.from_cells(2, p, mesh_a['coords'], mesh_a['cells'])
+    prob = hf.BurgersProblem(2, p)
+    prob.set_mesh(mesh)
+    prob.set_fields(orc.tau, D=orc.Dn, dirichlet=orc.g)
+    # damping makes the convergence linear (factor about damp per step): more steps
+    nmax = 10 if damp == 0.0 else 30
+    its, res, lin = prob.nonlinear_solve(max_iters=nmax, res_tol=1e-6, damp=damp, pc=hf.PC_BJACOBI, rtol=1e-13)
+    nC = mesh.n_cells
+    z0 = np.zeros((nC, orc.s['nN'] * 2))
+    u, buf, q, lam, its_o, res_o = orc.newton(z0, z0, np.zeros(orc.ndofs), max_iters=nmax, tol=1e-6, damp=damp)
+    sol, gbuf, flux, tr = prob.state()
+    assert its == its_o and res < 1e-6 and lin > 0
+    assert abs(res - res_o) <= 1e-3 * res_o
+    assert rel(sol.reshape(nC, -1), u) < 1e-8
+    assert rel(flux.reshape(nC, -1), q) < 1e-8
+    assert rel(tr.reshape(-1), lam) < 1e-8
+    ana = cole_hopf(mesh_a['coords'][mesh_a['cells'].reshape(-1)]).reshape(nC, -1)
+    err = np.sqrt(np.sum((ana - sol.reshape(nC, -1)) ** 2) / np.sum(ana ** 2))
+    err_o = np.sqrt(np.sum((ana - u) ** 2) / np.sum(ana ** 2))
+    assert abs(err - err_o) <= 1e-6 * err_o + 1e-12
+
+
+def test_checks():
+    with pytest.raises(hf.HfxError, match='2-D orders 1-5 and 3-D order 1'):
+        hf.BurgersProblem(3, 2)
+    prob = hf.BurgersProblem(2, 1)
+    with pytest.raises(hf.HfxError, match='set the mesh and the fields first'):
+        prob.assemble()
+    mesh, fields, state = workload(2, 1)
+    prob.set_mesh(mesh)
+    with pytest.raises(hf.HfxError, match='assemble first'):
+        prob.solve()
+    t1 = fields['tau'][:, :, :1]
+    prob.set_fields(t1, D=None)  # one-sided tau, no diffusion operator: accepted
+    with pytest.raises(hf.HfxError, match='unknown boundary model'):
+        prob.set_boundary_model(7)
+    with pytest.raises(hf.HfxError, match='unsupported preconditioner'):
+        prob.assemble()
+        prob.solve(pc=hf.PC_LEARNED)
