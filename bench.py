@@ -327,6 +327,71 @@ def batched(hf, pr, count=64, seed=7):
     return out
 
 
+def cole_hopf(X, D=1.0, A=0.5, x0=(1.0, 0.0)):
+    """Stationary viscous Burgers solution u = -2 D grad(pot) / pot of the
+    reference's regression case (A = 0.5, x0 = (1, 0))."""
+    x, y = X[:, 0] - x0[0], X[:, 1] - x0[1]
+    ep, em = np.exp(A * x), np.exp(-A * x)
+    pot = 0.001 * A * np.exp((1 + x0[0]) * A) * (1 + X[:, 0]) + (ep + em) * np.cos(A * y)
+    gx = 0.001 * A * np.exp((1 + x0[0]) * A) + A * (ep - em) * np.cos(A * y)
+    gy = -A * (ep + em) * np.sin(A * y)
+    return -2.0 * D * np.stack([gx, gy], 1) / pot[:, None]
+
+
+def burgers_flops(p):
+    """Dominant FP64 flops of one Burgers cell (2-D, m = 2): the two
+    Gauss-Jordan eliminations and the U, Q, S output products."""
+    nN, nNf = (p + 1) * (p + 2) // 2, p + 1
+    NL = 3 * nNf
+    MU, ML, NC1 = 2 * nN, 2 * NL, nN + 2 * nN + 2 * NL
+    gj1 = 2.0 * nN * nN * NC1
+    gj2 = 2.0 * MU * MU * (MU + ML + 1)
+    outs = 2.0 * 2 * MU * (ML + 1) * nN + 2.0 * ML * (ML + 1) * MU
+    return gj1 + gj2 + outs
+
+
+def burgers(hf, n=128, p=3, reps=5):
+    """The Burgers path on the reference's stationary regression case scaled
+    up: 2-D viscous Burgers (m = 2) on [0, 1]^2 with the Cole-Hopf solution
+    as integrated Dirichlet data, D = 1, tau = 2 D / h I, zero initial
+    guess, Newton to 1e-6 (BiCGStab + ILU(0) to 1e-10)."""
+    mesh = hf.Mesh.structured(2, p, n, 0.0, 0)
+    a = mesh.arrays()
+    s = mesh.sizes
+    h = 1.0 / n
+    tau = np.zeros((mesh.n_faces, s['nNf'], 2, 2, 2))
+    tau[:] = np.eye(2) * (2.0 / h)
+    g = cole_hopf(a['coords'][a['faces'].reshape(-1)]).reshape(mesh.n_faces, s['nNf'], 2)
+    prob = hf.BurgersProblem(2, p)
+    prob.set_mesh(mesh)
+    prob.set_fields(tau, D=np.ones(mesh.n_nodes), dirichlet=g)
+    prob.set_krylov(1)
+    prob.assemble()  # warm-up of the cell kernel and scatter
+    t = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        prob.assemble()
+        t.append(1e3 * (time.perf_counter() - t0))
+    asm_ms = float(np.median(t))
+    z = np.zeros((mesh.n_cells, s['nN'], 2))
+    prob.set_state(sol=z, buffer=z, trace=np.zeros((mesh.n_faces, s['nNf'], 2)))
+    t0 = time.perf_counter()
+    its, res, lin = prob.nonlinear_solve(max_iters=10, res_tol=1e-6, pc=4, rtol=1e-10, maxits=20000)
+    nl_ms = 1e3 * (time.perf_counter() - t0)
+    sol = prob.state()[0].reshape(mesh.n_cells, -1)
+    ana = cole_hopf(a['coords'][a['cells'].reshape(-1)]).reshape(mesh.n_cells, -1)
+    dofs = mesh.n_cells * s['nN'] * 2
+    out = {'workload': '2-D viscous Burgers p=%d, %d^2x2=%d triangles, Cole-Hopf data' % (p, n, mesh.n_cells),
+           'cells': mesh.n_cells, 'u_dofs': dofs, 'trace_dofs': mesh.n_faces * s['nNf'] * 2,
+           'assemble_ms': asm_ms, 'assemble_dof_per_s': dofs / (asm_ms * 1e-3),
+           'cell_flops': burgers_flops(p), 'newton_iters': its, 'residual': res, 'linear_iters': lin,
+           'nonlinear_solve_ms': nl_ms,
+           'rel_err_vs_exact': float(np.linalg.norm(sol - ana) / np.linalg.norm(ana))}
+    print('# burgers: assemble %.2f ms, %d Newton iterations (%d linear) in %.1f ms, error %.2e' % (
+        asm_ms, its, lin, nl_ms, out['rel_err_vs_exact']), file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration(mib):
     """Copy `mib` MiB device-to-device 5 times with torch (one plain copy
     kernel per call): the PMC reading of FETCH_SIZE / WRITE_SIZE for these
@@ -358,6 +423,7 @@ def main():
     ap.add_argument('--no-learned', action='store_true')
     ap.add_argument('--no-batch', action='store_true')
     ap.add_argument('--learned-n', type=int, default=256)
+    ap.add_argument('--no-burgers', action='store_true')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on exactly one GPU (multi-GPU sharding is designed, not built); '
@@ -429,6 +495,8 @@ def main():
         if not args.no_learned:
             res['learned_vs_plain'] = learned_vs_plain(hf, pr, args.learned_n)
     res['topology'] = topology_times(hf, w)
+    if not args.no_burgers:
+        res['burgers'] = burgers(hf)
     if not args.no_batch:
         res['batched'] = batched(hf, pr)
     if not args.no_cpu:

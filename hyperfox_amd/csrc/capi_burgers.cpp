@@ -387,7 +387,11 @@ extern "C" int hfx_burgers_nonlinear_solve(hfx_burgers* h, int max_iters, double
     hfx::KrylovInfo info;
     solve(h, pc, rtol, maxits, restart, info);
     lin += info.iters;
-    if (info.reason < 0) throw std::runtime_error("hfx_burgers_nonlinear_solve: the linear solve did not converge");
+    if (info.reason < 0)
+      throw std::runtime_error("hfx_burgers_nonlinear_solve: the linear solve of iteration " + std::to_string(it) +
+                               " did not converge (reason " + std::to_string(info.reason) + " after " +
+                               std::to_string(info.iters) + " iterations, relative residual " +
+                               std::to_string(info.true_relres) + ")");
     hfx::burgers_residual(h->n_sol(), h->sol.p, h->buf.p, h->red.p, h->red.p + 512, s);
     double sums[2];
     HFX_HIP(hipMemcpyAsync(sums, h->red.p + 512, sizeof(sums), hipMemcpyDeviceToHost, s));
