@@ -38,8 +38,8 @@ struct BurgersArgs {
 };
 
 // True when (dim, order) has a Burgers cell kernel: 2-D orders 1..5 and
-// 3-D order 1 (m nN <= 64 rows for the workgroup Gauss-Jordan, trace blocks
-// of the solver's block sizes).
+// 3-D orders 1..2 (m nN <= 64 rows for the workgroup Gauss-Jordan, the LDS
+// footprint within 150 KB, trace blocks of the solver's block sizes).
 bool burgers_supported(int dim, int order);
 
 // Launch over all a.nEl cells (one workgroup of 256 threads per cell,

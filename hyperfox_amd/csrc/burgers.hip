@@ -553,7 +553,7 @@ __global__ void damp_kernel(long long n, double damp, double* __restrict__ cur, 
 }  // namespace
 
 bool burgers_supported(int dim, int order) {
-  return (dim == 2 && order >= 1 && order <= 5) || (dim == 3 && order == 1);
+  return (dim == 2 && order >= 1 && order <= 5) || (dim == 3 && order >= 1 && order <= 2);
 }
 
 void launch_burgers(int dim, int order, const BurgersArgs& a, hipStream_t s) {
@@ -566,9 +566,11 @@ void launch_burgers(int dim, int order, const BurgersArgs& a, hipStream_t s) {
       case 4: launch_dp<2, 4>(a, s); return;
       case 5: launch_dp<2, 5>(a, s); return;
     }
-  } else if (dim == 3 && order == 1) {
-    launch_dp<3, 1>(a, s);
-    return;
+  } else if (dim == 3) {
+    switch (order) {
+      case 1: launch_dp<3, 1>(a, s); return;
+      case 2: launch_dp<3, 2>(a, s); return;
+    }
   }
   throw std::runtime_error("Burgers kernel: unsupported dimension/order");
 }

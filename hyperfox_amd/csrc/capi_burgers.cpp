@@ -106,9 +106,10 @@ std::function<void(const double*, double*)> make_pc(hfx_burgers* h, int pc) {
     return [h, s](const double* in, double* out) { hfx::apply_block_diag(h->A.nb, h->A.bs, h->dinv.p, in, out, s); };
   }
   if (pc == HFX_PC_ILU0 || pc == HFX_PC_ILU0_F32) {
-    if (!h->ilu) {
-      h->ilu.reset(new hfx::Ilu());
-      h->ilu->setup(h->A.nb, h->A.bs, h->pat.rowptr.data(), h->pat.col.data(), hfx::ILU_MULTICOLOUR, s);
+    if (!h->ilu) {  // kept only once its set-up succeeded
+      std::unique_ptr<hfx::Ilu> f(new hfx::Ilu());
+      f->setup(h->A.nb, h->A.bs, h->pat.rowptr.data(), h->pat.col.data(), hfx::ILU_MULTICOLOUR, s);
+      h->ilu = std::move(f);
     }
     h->ilu->low = pc == HFX_PC_ILU0_F32;
     h->ilu->factor(h->A, s);
@@ -145,7 +146,7 @@ extern "C" int hfx_burgers_create(int device, int dim, int order, hfx_burgers** 
   HFX_HIP(hipGetDeviceCount(&n));
   need(device >= 0 && device < n, "hfx_burgers_create: no such GPU");
   need(hfx::burgers_supported(dim, order),
-       "hfx_burgers_create: the Burgers kernel covers 2-D orders 1-5 and 3-D order 1");
+       "hfx_burgers_create: the Burgers kernel covers 2-D orders 1-5 and 3-D orders 1-2");
   auto h = std::unique_ptr<hfx_burgers>(new hfx_burgers());
   h->device = device;
   h->dim = dim;

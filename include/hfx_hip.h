@@ -509,7 +509,7 @@ int hfx_hdg_pc_setup_ms(hfx_hdg* h, double* ms);
  * HDGBurgersModel.cpp:118-128).  State: Solution and BufferSolution
  * nCells x nN x m, Flux nCells x nN x dim x m, Trace nFaces x nNf x m; the
  * model linearises about BufferSolution and Trace.  2-D orders 1-5 and 3-D
- * order 1. */
+ * orders 1-2. */
 typedef struct hfx_burgers hfx_burgers;
 int hfx_burgers_create(int device, int dim, int order, hfx_burgers** out);
 void hfx_burgers_destroy(hfx_burgers* h);

@@ -21,7 +21,7 @@ from test_burgers_oracle import cole_hopf, regression_case  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1)]
+CASES = [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2)]
 
 
 def rel(a, b):
@@ -79,7 +79,7 @@ def test_condensed_blocks_match_oracle(dim, p, curved):
 
 
 @pytest.mark.parametrize('bc', [BC_INTEGRATED_DIRICHLET, BC_DIRICHLET])
-@pytest.mark.parametrize('dim,p', [(2, 2), (2, 4), (3, 1)])
+@pytest.mark.parametrize('dim,p', [(2, 2), (2, 4), (3, 1), (3, 2)])
 def test_trace_system_matches_oracle(dim, p, bc):
     print('burgers trace system', dim, p, bc, flush=True)
     mesh, fields, state = workload(dim, p)
@@ -147,9 +147,30 @@ def test_stationary_regression_case_matches_oracle(p, damp):
     assert abs(err - err_o) <= 1e-6 * err_o + 1e-12
 
 
+@pytest.mark.parametrize('pc', [hf.PC_BJACOBI, hf.PC_JACOBI])
+def test_3d_p2_solve_matches_oracle(pc):
+    """Block size 18 (3-D p = 2, m = 3): the Jacobi preconditioners and the
+    Krylov kernels at that size; ILU(0) stops at block size 16 with an error."""
+    print('burgers solve 3-D p=2', pc, flush=True)
+    mesh, fields, state = workload(3, 2)
+    prob = gpu_problem(3, 2, mesh, fields, state)
+    prob.assemble()
+    it, reason, rr = prob.solve(pc=pc, rtol=1e-13, maxits=5000)
+    assert reason > 0 and rr <= 1e-13
+    sol, buf, flux, tr = prob.state()
+    orc = oracle_problem(3, 2, mesh, fields)
+    orc.assemble(state['buffer'].reshape(mesh.n_cells, -1), state['trace'])
+    lam = orc.solve()
+    u, q = orc.reconstruct(lam)
+    assert rel(tr.reshape(-1), lam) < 1e-9 and rel(sol.reshape(mesh.n_cells, -1), u) < 1e-9
+    assert rel(flux.reshape(mesh.n_cells, -1), q) < 1e-9
+    with pytest.raises(hf.HfxError, match='block size must be 1..16'):
+        prob.solve(pc=hf.PC_ILU0)
+
+
 def test_checks():
-    with pytest.raises(hf.HfxError, match='2-D orders 1-5 and 3-D order 1'):
-        hf.BurgersProblem(3, 2)
+    with pytest.raises(hf.HfxError, match='2-D orders 1-5 and 3-D orders 1-2'):
+        hf.BurgersProblem(3, 3)
     prob = hf.BurgersProblem(2, 1)
     with pytest.raises(hf.HfxError, match='set the mesh and the fields first'):
         prob.assemble()
