@@ -1,9 +1,10 @@
-"""CPU oracle of the Burgers path: HDGBurgersModel on HDGSolver (IMPLICIT)
-under NonLinearWrapper, with m = dim unknowns per node.
+"""CPU oracle of the multi-DOF nonlinear path: HDGBurgersModel (m = dim) and
+HDGnGammaModel (2-D, m = 2) on HDGSolver (IMPLICIT) under NonLinearWrapper.
 
 TEST INFRASTRUCTURE ONLY: loaded by tests/, never by the product
 (hyperfox_amd).  The local systems come from liboracle_hdg.so
-(orc_burgers_local, HDGBurgersModel.cpp:86-132; orc_condense_sizes,
+(orc_burgers_local, HDGBurgersModel.cpp:86-132; orc_ngamma_local,
+HDGnGammaModel.cpp:105-170; orc_condense_sizes,
 HDGSolver.cpp:331-348); the trace system is assembled here with numpy and
 solved with a sparse direct solver:
 
@@ -29,6 +30,8 @@ from oracle import _chk, _p, face_mass, lib, refel, sizes
 
 lib.orc_burgers_local.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 8
 lib.orc_burgers_local.restype = ctypes.c_int
+lib.orc_ngamma_local.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 6 + [ctypes.c_double] + [ctypes.c_void_p] * 3
+lib.orc_ngamma_local.restype = ctypes.c_int
 lib.orc_condense_sizes.argtypes = [ctypes.c_int] * 3 + [ctypes.c_void_p] * 8
 lib.orc_condense_sizes.restype = ctypes.c_int
 
@@ -46,6 +49,18 @@ def burgers_local(dim, p, x, Dn, tau, sol, trace, f_ip=None):
     F = np.zeros(n)
     args = [_c(v) for v in (x, Dn, tau, sol, trace, f_ip)]
     _chk(lib.orc_burgers_local(dim, p, *[_p(v) for v in args], _p(K), _p(F)))
+    return K, F
+
+
+def ngamma_local(p, x, D, G, b, sol, tau, cs, f_ip=None):
+    """(K, F) of one 2-D cell of HDGnGammaModel; see orc_ngamma_local."""
+    s = sizes(2, p)
+    n = 2 * (s['nN'] * 3 + s['nF'] * s['nNf'])
+    K = np.zeros((n, n))
+    F = np.zeros(n)
+    args = [_c(v) for v in (x, D, G, b, sol, tau)]
+    fi = _c(f_ip)
+    _chk(lib.orc_ngamma_local(p, *[_p(v) for v in args], float(cs), _p(fi), _p(K), _p(F)))
     return K, F
 
 
@@ -80,10 +95,13 @@ class BurgersSystem:
     tau: (nFaces, nNf, sides, m, m) with tau[F, p, s][a, b] = tau_ab of the
     face's node p on the side of cell face2cell[F, s]; Dn: nNodes scalars or
     None; f_ip: (nCells, nIP, m) or None; dirichlet: (nFaces, nNf, m);
-    boundary: 'integrated' or 'dirichlet'."""
+    boundary: 'integrated' or 'dirichlet'.  ngamma: a dict with the
+    HDGnGammaModel fields G (nNodes scalars or None), b (nNodes, 2) and cs
+    (sound speed); Dn is then its D field."""
 
-    def __init__(self, dim, p, mesh, tau, Dn=None, f_ip=None, dirichlet=None, boundary='integrated'):
+    def __init__(self, dim, p, mesh, tau, Dn=None, f_ip=None, dirichlet=None, boundary='integrated', ngamma=None):
         self.dim, self.p, self.m = dim, p, dim
+        self.ngamma = ngamma
         self.s = sizes(dim, p)
         self.mesh = {k: np.asarray(mesh[k]) for k in ('coords', 'cells', 'faces', 'cell2face', 'face2cell')}
         M = self.mesh
@@ -134,7 +152,12 @@ class BurgersSystem:
             taus, so, lam = self.local_fields(e, sol, trace)
             Dn = None if self.Dn is None else self.Dn[M['cells'][e]]
             fi = None if self.f_ip is None else self.f_ip[e]
-            K, F = burgers_local(dim, self.p, x, Dn, taus, so, lam, fi)
+            if self.ngamma is None:
+                K, F = burgers_local(dim, self.p, x, Dn, taus, so, lam, fi)
+            else:
+                ng = self.ngamma
+                Gn = None if ng.get('G') is None else np.asarray(ng['G'])[M['cells'][e]]
+                K, F = ngamma_local(self.p, x, Dn, Gn, np.asarray(ng['b'])[M['cells'][e]], so, taus, ng['cs'], fi)
             (self.S[e], self.S0[e], self.U[e], self.Q[e], self.U0[e], self.Q0[e]) = condense_sizes(lu, lq, ll, K, F)
         rows = np.repeat(self.gidx, ll, axis=1).reshape(-1)
         cols = np.tile(self.gidx, (1, ll)).reshape(-1)

@@ -842,6 +842,183 @@ void check_rk(double dt, int nst, int stage) {
 }
 
 
+// HDGBase (HDGBase.cpp:18-32, 67-158) with m unknowns per node and an m x m
+// tau per face node (tau_ab at (f nNf + j) m^2 + b m + a), added into K
+// (order u (i m + a), q ((i d + k) m + a), lambda ((f nNf + j) m + a)).
+void base_m(const Ref& R, const Geo& g, const double* tau, int m, double* K) {
+  const int d = R.d, nN = R.nN, nNf = R.nNf, nF = R.nF;
+  const int lu = nN * m, lq = nN * d * m, ll = nF * nNf * m, n = lu + lq + ll;
+  const int oq = lu, ol = lu + lq;
+  auto at = [&](int r, int c) -> double& { return K[(size_t)r * n + c]; };
+  auto L = [&](int f, int j, int a) { return ol + (f * nNf + j) * m + a; };
+  for (int f = 0; f < nF; ++f)
+    for (int q = 0; q < R.nipf; ++q) {
+      const int o = f * R.nipf + q;
+      const double dv = g.dV[R.nip + o];
+      const double* ps = &R.fphi[q * nNf];
+      double T[9] = {0};
+      for (int t = 0; t < m * m; ++t)
+        for (int j = 0; j < nNf; ++j) T[t] += tau[(f * nNf + j) * m * m + t] * ps[j];
+      for (int a = 0; a < nNf; ++a)
+        for (int c = 0; c < nNf; ++c) {
+          const int ia = R.fmap[f][a], ic = R.fmap[f][c];
+          const double w = dv * ps[a] * ps[c];
+          for (int na = 0; na < m; ++na)
+            for (int mb = 0; mb < m; ++mb) {
+              const double val = T[mb * m + na] * w;
+              at(L(f, a, na), L(f, c, mb)) -= val;
+              at(L(f, a, na), ic * m + mb) += val;
+              at(ia * m + na, ic * m + mb) += val;
+              at(ia * m + na, L(f, c, mb)) -= val;
+            }
+          for (int na = 0; na < m; ++na)
+            for (int k = 0; k < d; ++k) at(oq + (ia * d + k) * m + na, L(f, c, na)) -= w * g.nrm[o * d + k];
+        }
+    }
+  for (int q = 0; q < R.nip; ++q) {
+    const double dv = g.dV[q];
+    const double* ph = &R.phi[q * nN];
+    const double* Ji = &g.Jinv[q * d * d];
+    for (int i = 0; i < nN; ++i) {
+      double gr[3];
+      for (int k = 0; k < d; ++k) {
+        double s = 0;
+        for (int a = 0; a < d; ++a) s += Ji[k * d + a] * R.dphi[(q * nN + i) * d + a];
+        gr[k] = s;
+      }
+      for (int k = 0; k < d; ++k)
+        for (int a = 0; a < m; ++a)
+          for (int j = 0; j < nN; ++j) {
+            at(oq + (i * d + k) * m + a, j * m + a) += dv * gr[k] * ph[j];
+            at(oq + (i * d + k) * m + a, oq + (j * d + k) * m + a) += dv * ph[i] * ph[j];
+          }
+    }
+  }
+}
+
+// Scalar HDGConvection (HDGConvection.cpp:31-104 + Convection.cpp:5-49, face
+// Mass) with the nodal velocity v (nN x d), added into the scalar local
+// matrix K1 (n = R.n(), the m = 1 order): Suu -= C^T, Sul and Sll += (v.n)
+// face mass.
+void convection_op(const Ref& R, const Geo& g, const double* v, double* K1) {
+  const int d = R.d, nN = R.nN, nNf = R.nNf, nF = R.nF, n = R.n(), ol = nN * (d + 1);
+  auto at = [&](int r, int c) -> double& { return K1[(size_t)r * n + c]; };
+  for (int q = 0; q < R.nip; ++q) {
+    const double dv = g.dV[q];
+    const double* ph = &R.phi[q * nN];
+    const double* Ji = &g.Jinv[q * d * d];
+    double vq[3] = {0};
+    for (int i = 0; i < nN; ++i) for (int k = 0; k < d; ++k) vq[k] += ph[i] * v[i * d + k];
+    for (int l = 0; l < nN; ++l) {
+      double s = 0;
+      for (int k = 0; k < d; ++k) {
+        double gr = 0;
+        for (int a = 0; a < d; ++a) gr += Ji[k * d + a] * R.dphi[(q * nN + l) * d + a];
+        s += vq[k] * gr;
+      }
+      for (int kk = 0; kk < nN; ++kk) at(l, kk) -= dv * s * ph[kk];
+    }
+  }
+  for (int f = 0; f < nF; ++f)
+    for (int q = 0; q < R.nipf; ++q) {
+      const int o = f * R.nipf + q;
+      const double* ps = &R.fphi[q * nNf];
+      double vq[3] = {0};
+      for (int a = 0; a < nNf; ++a) for (int k = 0; k < d; ++k) vq[k] += ps[a] * v[R.fmap[f][a] * d + k];
+      double vn = 0;
+      for (int k = 0; k < d; ++k) vn += vq[k] * g.nrm[o * d + k];
+      const double w = g.dV[R.nip + o] * vn;
+      for (int a = 0; a < nNf; ++a)
+        for (int c = 0; c < nNf; ++c) {
+          const double val = w * ps[a] * ps[c];
+          at(R.fmap[f][a], ol + f * nNf + c) += val;
+          at(ol + f * nNf + a, ol + f * nNf + c) += val;
+        }
+    }
+}
+
+// Scalar HDGDiffusion (HDGDiffusion.cpp:31-145) with the coefficient Dn I at
+// each node, added into the scalar local matrix K1.
+void diffusion_op(const Ref& R, const Geo& g, const double* Dn, double* K1) {
+  const int d = R.d, nN = R.nN, nNf = R.nNf, nF = R.nF, n = R.n(), oq = nN, ol = nN * (d + 1);
+  auto at = [&](int r, int c) -> double& { return K1[(size_t)r * n + c]; };
+  for (int f = 0; f < nF; ++f)
+    for (int q = 0; q < R.nipf; ++q) {
+      const int o = f * R.nipf + q;
+      const double dv = g.dV[R.nip + o];
+      const double* ps = &R.fphi[q * nNf];
+      double Dq = 0;
+      for (int a = 0; a < nNf; ++a) Dq += ps[a] * Dn[R.fmap[f][a]];
+      for (int a = 0; a < nNf; ++a)
+        for (int c = 0; c < nNf; ++c)
+          for (int k = 0; k < d; ++k) {
+            const double val = Dq * g.nrm[o * d + k] * dv * ps[a] * ps[c];
+            at(ol + f * nNf + a, oq + R.fmap[f][c] * d + k) -= val;
+            at(R.fmap[f][a], oq + R.fmap[f][c] * d + k) -= val;
+          }
+    }
+  for (int q = 0; q < R.nip; ++q) {
+    const double dv = g.dV[q];
+    const double* ph = &R.phi[q * nN];
+    const double* Ji = &g.Jinv[q * d * d];
+    double Dq = 0;
+    for (int i = 0; i < nN; ++i) Dq += ph[i] * Dn[i];
+    for (int i = 0; i < nN; ++i) {
+      double gr[3];
+      for (int k = 0; k < d; ++k) {
+        double s = 0;
+        for (int a = 0; a < d; ++a) s += Ji[k * d + a] * R.dphi[(q * nN + i) * d + a];
+        gr[k] = s;
+      }
+      for (int j = 0; j < nN; ++j)
+        for (int k = 0; k < d; ++k) at(i, oq + j * d + k) += Dq * gr[k] * dv * ph[j];
+    }
+  }
+}
+
+// HDGnGammaModel::computeLocalMatrix / computeLocalRHS
+// (HDGnGammaModel.cpp:105-170), 2-D, m = 2 (n, Gamma): HDGBase with the
+// 2 x 2 tau, then four scalar HDGConvection operators with the nodal
+// velocities v_00 = 0, v_01 = b, v_10 = (c^2 - (Gamma/n)^2) b,
+// v_11 = 2 (Gamma/n) b from the BufferSolution, HDGDiffusion with D on
+// (n, n) and G on (Gamma, Gamma) (each only when given), interleaved into the
+// component blocks (scalar index i -> 2 i + a); the RHS is one Source per
+// component.
+void ngamma_local(const Ref& R, const double* x, const double* D, const double* G, const double* b, const double* sol,
+                  const double* tau, double cs, const double* f_ip, double* K, double* F) {
+  const int d = R.d, m = 2, nN = R.nN, n1 = R.n(), n = n1 * m;
+  if (d != 2) throw std::runtime_error("nGamma: 2-D only (b has 2 components)");
+  Geo g; geometry(R, x, g);
+  std::fill(K, K + (size_t)n * n, 0.0);
+  std::fill(F, F + n, 0.0);
+  base_m(R, g, tau, m, K);
+  std::vector<double> v(4 * nN * d, 0.0);
+  for (int i = 0; i < nN; ++i) {
+    const double r = sol[i * 2 + 1] / sol[i * 2];
+    for (int k = 0; k < d; ++k) {
+      v[(1 * nN + i) * d + k] = b[i * 2 + k];
+      v[(2 * nN + i) * d + k] = (cs * cs - r * r) * b[i * 2 + k];
+      v[(3 * nN + i) * d + k] = r * 2.0 * b[i * 2 + k];
+    }
+  }
+  std::vector<double> C((size_t)4 * n1 * n1, 0.0), Df((size_t)2 * n1 * n1, 0.0);
+  for (int t = 0; t < 4; ++t) convection_op(R, g, &v[(size_t)t * nN * d], &C[(size_t)t * n1 * n1]);
+  if (D) diffusion_op(R, g, D, &Df[0]);
+  if (G) diffusion_op(R, g, G, &Df[(size_t)n1 * n1]);
+  for (int i = 0; i < n1; ++i)
+    for (int j = 0; j < n1; ++j) {
+      const size_t ij = (size_t)i * n1 + j, nn = (size_t)n1 * n1;
+      K[(size_t)(2 * i) * n + 2 * j] += Df[ij] + C[ij];
+      K[(size_t)(2 * i) * n + 2 * j + 1] += C[nn + ij];
+      K[(size_t)(2 * i + 1) * n + 2 * j] += C[2 * nn + ij];
+      K[(size_t)(2 * i + 1) * n + 2 * j + 1] += Df[nn + ij] + C[3 * nn + ij];
+    }
+  if (f_ip)
+    for (int q = 0; q < R.nip; ++q)
+      for (int i = 0; i < nN; ++i)
+        for (int a = 0; a < m; ++a) F[i * m + a] += g.dV[q] * f_ip[q * m + a] * R.phi[q * nN + i];
+}
+
 // HDGBurgersModel::computeLocalMatrix / computeLocalRHS
 // (HDGBurgersModel.cpp:86-132) with m = d unknowns per node: HDGBase with an
 // m x m tau per face node (HDGBase.cpp:18-32, 67-158), HDGDiffusion with a
@@ -870,45 +1047,7 @@ void burgers_local(const Ref& R, const double* x, const double* Dn, const double
     }
   };
   auto L = [&](int f, int j, int a) { return ol + (f * nNf + j) * m + a; };
-  // --- HDGBase
-  for (int f = 0; f < nF; ++f)
-    for (int q = 0; q < R.nipf; ++q) {
-      const int o = f * R.nipf + q;
-      const double dv = g.dV[R.nip + o];
-      const double* ps = &R.fphi[q * nNf];
-      double T[9] = {0};
-      for (int t = 0; t < m * m; ++t)
-        for (int j = 0; j < nNf; ++j) T[t] += tau[(f * nNf + j) * m * m + t] * ps[j];
-      for (int a = 0; a < nNf; ++a)
-        for (int c = 0; c < nNf; ++c) {
-          const int ia = R.fmap[f][a], ic = R.fmap[f][c];
-          const double w = dv * ps[a] * ps[c];
-          for (int na = 0; na < m; ++na)
-            for (int mb = 0; mb < m; ++mb) {
-              const double val = T[mb * m + na] * w;
-              at(L(f, a, na), L(f, c, mb)) -= val;
-              at(L(f, a, na), ic * m + mb) += val;
-              at(ia * m + na, ic * m + mb) += val;
-              at(ia * m + na, L(f, c, mb)) -= val;
-            }
-          for (int na = 0; na < m; ++na)
-            for (int k = 0; k < d; ++k) at(oq + (ia * d + k) * m + na, L(f, c, na)) -= w * g.nrm[o * d + k];
-        }
-    }
-  for (int q = 0; q < R.nip; ++q) {
-    const double dv = g.dV[q];
-    const double* ph = &R.phi[q * nN];
-    for (int i = 0; i < nN; ++i) {
-      double gr[3];
-      grad(q, i, gr);
-      for (int k = 0; k < d; ++k)
-        for (int a = 0; a < m; ++a)
-          for (int j = 0; j < nN; ++j) {
-            at(oq + (i * d + k) * m + a, j * m + a) += dv * gr[k] * ph[j];
-            at(oq + (i * d + k) * m + a, oq + (j * d + k) * m + a) += dv * ph[i] * ph[j];
-          }
-    }
-  }
+  base_m(R, g, tau, m, K);
   // --- HDGUNabU, in its own matrix for the right-hand side
   std::vector<double> op((size_t)n * n, 0.0);
   auto opa = [&](int r, int c) -> double& { return op[(size_t)r * n + c]; };
@@ -1016,6 +1155,15 @@ extern "C" int orc_burgers_local(int dim, int p, const double* x, const double* 
   ORC_TRY
   check_dim(dim, p);
   burgers_local(ref(dim, p), x, Dn, tau, sol, trace, f_ip, K, F);
+  ORC_END
+}
+
+extern "C" int orc_ngamma_local(int p, const double* x, const double* D, const double* G, const double* b,
+                                const double* sol, const double* tau, double cs, const double* f_ip, double* K,
+                                double* F) {
+  ORC_TRY
+  check_dim(2, p);
+  ngamma_local(ref(2, p), x, D, G, b, sol, tau, cs, f_ip, K, F);
   ORC_END
 }
 

@@ -143,6 +143,13 @@ int orc_system_local_vectors(const orc_system* s, const double* u, const double*
  * nIP m source values (NULL: none).  K n x n row-major, F n. */
 int orc_burgers_local(int dim, int p, const double* x, const double* Dn, const double* tau, const double* sol,
                       const double* trace, const double* f_ip, double* K, double* F);
+/* HDGnGammaModel local system (HDGnGammaModel.cpp:105-170), 2-D, m = 2
+ * (n, Gamma), in orc_burgers_local's order.  D, G: nN scalars on the n and
+ * Gamma diagonal blocks (NULL: absent); b: nN x 2; sol: nN x 2 (the
+ * BufferSolution, n != 0); tau as orc_burgers_local; cs: the sound speed;
+ * f_ip: nIP x 2 source values (NULL: none). */
+int orc_ngamma_local(int p, const double* x, const double* D, const double* G, const double* b, const double* sol,
+                     const double* tau, double cs, const double* f_ip, double* K, double* F);
 /* Static condensation (HDGSolver.cpp:331-348) of a local system with lu, lq
  * and ll rows of u, q and lambda. */
 int orc_condense_sizes(int lu, int lq, int ll, const double* K, const double* F, double* S, double* S0, double* U,
