@@ -1,7 +1,8 @@
-"""The Burgers path on the GPU: HDGBurgersModel (src/model/HDGBurgersModel.cpp)
-on HDGSolver's IMPLICIT assembly with m = dim unknowns per node, and
-NonLinearWrapper's loop (src/solver/NonLinearWrapper.cpp), through the
-hfx_burgers_* entry points of libhfx_hip.so.
+"""The multi-DOF nonlinear path on the GPU: HDGBurgersModel
+(src/model/HDGBurgersModel.cpp, m = dim unknowns per node) and HDGnGammaModel
+(src/model/HDGnGammaModel.cpp, 2-D, m = 2) on HDGSolver's IMPLICIT assembly,
+and NonLinearWrapper's loop (src/solver/NonLinearWrapper.cpp), through the
+hfx_burgers_* / hfx_ngamma_* entry points of libhfx_hip.so.
 
 Field shapes follow the reference's Field values: tau (nFaces, nNf, sides,
 m, m) with tau[F, p, s][a, b] = tau_ab of the face's node p seen from the
@@ -111,3 +112,27 @@ class BurgersProblem:
                                               float(rtol), int(maxits), int(restart), ctypes.byref(it),
                                               ctypes.byref(res), ctypes.byref(lin)))
         return it.value, res.value, lin.value
+
+
+class NGammaProblem(BurgersProblem):
+    """HDGSolver + HDGnGammaModel (2-D; unknowns n and Gamma) on one GPU; the
+    state fields hold (n, Gamma) per node."""
+
+    def __init__(self, order, device=0):
+        h = ctypes.c_void_p()
+        check(lib.hfx_ngamma_create(int(device), int(order), ctypes.byref(h)))
+        self._h = h
+        self.dim, self.order, self.m = 2, order, 2
+        self.mesh = None
+
+    def set_fields(self, tau, D=None, G=None, b=None, sound_speed=1.0, dirichlet=None, source=None):
+        """tau (nFaces, nNf, sides, 2, 2); D, G (nNodes,) or None; b (nNodes, 2)."""
+        t = np.asarray(tau, dtype=np.float64)
+        if t.ndim != 5 or t.shape[3:] != (2, 2) or t.shape[2] not in (1, 2):
+            raise ValueError('tau must be (nFaces, nNf, sides, 2, 2) with sides 1 or 2')
+        tcol = _f64(np.swapaxes(t, 3, 4))
+        arrs = [None if v is None else _f64(v) for v in (D, G, b, dirichlet, source)]
+        self._keep = (tcol, *arrs)
+        Dd, Gd, bd, g, f = arrs
+        check(lib.hfx_ngamma_set_fields(self._h, _ptr(tcol), t.shape[2], _ptr(Dd), _ptr(Gd), _ptr(bd),
+                                        float(sound_speed), _ptr(g), _ptr(f)))

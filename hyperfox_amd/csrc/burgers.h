@@ -34,6 +34,13 @@ struct BurgersArgs {
   const double* sol = nullptr;
   const double* trace = nullptr;
   const double* f_ip = nullptr;
+  // model 1: HDGnGammaModel (2-D, m = 2): Dn is its D field (n block), G
+  // (nNodes, null: absent) the Gamma block's coefficient, bfield the nodal
+  // b vectors (nNodes x 2), sound_speed c_s
+  int model = 0;
+  const double* G = nullptr;
+  const double* bfield = nullptr;
+  double sound_speed = 1.0;
   double *S = nullptr, *S0 = nullptr, *U = nullptr, *Q = nullptr, *U0 = nullptr, *Q0 = nullptr;
 };
 

@@ -53,17 +53,23 @@ __device__ inline void binv(const double* J, double dt, double* Ji) {
   }
 }
 
-template <int D, int P>
+// MDL 0: HDGBurgersModel (one diffusion coefficient for every component,
+// HDGUNabU); MDL 1: HDGnGammaModel (D on n, G on Gamma, four HDGConvection
+// velocities, no HDGUNabU).
+template <int D, int P, int MDL>
 struct BLay {
   using Z = ESz<D, P>;
   static constexpr int M = D;
+  static constexpr int NDC = MDL == 1 ? M : 1;  // diffusion coefficient sets
+  static constexpr int NV = MDL == 1 ? M * M : 0;  // convection velocities
   static constexpr int NN = Z::NN, NNF = Z::NNF, NIP = Z::NIP, NIPF = Z::NIPF, NF = Z::NF, NL = Z::NL;
   static constexpr int MU = M * NN, ML = M * NL, MM = M * M, NFQ = NF * NIPF;
   static constexpr int NC1 = NN + D * NN + D * NL;  // [M | Squ_k | Sql_k]
   static constexpr int NC2 = MU + ML + 1;           // [K | R | F_u]
   static constexpr int O_X = Z::TAB;
-  static constexpr int O_DN = O_X + NN * D;
-  static constexpr int O_TAU = O_DN + NN;       // [f][j][t]
+  static constexpr int O_DN = O_X + NN * D;     // [c][i]
+  static constexpr int O_VN = O_DN + NDC * NN;  // [i][t][k] nodal velocities
+  static constexpr int O_TAU = O_VN + NN * NV * D;  // [f][j][t]
   static constexpr int O_SOL = O_TAU + NL * MM;  // [i][a]
   static constexpr int O_TR = O_SOL + MU;        // [f][j][a]
   static constexpr int O_FIP = O_TR + ML;        // [q][a]
@@ -71,30 +77,33 @@ struct BLay {
   static constexpr int O_GR = O_DV + NIP;        // [q][i][k] physical grad phi_i
   static constexpr int O_UQ = O_GR + NIP * NN * D;  // [q][a]
   static constexpr int O_DIV = O_UQ + NIP * M;
-  static constexpr int O_DQ = O_DIV + NIP;
-  static constexpr int O_FDS = O_DQ + NIP;       // [f][q] face measure
+  static constexpr int O_DQ = O_DIV + NIP;      // [c][q]
+  static constexpr int O_VQ = O_DQ + NDC * NIP;  // [q][t][k]
+  static constexpr int O_FDS = O_VQ + NIP * NV * D;  // [f][q] face measure
   static constexpr int O_FN = O_FDS + NFQ;       // [f][q][k] outward normal
   static constexpr int O_LAM = O_FN + NFQ * D;   // [f][q][a] trace
   static constexpr int O_USF = O_LAM + NFQ * M;  // [f][q][a] cell solution
   static constexpr int O_TQ = O_USF + NFQ * M;   // [f][q][t] tau
-  static constexpr int O_DFQ = O_TQ + NFQ * MM;
-  static constexpr int O_FL = O_DFQ + NFQ;       // F_l [ML]
+  static constexpr int O_DFQ = O_TQ + NFQ * MM;  // [c][f q]
+  static constexpr int O_VFN = O_DFQ + NDC * NFQ;  // [f q][t] v_t . n
+  static constexpr int O_FL = O_VFN + NFQ * NV;  // F_l [ML]
   static constexpr int O_AUG1 = O_FL + ML;
-  static constexpr int O_SUQ = O_AUG1 + NN * NC1;    // [k][i][j]
-  static constexpr int O_SLQ = O_SUQ + D * NN * NN;  // [k][f nNf + a][j]
-  static constexpr int O_KS = O_SLQ + D * NL * NN;   // sum_k Suq_k A_k
-  static constexpr int O_RS = O_KS + NN * NN;        // sum_k Suq_k B_k
-  static constexpr int O_P = O_RS + NN * NL;         // sum_k Slq_k A_k
-  static constexpr int O_PB = O_P + NL * NN;         // sum_k Slq_k B_k
-  static constexpr int O_SLU = O_PB + NL * NL;       // Slu - P (x) I  [ML][MU]
+  static constexpr int O_SUQ = O_AUG1 + NN * NC1;          // [c][k][i][j]
+  static constexpr int O_SLQ = O_SUQ + NDC * D * NN * NN;  // [c][k][f nNf + a][j]
+  static constexpr int O_KS = O_SLQ + NDC * D * NL * NN;   // [c] sum_k Suq_k A_k
+  static constexpr int O_RS = O_KS + NDC * NN * NN;        // [c] sum_k Suq_k B_k
+  static constexpr int O_P = O_RS + NDC * NN * NL;         // [c] sum_k Slq_k A_k
+  static constexpr int O_PB = O_P + NDC * NL * NN;         // [c] sum_k Slq_k B_k
+  static constexpr int O_SLU = O_PB + NDC * NL * NL;       // Slu - P (x) I  [ML][MU]
   static constexpr int O_AUG2 = O_SLU + ML * MU;
   static constexpr int TOTAL = O_AUG2 + MU * NC2;
   static_assert(MU <= 64, "Burgers kernel: at most 64 coupled u rows");
 };
 
-template <int D, int P>
+template <int D, int P, int MDL>
 __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
-  using L = BLay<D, P>;
+  using L = BLay<D, P, MDL>;
+  constexpr int NDC = L::NDC, NV = L::NV;
   using Z = ESz<D, P>;
   constexpr int M = L::M, NN = L::NN, NNF = L::NNF, NIP = L::NIP, NIPF = L::NIPF, NF = L::NF, NL = L::NL;
   constexpr int MU = L::MU, ML = L::ML, MM = L::MM, NFQ = L::NFQ, NC1 = L::NC1, NC2 = L::NC2;
@@ -138,6 +147,9 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
   double* UQ = sm + L::O_UQ;
   double* DIV = sm + L::O_DIV;
   double* DQ = sm + L::O_DQ;
+  double* VN = sm + L::O_VN;
+  double* VQ = sm + L::O_VQ;
+  double* VFN = sm + L::O_VFN;
   double* FDS = sm + L::O_FDS;
   double* FN = sm + L::O_FN;
   double* LAM = sm + L::O_LAM;
@@ -154,13 +166,19 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
   double* PB = sm + L::O_PB;
   double* SLU = sm + L::O_SLU;
   double* AUG2 = sm + L::O_AUG2;
-  const bool diff = a.Dn != nullptr;
+  // coefficient set c of component a: cs_of(a); set c present: has_c(c)
+  auto cs_of = [](int comp) { return MDL == 1 ? comp : 0; };
+  auto has_c = [&](int c) { return c == 0 ? a.Dn != nullptr : a.G != nullptr; };
 
   for (int e = blockIdx.x; e < a.nEl; e += gridDim.x) {
     __syncthreads();  // the previous cell's outputs are written
     // ------------------------------------------------------------ gather
     for (int t = tid; t < NN * D; t += BT) X[t] = a.coords[(size_t)a.cells[(size_t)e * NN + t / D] * D + t % D];
-    for (int t = tid; t < NN; t += BT) DN[t] = diff ? a.Dn[a.cells[(size_t)e * NN + t]] : 0.0;
+    for (int t = tid; t < NDC * NN; t += BT) {
+      const int c = t / NN, node = a.cells[(size_t)e * NN + t % NN];
+      const double* src = c == 0 ? a.Dn : a.G;
+      DN[t] = src ? src[node] : 0.0;
+    }
     for (int t = tid; t < NL * MM; t += BT) {
       const int fj = t / MM, c = t % MM, f = fj / NNF;
       const int F = a.cell2face[(size_t)e * NF + f];
@@ -176,6 +194,20 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
     }
     for (int t = tid; t < NIP * M; t += BT) FIP[t] = a.f_ip ? a.f_ip[(size_t)e * NIP * M + t] : 0.0;
     __syncthreads();
+    if constexpr (MDL == 1) {
+      // HDGnGammaModel.cpp:116-124: v_00 = 0, v_01 = b, v_10 = (c^2 - (G/n)^2) b,
+      // v_11 = 2 (G/n) b at the nodes, from the BufferSolution (n, Gamma)
+      for (int t = tid; t < NN * D; t += BT) {
+        const int i = t / D, k = t % D;
+        const double bk = a.bfield[(size_t)a.cells[(size_t)e * NN + i] * D + k];
+        const double r = SOL[i * M + 1] / SOL[i * M];
+        VN[(i * NV + 0) * D + k] = 0.0;
+        VN[(i * NV + 1) * D + k] = bk;
+        VN[(i * NV + 2) * D + k] = (a.sound_speed * a.sound_speed - r * r) * bk;
+        VN[(i * NV + 3) * D + k] = r * 2.0 * bk;
+      }
+      __syncthreads();
+    }
     // ------------------------------------------------------- IP geometry
     for (int q = tid; q < NIP; q += BT) {
       double J[D * D];
@@ -187,7 +219,8 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
       double Ji[D * D];
       binv<D>(J, dt, Ji);
       DV[q] = W[q] * dt;
-      double uq[D], dv = 0.0, dq = 0.0;
+      double uq[D], dv = 0.0, dq[NDC];
+      for (int c = 0; c < NDC; ++c) dq[c] = 0.0;
       for (int c = 0; c < M; ++c) uq[c] = 0.0;
       for (int i = 0; i < NN; ++i) {
         const double ph = PHI[q * NN + i];
@@ -198,11 +231,16 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
           dv += s * SOL[i * M + k];
         }
         for (int c = 0; c < M; ++c) uq[c] += ph * SOL[i * M + c];
-        dq += ph * DN[i];
+        for (int c = 0; c < NDC; ++c) dq[c] += ph * DN[c * NN + i];
       }
       for (int c = 0; c < M; ++c) UQ[q * M + c] = uq[c];
       DIV[q] = dv;
-      DQ[q] = dq;
+      for (int c = 0; c < NDC; ++c) DQ[c * NIP + q] = dq[c];
+      for (int tk = 0; tk < NV * D; ++tk) {
+        double v = 0.0;
+        for (int i = 0; i < NN; ++i) v += PHI[q * NN + i] * VN[i * NV * D + tk];
+        VQ[q * NV * D + tk] = v;
+      }
     }
     for (int t = tid; t < NFQ; t += BT) {
       const int f = t / NIPF, q = t % NIPF;
@@ -242,7 +280,8 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
         for (int c = 0; c < D; ++c) nv[c] = -nv[c];
       FDS[t] = FW[q] * meas;
       for (int c = 0; c < D; ++c) FN[t * D + c] = nv[c];
-      double lam[D], us[D], tq[MM], dfq = 0.0;
+      double lam[D], us[D], tq[MM], dfq[NDC];
+      for (int c = 0; c < NDC; ++c) dfq[c] = 0.0;
       for (int c = 0; c < M; ++c) lam[c] = us[c] = 0.0;
       for (int c = 0; c < MM; ++c) tq[c] = 0.0;
       for (int j = 0; j < NNF; ++j) {
@@ -253,14 +292,23 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
           us[c] += ps * SOL[nj * M + c];
         }
         for (int c = 0; c < MM; ++c) tq[c] += ps * TAU[(f * NNF + j) * MM + c];
-        dfq += ps * DN[nj];
+        for (int c = 0; c < NDC; ++c) dfq[c] += ps * DN[c * NN + nj];
       }
       for (int c = 0; c < M; ++c) {
         LAM[t * M + c] = lam[c];
         USF[t * M + c] = us[c];
       }
       for (int c = 0; c < MM; ++c) TQ[t * MM + c] = tq[c];
-      DFQ[t] = dfq;
+      for (int c = 0; c < NDC; ++c) DFQ[c * NFQ + t] = dfq[c];
+      for (int tv = 0; tv < NV; ++tv) {
+        double vn = 0.0;
+        for (int k = 0; k < D; ++k) {
+          double v = 0.0;
+          for (int j = 0; j < NNF; ++j) v += FPHI[q * NNF + j] * VN[(fn[f * NNF + j] * NV + tv) * D + k];
+          vn += v * nv[k];
+        }
+        VFN[t * (NV > 0 ? NV : 1) + tv] = vn;
+      }
     }
     __syncthreads();
     // ------------------------------------------------ scalar q blocks
@@ -283,51 +331,59 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
       }
       AUG1[t] = v;
     }
-    for (int t = tid; t < D * NN * NN; t += BT) {  // Suq_k
-      const int k = t / (NN * NN), i = (t / NN) % NN, j = t % NN;
+    for (int t = tid; t < NDC * D * NN * NN; t += BT) {  // Suq_k per coefficient set
+      const int c = t / (D * NN * NN), k = (t / (NN * NN)) % D, i = (t / NN) % NN, j = t % NN;
+      const double* dqc = DQ + c * NIP;
+      const double* dfc = DFQ + c * NFQ;
       double v = 0.0;
-      if (diff) {
-        for (int q = 0; q < NIP; ++q) v += DV[q] * DQ[q] * GR[(q * NN + i) * D + k] * PHI[q * NN + j];
+      if (has_c(c)) {
+        for (int q = 0; q < NIP; ++q) v += DV[q] * dqc[q] * GR[(q * NN + i) * D + k] * PHI[q * NN + j];
         for (int f = 0; f < NF; ++f) {
           const int ai = posf[f * NN + i], cj = posf[f * NN + j];
           if (ai < 0 || cj < 0) continue;
           for (int q = 0; q < NIPF; ++q) {
             const int o = f * NIPF + q;
-            v -= FDS[o] * DFQ[o] * FN[o * D + k] * FPHI[q * NNF + ai] * FPHI[q * NNF + cj];
+            v -= FDS[o] * dfc[o] * FN[o * D + k] * FPHI[q * NNF + ai] * FPHI[q * NNF + cj];
           }
         }
       }
       SUQ[t] = v;
     }
-    for (int t = tid; t < D * NL * NN; t += BT) {  // Slq_k
-      const int k = t / (NL * NN), fa = (t / NN) % NL, j = t % NN, f = fa / NNF, ai = fa % NNF;
+    for (int t = tid; t < NDC * D * NL * NN; t += BT) {  // Slq_k per coefficient set
+      const int c = t / (D * NL * NN), k = (t / (NL * NN)) % D, fa = (t / NN) % NL, j = t % NN;
+      const int f = fa / NNF, ai = fa % NNF;
       const int cj = posf[f * NN + j];
+      const double* dfc = DFQ + c * NFQ;
       double v = 0.0;
-      if (diff && cj >= 0)
+      if (has_c(c) && cj >= 0)
         for (int q = 0; q < NIPF; ++q) {
           const int o = f * NIPF + q;
-          v -= FDS[o] * DFQ[o] * FN[o * D + k] * FPHI[q * NNF + ai] * FPHI[q * NNF + cj];
+          v -= FDS[o] * dfc[o] * FN[o * D + k] * FPHI[q * NNF + ai] * FPHI[q * NNF + cj];
         }
       SLQ[t] = v;
     }
     __syncthreads();
     gauss_jordan<NN, NC1, BT>(AUG1, perm1);
     // A_k[i][j] = AUG1[perm1[i] NC1 + NN + k NN + j], B_k[i][fc] = AUG1[perm1[i] NC1 + NN + D NN + k NL + fc]
-    for (int t = tid; t < NN * NN + NN * NL + NL * NN + NL * NL; t += BT) {
-      int u = t;
+    for (int t = tid; t < NDC * (NN * NN + NN * NL + NL * NN + NL * NL); t += BT) {
+      constexpr int PER = NN * NN + NN * NL + NL * NN + NL * NL;
+      const int c = t / PER;
+      int u = t % PER;
+      const double* suq = SUQ + c * D * NN * NN;
+      const double* slq = SLQ + c * D * NL * NN;
       const double* Lrow;
       int ncol, col;
       double* dst;
       bool isA;
       if (u < NN * NN) {
-        Lrow = SUQ + (u / NN) * NN; ncol = NN * NN; col = u % NN; isA = true; dst = KS + u;
+        Lrow = suq + (u / NN) * NN; ncol = NN * NN; col = u % NN; isA = true; dst = KS + c * NN * NN + u;
       } else if ((u -= NN * NN) < NN * NL) {
-        Lrow = SUQ + (u / NL) * NN; ncol = NN * NN; col = u % NL; isA = false; dst = RS + u;
+        Lrow = suq + (u / NL) * NN; ncol = NN * NN; col = u % NL; isA = false; dst = RS + c * NN * NL + u;
       } else if ((u -= NN * NL) < NL * NN) {
-        Lrow = SLQ + (u / NN) * NN; ncol = NL * NN; col = u % NN; isA = true; dst = PP + u;
+        Lrow = slq + (u / NN) * NN; ncol = NL * NN; col = u % NN; isA = true; dst = PP + c * NL * NN + u;
       } else {
         u -= NL * NN;
-        Lrow = SLQ + (u / NL) * NN; ncol = NL * NN; col = u % NL; isA = false; dst = PB + u;
+        Lrow = slq + (u / NL) * NN; ncol = NL * NN; col = u % NL; isA = false; dst = PB + c * NL * NL + u;
       }
       double v = 0.0;
       for (int k = 0; k < D; ++k)
@@ -352,41 +408,58 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
             v += FDS[o] * TQ[o * MM + mb * M + na] * FPHI[q * NNF + ai] * FPHI[q * NNF + cj];
           }
         }
-        for (int q = 0; q < NIP; ++q) {  // HDGUNabU bulk
-          const double* gi = GR + (q * NN + i) * D;
-          const double* gj = GR + (q * NN + j) * D;
-          const double pi = PHI[q * NN + i], pj = PHI[q * NN + j];
-          double w = UQ[q * M + na] * (gj[mb] * pi + gi[mb] * pj);
-          if (na == mb) {
-            double ug = 0.0;
-            for (int k = 0; k < D; ++k) ug += UQ[q * M + k] * gi[k];
-            w += (DIV[q] * pi + ug) * pj;
+        if constexpr (MDL == 0) {
+          for (int q = 0; q < NIP; ++q) {  // HDGUNabU bulk
+            const double* gi = GR + (q * NN + i) * D;
+            const double* gj = GR + (q * NN + j) * D;
+            const double pi = PHI[q * NN + i], pj = PHI[q * NN + j];
+            double w = UQ[q * M + na] * (gj[mb] * pi + gi[mb] * pj);
+            if (na == mb) {
+              double ug = 0.0;
+              for (int k = 0; k < D; ++k) ug += UQ[q * M + k] * gi[k];
+              w += (DIV[q] * pi + ug) * pj;
+            }
+            v -= DV[q] * w;
           }
-          v -= DV[q] * w;
+        } else {
+          for (int q = 0; q < NIP; ++q) {  // HDGConvection of v_(na mb): Suu -= C^T
+            const double* gi = GR + (q * NN + i) * D;
+            const double* vq = VQ + (q * NV + na * M + mb) * D;
+            double vg = 0.0;
+            for (int k = 0; k < D; ++k) vg += vq[k] * gi[k];
+            v -= DV[q] * vg * PHI[q * NN + j];
+          }
         }
-        if (na == mb) v -= KS[i * NN + j];
+        if (na == mb) v -= KS[cs_of(na) * NN * NN + i * NN + j];
       } else if (c < MU + ML) {
         const int l = c - MU, f = l / (NNF * M), cn = (l / M) % NNF, mb = l % M;
         const int ai = posf[f * NN + i];
         if (ai >= 0)
           for (int q = 0; q < NIPF; ++q) {
             const int o = f * NIPF + q;
-            double ln = 0.0;
-            for (int k = 0; k < D; ++k) ln += LAM[o * M + k] * FN[o * D + k];
-            double w = USF[o * M + na] * FN[o * D + mb] - TQ[o * MM + mb * M + na];
-            if (na == mb) w += ln;
+            double w = -TQ[o * MM + mb * M + na];
+            if constexpr (MDL == 0) {  // HDGUNabU face rows summed into the u rows
+              double ln = 0.0;
+              for (int k = 0; k < D; ++k) ln += LAM[o * M + k] * FN[o * D + k];
+              w += USF[o * M + na] * FN[o * D + mb];
+              if (na == mb) w += ln;
+            } else {  // HDGConvection: Sul += (v.n) face mass
+              w += VFN[o * (NV > 0 ? NV : 1) + na * M + mb];
+            }
             v += FDS[o] * w * FPHI[q * NNF + ai] * FPHI[q * NNF + cn];
           }
-        if (na == mb) v -= RS[i * NL + f * NNF + cn];
+        if (na == mb) v -= RS[cs_of(na) * NN * NL + i * NL + f * NNF + cn];
         v = -v;
       } else {
         for (int q = 0; q < NIP; ++q) {
           const double* gi = GR + (q * NN + i) * D;
           double ug = 0.0;
           for (int k = 0; k < D; ++k) ug += UQ[q * M + k] * gi[k];
-          v += DV[q] * (PHI[q * NN + i] * FIP[q * M + na] - UQ[q * M + na] * (DIV[q] * PHI[q * NN + i] + ug));
+          double w = PHI[q * NN + i] * FIP[q * M + na];
+          if constexpr (MDL == 0) w -= UQ[q * M + na] * (DIV[q] * PHI[q * NN + i] + ug);
+          v += DV[q] * w;
         }
-        for (int f = 0; f < NF; ++f) {
+        for (int f = 0; f < NF && MDL == 0; ++f) {
           const int ai = posf[f * NN + i];
           if (ai < 0) continue;
           for (int q = 0; q < NIPF; ++q) {
@@ -409,12 +482,12 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
             const int o = f * NIPF + q;
             v += FDS[o] * TQ[o * MM + mb * M + na] * FPHI[q * NNF + ai] * FPHI[q * NNF + cj];
           }
-        if (na == mb) v -= PP[(f * NNF + ai) * NN + j];
+        if (na == mb) v -= PP[cs_of(na) * NL * NN + (f * NNF + ai) * NN + j];
         SLU[t] = v;
       } else {
         const int rl = t - ML * MU, f = rl / (NNF * M), ai = (rl / M) % NNF, na = rl % M;
         double v = 0.0;
-        for (int q = 0; q < NIPF; ++q) {
+        for (int q = 0; q < NIPF && MDL == 0; ++q) {
           const int o = f * NIPF + q;
           double ln = 0.0;
           for (int k = 0; k < D; ++k) ln += LAM[o * M + k] * FN[o * D + k];
@@ -457,15 +530,20 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
         const int f = rl / (NNF * M), ai = (rl / M) % NNF, na = rl % M;
         const int g = l / (NNF * M), cn = (l / M) % NNF, mb = l % M;
         if (f == g)
-          for (int q = 0; q < NIPF; ++q) {  // Sll: HDGBase tau and the HDGUNabU face terms
+          for (int q = 0; q < NIPF; ++q) {  // Sll: HDGBase tau and HDGUNabU / HDGConvection face terms
             const int o = f * NIPF + q;
-            double ln = 0.0;
-            for (int k = 0; k < D; ++k) ln += LAM[o * M + k] * FN[o * D + k];
-            double w = USF[o * M + na] * FN[o * D + mb] - TQ[o * MM + mb * M + na];
-            if (na == mb) w += ln;
+            double w = -TQ[o * MM + mb * M + na];
+            if constexpr (MDL == 0) {
+              double ln = 0.0;
+              for (int k = 0; k < D; ++k) ln += LAM[o * M + k] * FN[o * D + k];
+              w += USF[o * M + na] * FN[o * D + mb];
+              if (na == mb) w += ln;
+            } else {
+              w += VFN[o * (NV > 0 ? NV : 1) + na * M + mb];
+            }
             v += FDS[o] * w * FPHI[q * NNF + ai] * FPHI[q * NNF + cn];
           }
-        if (na == mb) v -= PB[(f * NNF + ai) * NL + g * NNF + cn];
+        if (na == mb) v -= PB[cs_of(na) * NL * NL + (f * NNF + ai) * NL + g * NNF + cn];
         a.S[((size_t)e * ML + rl) * ML + l] = v;
       } else {
         double v = FL[rl];
@@ -476,18 +554,18 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
   }
 }
 
-template <int D, int P>
+template <int D, int P, int MDL>
 void launch_dp(const BurgersArgs& a, hipStream_t s) {
-  const size_t lds = (size_t)BLay<D, P>::TOTAL * sizeof(double);
+  const size_t lds = (size_t)BLay<D, P, MDL>::TOTAL * sizeof(double);
   if (lds > 150 * 1024) throw std::runtime_error("Burgers kernel: LDS footprint above 150 KB");
   static bool attr = false;
   if (!attr) {
-    HFX_HIP(hipFuncSetAttribute((const void*)burgers_cell_kernel<D, P>,
+    HFX_HIP(hipFuncSetAttribute((const void*)burgers_cell_kernel<D, P, MDL>,
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     attr = true;
   }
   const int grid = a.nEl < 65536 ? a.nEl : 65536;
-  hipLaunchKernelGGL((burgers_cell_kernel<D, P>), dim3(grid), dim3(BT), lds, s, a);
+  hipLaunchKernelGGL((burgers_cell_kernel<D, P, MDL>), dim3(grid), dim3(BT), lds, s, a);
   HFX_HIP(hipGetLastError());
 }
 
@@ -558,18 +636,27 @@ bool burgers_supported(int dim, int order) {
 
 void launch_burgers(int dim, int order, const BurgersArgs& a, hipStream_t s) {
   if (a.nEl <= 0) return;
-  if (dim == 2) {
+  if (a.model == 1) {
+    if (dim != 2) throw std::runtime_error("nGamma kernel: 2-D only");
     switch (order) {
-      case 1: launch_dp<2, 1>(a, s); return;
-      case 2: launch_dp<2, 2>(a, s); return;
-      case 3: launch_dp<2, 3>(a, s); return;
-      case 4: launch_dp<2, 4>(a, s); return;
-      case 5: launch_dp<2, 5>(a, s); return;
+      case 1: launch_dp<2, 1, 1>(a, s); return;
+      case 2: launch_dp<2, 2, 1>(a, s); return;
+      case 3: launch_dp<2, 3, 1>(a, s); return;
+      case 4: launch_dp<2, 4, 1>(a, s); return;
+      case 5: launch_dp<2, 5, 1>(a, s); return;
+    }
+  } else if (dim == 2) {
+    switch (order) {
+      case 1: launch_dp<2, 1, 0>(a, s); return;
+      case 2: launch_dp<2, 2, 0>(a, s); return;
+      case 3: launch_dp<2, 3, 0>(a, s); return;
+      case 4: launch_dp<2, 4, 0>(a, s); return;
+      case 5: launch_dp<2, 5, 0>(a, s); return;
     }
   } else if (dim == 3) {
     switch (order) {
-      case 1: launch_dp<3, 1>(a, s); return;
-      case 2: launch_dp<3, 2>(a, s); return;
+      case 1: launch_dp<3, 1, 0>(a, s); return;
+      case 2: launch_dp<3, 2, 0>(a, s); return;
     }
   }
   throw std::runtime_error("Burgers kernel: unsupported dimension/order");

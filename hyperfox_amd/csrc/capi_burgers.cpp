@@ -34,7 +34,9 @@ struct hfx_burgers {
   hfx::Mesh mesh;
   bool has_mesh = false, has_fields = false, assembled = false;
   int tau_sides = 1, bc_kind = HFX_BC_INTEGRATED_DIRICHLET, krylov = HFX_KRYLOV_FGMRES;
-  hfx::DevBuf<double> tabs, coords, Dn, tau, f_ip, g;
+  int model = 0;            // 0 HDGBurgersModel, 1 HDGnGammaModel
+  double sound_speed = 1.0;  // nGammaParams::soundSpeed
+  hfx::DevBuf<double> tabs, coords, Dn, tau, f_ip, g, G, bfield;
   hfx::DevBuf<double> sol, buf, flux, trace;  // Solution, BufferSolution, Flux, Trace
   hfx::DevBuf<double> S, S0, U, Q, U0, Q0, rhs, dinv, red;
   hfx::DevBuf<int> faceNodes, cells, c2f, f2c, bfaces;
@@ -57,6 +59,7 @@ void need(bool c, const char* msg) {
 
 void assemble(hfx_burgers* h) {
   need(h->has_mesh && h->has_fields, "hfx_burgers_assemble: set the mesh and the fields first");
+  need(h->model == 0 || h->bfield.p != nullptr, "hfx_burgers_assemble: the nGamma b field is not set");
   const hfx::Mesh& m = h->mesh;
   hipStream_t s = h->stream;
   hfx::BurgersArgs a;
@@ -74,6 +77,10 @@ void assemble(hfx_burgers* h) {
   a.sol = h->buf.p;
   a.trace = h->trace.p;
   a.f_ip = h->f_ip.p;
+  a.model = h->model;
+  a.G = h->G.p;
+  a.bfield = h->bfield.p;
+  a.sound_speed = h->sound_speed;
   a.S = h->S.p; a.S0 = h->S0.p; a.U = h->U.p; a.Q = h->Q.p; a.U0 = h->U0.p; a.Q0 = h->Q0.p;
   hfx::launch_burgers(h->dim, h->order, a, s);
   // trace system: HDGSolver's scatter (Dirichlet rows identity / g), then the
@@ -139,15 +146,13 @@ void solve(hfx_burgers* h, int pc, double rtol, int maxits, int restart, hfx::Kr
 
 }  // namespace
 
-extern "C" int hfx_burgers_create(int device, int dim, int order, hfx_burgers** out) {
-  HFX_TRY
-  need(out != nullptr, "hfx_burgers_create: null output");
+namespace {
+hfx_burgers* create(int device, int dim, int order, int model) {
   int n = 0;
   HFX_HIP(hipGetDeviceCount(&n));
   need(device >= 0 && device < n, "hfx_burgers_create: no such GPU");
-  need(hfx::burgers_supported(dim, order),
-       "hfx_burgers_create: the Burgers kernel covers 2-D orders 1-5 and 3-D orders 1-2");
   auto h = std::unique_ptr<hfx_burgers>(new hfx_burgers());
+  h->model = model;
   h->device = device;
   h->dim = dim;
   h->order = order;
@@ -162,7 +167,24 @@ extern "C" int hfx_burgers_create(int device, int dim, int order, hfx_burgers** 
   for (auto& f : h->re->faceNodes) fnodes.insert(fnodes.end(), f.begin(), f.end());
   h->faceNodes.upload(fnodes, h->stream);
   HFX_HIP(hipStreamSynchronize(h->stream));
-  *out = h.release();
+  return h.release();
+}
+}  // namespace
+
+extern "C" int hfx_burgers_create(int device, int dim, int order, hfx_burgers** out) {
+  HFX_TRY
+  need(out != nullptr, "hfx_burgers_create: null output");
+  need(hfx::burgers_supported(dim, order),
+       "hfx_burgers_create: the Burgers kernel covers 2-D orders 1-5 and 3-D orders 1-2");
+  *out = create(device, dim, order, 0);
+  HFX_END
+}
+
+extern "C" int hfx_ngamma_create(int device, int order, hfx_burgers** out) {
+  HFX_TRY
+  need(out != nullptr, "hfx_ngamma_create: null output");
+  need(order >= 1 && order <= 5, "hfx_ngamma_create: the nGamma kernel covers 2-D orders 1-5");
+  *out = create(device, 2, order, 1);
   HFX_END
 }
 
@@ -241,9 +263,11 @@ extern "C" int hfx_burgers_set_mesh(hfx_burgers* h, const hfx_mesh* mm) {
   HFX_END
 }
 
-extern "C" int hfx_burgers_set_fields(hfx_burgers* h, const double* tau, int tau_sides, const double* D,
-                                      const double* dirichlet, const double* f_ip) {
-  HFX_TRY
+namespace {
+// The fields both models take: Tau, the first diffusion coefficient, the
+// Dirichlet values and the source values.
+void set_common_fields(hfx_burgers* h, const double* tau, int tau_sides, const double* D, const double* dirichlet,
+                       const double* f_ip) {
   need(h && h->has_mesh, "hfx_burgers_set_fields: set the mesh first");
   need(tau != nullptr, "hfx_burgers_set_fields: the Tau field is required");
   need(tau_sides == 1 || tau_sides == 2,
@@ -265,6 +289,33 @@ extern "C" int hfx_burgers_set_fields(hfx_burgers* h, const double* tau, int tau
   HFX_HIP(hipStreamSynchronize(s));
   h->has_fields = true;
   h->assembled = false;
+}
+}  // namespace
+
+extern "C" int hfx_burgers_set_fields(hfx_burgers* h, const double* tau, int tau_sides, const double* D,
+                                      const double* dirichlet, const double* f_ip) {
+  HFX_TRY
+  need(h != nullptr, "hfx_burgers_set_fields: null handle");
+  need(h->model == 0, "hfx_burgers_set_fields: an nGamma problem takes hfx_ngamma_set_fields");
+  set_common_fields(h, tau, tau_sides, D, dirichlet, f_ip);
+  HFX_END
+}
+
+extern "C" int hfx_ngamma_set_fields(hfx_burgers* h, const double* tau, int tau_sides, const double* D,
+                                     const double* G, const double* b, double sound_speed, const double* dirichlet,
+                                     const double* f_ip) {
+  HFX_TRY
+  need(h && h->has_mesh, "hfx_ngamma_set_fields: set the mesh first");
+  need(h->model == 1, "hfx_ngamma_set_fields: not an nGamma problem");
+  need(b != nullptr, "hfx_ngamma_set_fields: must provide a b vector field to the nGamma model");
+  need(std::isfinite(sound_speed), "hfx_ngamma_set_fields: the sound speed must be finite");
+  set_common_fields(h, tau, tau_sides, D, dirichlet, f_ip);
+  const hfx::Mesh& m = h->mesh;
+  if (G) h->G.upload(G, (size_t)m.nNodes, h->stream);
+  else h->G.release();
+  h->bfield.upload(b, (size_t)m.nNodes * 2, h->stream);
+  h->sound_speed = sound_speed;
+  HFX_HIP(hipStreamSynchronize(h->stream));
   HFX_END
 }
 

@@ -513,6 +513,17 @@ int hfx_hdg_pc_setup_ms(hfx_hdg* h, double* ms);
 typedef struct hfx_burgers hfx_burgers;
 int hfx_burgers_create(int device, int dim, int order, hfx_burgers** out);
 void hfx_burgers_destroy(hfx_burgers* h);
+/* HDGnGammaModel (src/model/HDGnGammaModel.cpp:5-170): 2-D, m = 2 unknowns
+ * (n, Gamma) per node, orders 1-5, on the same handle type: every
+ * hfx_burgers_* call below applies, with hfx_ngamma_set_fields in place of
+ * hfx_burgers_set_fields. */
+int hfx_ngamma_create(int device, int order, hfx_burgers** out);
+/* HDGnGammaModel::setFieldMap (HDGnGammaModel.cpp:5-44): Tau, Dirichlet and
+ * the source values as for Burgers; D (n block) and G (Gamma block) nNodes
+ * scalars or NULL (absent); b nNodes x 2 (required); sound_speed
+ * (nGammaParams::soundSpeed).  The BufferSolution's n must not vanish. */
+int hfx_ngamma_set_fields(hfx_burgers* h, const double* tau, int tau_sides, const double* D, const double* G,
+                          const double* b, double sound_speed, const double* dirichlet, const double* f_ip);
 /* HDGSolver::setMesh + allocate (HDGSolver.cpp:5-164, block size nNf m);
  * the state fields start at zero. */
 int hfx_burgers_set_mesh(hfx_burgers* h, const hfx_mesh* m);
