@@ -392,6 +392,69 @@ def burgers(hf, n=128, p=3, reps=5):
     return out
 
 
+def ngamma(hf, n=128, p=3):
+    """HDGnGammaModel on the manufactured (n, Gamma) problem of the tests
+    scaled up: n = 1 + 0.3 sin(pi x) sin(pi y), Gamma = 0.2 cos(pi x)
+    sin(pi y), b = (cos 0.4, sin 0.4), c_s = 1, D = 0.1, G = 0.15, tau = (2 +
+    D n) I, integrated Dirichlet data; Newton from (1, 0) to 1e-10 with
+    BiCGStab + ILU(0) to 1e-12."""
+    A_N, B_G, CS, DIF, GDIF = 0.3, 0.2, 1.0, 0.1, 0.15
+    bdir = np.array([np.cos(0.4), np.sin(0.4)])
+
+    def exact(X):
+        return np.stack([1.0 + A_N * np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1]),
+                         B_G * np.cos(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1])], 1)
+
+    def source(X):
+        s1, c1 = np.sin(np.pi * X[:, 0]), np.cos(np.pi * X[:, 0])
+        s2, c2 = np.sin(np.pi * X[:, 1]), np.cos(np.pi * X[:, 1])
+        nn, g = 1.0 + A_N * s1 * s2, B_G * c1 * s2
+        gn = A_N * np.pi * np.stack([c1 * s2, s1 * c2], 1)
+        gg = B_G * np.pi * np.stack([-s1 * s2, c1 * c2], 1)
+        gF = (2 * g / nn)[:, None] * gg - (g * g / (nn * nn))[:, None] * gn + CS * CS * gn
+        return np.stack([gg @ bdir + DIF * 2 * np.pi ** 2 * A_N * s1 * s2,
+                         gF @ bdir + GDIF * 2 * np.pi ** 2 * g], 1)
+
+    mesh = hf.Mesh.structured(2, p, n, 0.0, 0)
+    a = mesh.arrays()
+    s = mesh.sizes
+    tabs = hf.refel_tables(2, p)
+    xq = np.einsum('qi,eic->eqc', tabs['phi'], a['coords'][a['cells']])
+    f = source(xq.reshape(-1, 2)).reshape(mesh.n_cells, s['nIP'], 2)
+    tau = np.zeros((mesh.n_faces, s['nNf'], 2, 2, 2))
+    tau[:] = np.eye(2) * (2.0 + DIF * n)
+    g = exact(a['coords'][a['faces'].reshape(-1)]).reshape(mesh.n_faces, s['nNf'], 2)
+    nn = mesh.n_nodes
+    prob = hf.NGammaProblem(p)
+    prob.set_mesh(mesh)
+    prob.set_fields(tau, D=np.full(nn, DIF), G=np.full(nn, GDIF), b=np.tile(bdir, (nn, 1)), sound_speed=CS,
+                    dirichlet=g, source=f)
+    prob.set_krylov(1)
+    init = np.tile([1.0, 0.0], (mesh.n_cells, s['nN']))
+    prob.set_state(sol=init, buffer=init)
+    prob.assemble()
+    t = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        prob.assemble()
+        t.append(1e3 * (time.perf_counter() - t0))
+    asm_ms = float(np.median(t))
+    t0 = time.perf_counter()
+    its, res, lin = prob.nonlinear_solve(max_iters=20, res_tol=1e-10, pc=4, rtol=1e-12, maxits=20000)
+    nl_ms = 1e3 * (time.perf_counter() - t0)
+    sol = prob.state()[0].reshape(mesh.n_cells, -1)
+    ex = exact(a['coords'][a['cells'].reshape(-1)]).reshape(mesh.n_cells, -1)
+    dofs = mesh.n_cells * s['nN'] * 2
+    out = {'workload': '2-D nGamma (n, Gamma) p=%d, %d^2x2=%d triangles, manufactured' % (p, n, mesh.n_cells),
+           'cells': mesh.n_cells, 'u_dofs': dofs, 'assemble_ms': asm_ms,
+           'assemble_dof_per_s': dofs / (asm_ms * 1e-3), 'newton_iters': its, 'residual': res,
+           'linear_iters': lin, 'nonlinear_solve_ms': nl_ms,
+           'rel_err_vs_exact': float(np.linalg.norm(sol - ex) / np.linalg.norm(ex))}
+    print('# ngamma: assemble %.2f ms, %d Newton iterations (%d linear) in %.1f ms, error %.2e' % (
+        asm_ms, its, lin, nl_ms, out['rel_err_vs_exact']), file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration(mib):
     """Copy `mib` MiB device-to-device 5 times with torch (one plain copy
     kernel per call): the PMC reading of FETCH_SIZE / WRITE_SIZE for these
@@ -497,6 +560,7 @@ def main():
     res['topology'] = topology_times(hf, w)
     if not args.no_burgers:
         res['burgers'] = burgers(hf)
+        res['ngamma'] = ngamma(hf)
     if not args.no_batch:
         res['batched'] = batched(hf, pr)
     if not args.no_cpu:
