@@ -60,6 +60,14 @@ class BurgersProblem:
     def set_boundary_model(self, kind):
         check(lib.hfx_burgers_set_boundary_model(self._h, int(kind)))
 
+    def set_boundary_hdg(self, faces, B, r, matrix_set=True, rhs_set=True):
+        """HDG-type boundary model on `faces`: B (n, bs, bs (2 + dim)), r (n, bs)."""
+        f = np.ascontiguousarray(faces, dtype=np.int32)
+        Bd, rd = _f64(B), _f64(r)
+        self._keep_hdg = (f, Bd, rd)
+        check(lib.hfx_burgers_set_boundary_hdg(self._h, _ptr(f), int(f.size), _ptr(Bd), _ptr(rd), int(bool(matrix_set)),
+                                               int(bool(rhs_set))))
+
     def set_krylov(self, krylov):
         check(lib.hfx_burgers_set_krylov(self._h, int(krylov)))
 

@@ -39,8 +39,13 @@ struct hfx_burgers {
   hfx::DevBuf<double> tabs, coords, Dn, tau, f_ip, g, G, bfield;
   hfx::DevBuf<double> sol, buf, flux, trace;  // Solution, BufferSolution, Flux, Trace
   hfx::DevBuf<double> S, S0, U, Q, U0, Q0, rhs, dinv, red;
-  hfx::DevBuf<int> faceNodes, cells, c2f, f2c, bfaces;
-  hfx::DevBuf<int8_t> perm, perm_m, invperm_m;
+  hfx::DevBuf<int> faceNodes, cells, c2f, f2c, bfaces, bpos;
+  hfx::DevBuf<int8_t> perm, invperm, perm_m, invperm_m;
+  // HDG-type boundary model (hfx_burgers_set_boundary_hdg): faces, local
+  // matrices, rhs, Set / Add; bfaces then lists the other boundary faces
+  hfx::DevBuf<int> hdg_faces;
+  hfx::DevBuf<double> hdg_B, hdg_r;
+  int n_hdg = 0, n_dir = 0, hdg_mset = 1, hdg_rset = 1;
   hfx::DevBuf<uint8_t> is_dir;
   hfx::BsrPattern pat;
   hfx::BsrDev A;
@@ -86,15 +91,20 @@ void assemble(hfx_burgers* h) {
   // trace system: HDGSolver's scatter (Dirichlet rows identity / g), then the
   // IntegratedDirichletModel rows M (x) I and (M (x) I) g
   hfx::launch_scatter(h->A, m.nFpc, h->S.p, h->S0.p, h->invperm_m.p, h->is_dir.p, h->g.p, h->rhs.p, s);
-  if (h->bc_kind == HFX_BC_INTEGRATED_DIRICHLET && !m.boundary.empty()) {
+  if (h->bc_kind == HFX_BC_INTEGRATED_DIRICHLET && h->n_dir > 0) {
     const hfx::RefEl& re = *h->re;
     const int nN = re.nN, nNf = re.nNf(), nip = re.nip(), nipf = re.face->nip(), d = h->dim;
     const size_t t_fphi = (size_t)nip * nN * (1 + d), t_fdphi = t_fphi + (size_t)nipf * nNf;
     const size_t t_fw = t_fdphi + (size_t)nipf * nNf * (d - 1) + nip;
-    hfx::launch_integrated_dirichlet(h->A, (int)m.boundary.size(), h->bfaces.p, d, nN, nipf, m.nFpc, h->faceNodes.p,
-                                     h->perm.p, h->cells.p, h->coords.p, h->tabs.p + t_fphi, h->tabs.p + t_fdphi,
+    hfx::launch_integrated_dirichlet(h->A, h->n_dir, h->bfaces.p, d, nN, nipf, m.nFpc, h->faceNodes.p, h->perm.p,
+                                     h->cells.p, h->coords.p, h->tabs.p + t_fphi, h->tabs.p + t_fdphi,
                                      h->tabs.p + t_fw, h->g.p, h->rhs.p, s, h->m);
   }
+  // HDG-type boundary rows (HDGSolver.cpp:503-518) on the faces of that model
+  if (h->n_hdg > 0)
+    hfx::launch_hdg_boundary(h->A, h->n_hdg, h->hdg_faces.p, h->dim, m.nN, m.nFpc, h->faceNodes.p, h->perm.p,
+                             h->invperm.p, h->bpos.p, h->U.p, h->Q.p, h->U0.p, h->Q0.p, h->hdg_B.p, h->hdg_r.p,
+                             h->hdg_mset, h->hdg_rset, h->rhs.p, s, h->m);
   h->assembled = true;
 }
 
@@ -220,13 +230,14 @@ extern "C" int hfx_burgers_set_mesh(hfx_burgers* h, const hfx_mesh* mm) {
   std::vector<int8_t> inv(m.face_perm.size());
   for (size_t blk = 0; blk < m.face_perm.size() / m.nNf; ++blk)
     for (int j = 0; j < m.nNf; ++j) inv[blk * m.nNf + m.face_perm[blk * m.nNf + j]] = (int8_t)j;
-  hfx::DevBuf<int8_t> inv_d;
-  inv_d.upload(inv, s);
+  h->invperm.upload(inv, s);
+  h->n_hdg = 0;
+  h->n_dir = (int)m.boundary.size();
   const long long nblk = (long long)m.nCells * m.nFpc;
   h->perm_m.alloc((size_t)nblk * bs);
   h->invperm_m.alloc((size_t)nblk * bs);
   hfx::launch_expand_perm(nblk, m.nNf, md, h->perm.p, h->perm_m.p, s);
-  hfx::launch_expand_perm(nblk, m.nNf, md, inv_d.p, h->invperm_m.p, s);
+  hfx::launch_expand_perm(nblk, m.nNf, md, h->invperm.p, h->invperm_m.p, s);
   h->pat = hfx::build_bsr_pattern(m.nCells, m.nFaces, m.nFpc, bs, m.cell2face.data(), m.face2cell.data());
   h->A.nb = h->pat.nb;
   h->A.bs = h->pat.bs;
@@ -238,6 +249,7 @@ extern "C" int hfx_burgers_set_mesh(hfx_burgers* h, const hfx_mesh* mm) {
   h->A.contrib.upload(h->pat.contrib, s);
   h->A.face_cells.upload(h->pat.face_cells, s);
   h->A.vals.alloc((size_t)h->A.nnzb * bs * bs);
+  h->bpos.upload(hfx::cell_block_positions(h->pat, m.nCells, m.nFpc, m.cell2face.data()), s);
   std::vector<uint8_t> dir(m.nFaces, 0);
   for (int f : m.boundary) dir[f] = 1;
   h->is_dir.upload(dir, s);
@@ -325,6 +337,47 @@ extern "C" int hfx_burgers_set_boundary_model(hfx_burgers* h, int kind) {
   need(kind == HFX_BC_DIRICHLET || kind == HFX_BC_INTEGRATED_DIRICHLET,
        "hfx_burgers_set_boundary_model: unknown boundary model");
   h->bc_kind = kind;
+  h->assembled = false;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_set_boundary_hdg(hfx_burgers* h, const int* faces, int n, const double* B,
+                                            const double* r, int matrix_set, int rhs_set) {
+  HFX_TRY
+  need(h && h->has_mesh, "hfx_burgers_set_boundary_hdg: set the mesh first");
+  need(n >= 0 && (n == 0 || (faces && B && r)), "hfx_burgers_set_boundary_hdg: null argument");
+  const hfx::Mesh& m = h->mesh;
+  const int bs = h->bs(), nc = bs * (2 + h->dim);
+  std::vector<char> bnd(m.nFaces, 0), seen(m.nFaces, 0);
+  for (int f : m.boundary) bnd[f] = 1;
+  for (int i = 0; i < n; ++i) {
+    need(faces[i] >= 0 && faces[i] < m.nFaces && bnd[faces[i]], "hfx_burgers_set_boundary_hdg: not a boundary face");
+    need(!seen[faces[i]], "hfx_burgers_set_boundary_hdg: a face is listed twice");
+    seen[faces[i]] = 1;
+  }
+  for (size_t i = 0; i < (size_t)n * bs * nc; ++i)
+    need(std::isfinite(B[i]), "hfx_burgers_set_boundary_hdg: local matrices must be finite");
+  h->set_device();
+  hipStream_t s = h->stream;
+  h->hdg_faces.upload(std::vector<int>(faces, faces + n), s);
+  h->hdg_B.upload(std::vector<double>(B, B + (size_t)n * bs * nc), s);
+  h->hdg_r.upload(std::vector<double>(r, r + (size_t)n * bs), s);
+  h->n_hdg = n;
+  h->hdg_mset = matrix_set != 0;
+  h->hdg_rset = rhs_set != 0;
+  // the Dirichlet models keep the other boundary faces; the scatter leaves
+  // the cell's own rows on these
+  std::vector<int> dirl;
+  std::vector<uint8_t> dir(m.nFaces, 0);
+  for (int f : m.boundary)
+    if (!seen[f]) {
+      dirl.push_back(f);
+      dir[f] = 1;
+    }
+  h->bfaces.upload(dirl, s);
+  h->n_dir = (int)dirl.size();
+  h->is_dir.upload(dir, s);
+  HFX_HIP(hipStreamSynchronize(s));
   h->assembled = false;
   HFX_END
 }

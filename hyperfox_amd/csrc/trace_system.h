@@ -65,11 +65,13 @@ void launch_integrated_dirichlet(const BsrDev& A, int nbf, const int* bfaces, in
 // Dirichlet, so their rows hold the cell's own S and S0), after either
 // scatter: B (nhf x bs x bs(2+dim)) = [B_u | B_q | B_lambda] in face-node
 // order and r (nhf x bs); the rows are set (matrix_set / rhs_set) or added.
-// bpos: cell_block_positions of the pattern.
+// bpos: cell_block_positions of the pattern; face_perm / invperm: the
+// node-level permutations.  With m unknowns per node (bs = nNf m) the
+// columns are B_u (pj m + b), B_q ((pj dim + k) m + b), B_lambda (pj m + b).
 void launch_hdg_boundary(const BsrDev& A, int nhf, const int* hfaces, int dim, int nN, int nFpc, const int* fnodes,
                          const int8_t* face_perm, const int8_t* invperm, const int* bpos, const double* U,
                          const double* Q, const double* U0, const double* Q0, const double* B, const double* r,
-                         int matrix_set, int rhs_set, double* rhs, hipStream_t s);
+                         int matrix_set, int rhs_set, double* rhs, hipStream_t s, int m = 1);
 
 // BSR block index of every (face f, face g) pair of every cell:
 // out[(e*nFpc + f)*nFpc + g] (the diagonal block when f == g).

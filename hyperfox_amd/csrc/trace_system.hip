@@ -197,60 +197,69 @@ __global__ void integrated_dirichlet_rhs_kernel(int nbf, const int* __restrict__
 }
 
 // HDG-type boundary model rows (HDGSolver.cpp:503-518) of boundary face
-// F = hfaces[ib], whose rows the scatter left as the cell's own: row pa (face
-// order) over every cell-local trace column c becomes (Set) or gains (Add)
-// sum_j B_u[pa][j] U[i_j][c] + sum_(j,k) B_q[pa][j,k] Q[i_j d + k][c]
-// (+ B_lambda[pa][pc] on the face's own block), i_j the cell node at face
-// position j.  One thread per (face, row, column).
-__global__ void hdg_boundary_kernel(int nhf, const int* __restrict__ hfaces, int dim, int nN, int nNf, int nFpc,
-                                    const int* __restrict__ face_cells, const int* __restrict__ fnodes,
+// F = hfaces[ib], whose rows the scatter left as the cell's own, with m
+// unknowns per node (block size bs = nNf m, face order p m + a): row pa over
+// every cell-local trace column c becomes (Set) or gains (Add)
+//   sum_(pj, b) B_u[pa][pj m + b] U[i_pj m + b][c]
+//   + sum_(pj, k, b) B_q[pa][(pj d + k) m + b] Q[(i_pj d + k) m + b][c]
+// (+ B_lambda[pa][pc] on the face's own block), i_pj the cell node at face
+// position pj (the U and Q rows of the face's nodes, HDGSolver.cpp:508-511).
+// One thread per (face, row, column).
+__global__ void hdg_boundary_kernel(int nhf, const int* __restrict__ hfaces, int dim, int nN, int nNf, int m,
+                                    int nFpc, const int* __restrict__ face_cells, const int* __restrict__ fnodes,
                                     const int8_t* __restrict__ face_perm, const int8_t* __restrict__ invperm,
                                     const int* __restrict__ bpos, const double* __restrict__ U,
                                     const double* __restrict__ Q, const double* __restrict__ B, int mset,
                                     double* __restrict__ vals) {
-  const int NL = nFpc * nNf, nc = nNf * (2 + dim);
+  const int bs = nNf * m, NL = nFpc * bs, nc = bs * (2 + dim);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)nhf * nNf * NL) return;
-  const int ib = (int)(t / ((long long)nNf * NL)), pa = (int)((t / NL) % nNf), c = (int)(t % NL);
+  if (t >= (long long)nhf * bs * NL) return;
+  const int ib = (int)(t / ((long long)bs * NL)), pa = (int)((t / NL) % bs), c = (int)(t % NL);
   const int F = hfaces[ib];
   const int e = face_cells[4 * (size_t)F], lf = face_cells[4 * (size_t)F + 1];
   const int8_t* ipf = invperm + ((size_t)e * nFpc + lf) * nNf;
-  const double* Bi = B + ((size_t)ib * nNf + pa) * nc;
-  const int nQ = nN * dim;
+  const double* Bi = B + ((size_t)ib * bs + pa) * nc;
+  const int mu = nN * m, mq = mu * dim;
   double v = 0.0;
   for (int pj = 0; pj < nNf; ++pj) {
     const int i = fnodes[lf * nNf + ipf[pj]];
-    v += Bi[pj] * U[((size_t)e * nN + i) * NL + c];
-    for (int k = 0; k < dim; ++k) v += Bi[nNf + pj * dim + k] * Q[((size_t)e * nQ + i * dim + k) * NL + c];
+    for (int b = 0; b < m; ++b) {
+      v += Bi[pj * m + b] * U[((size_t)e * mu + i * m + b) * NL + c];
+      for (int k = 0; k < dim; ++k)
+        v += Bi[bs + (pj * dim + k) * m + b] * Q[((size_t)e * mq + (i * dim + k) * m + b) * NL + c];
+    }
   }
-  const int G = c / nNf, b = c % nNf;
-  const int pc = face_perm[((size_t)e * nFpc + G) * nNf + b];
-  if (G == lf) v += Bi[nNf * (1 + dim) + pc];
-  double* dst = vals + (size_t)bpos[((size_t)e * nFpc + lf) * nFpc + G] * nNf * nNf + pa * nNf + pc;
+  const int G = c / bs, bn = (c % bs) / m, bc = c % m;
+  const int pc = face_perm[((size_t)e * nFpc + G) * nNf + bn] * m + bc;
+  if (G == lf) v += Bi[bs * (1 + dim) + pc];
+  double* dst = vals + (size_t)bpos[((size_t)e * nFpc + lf) * nFpc + G] * bs * bs + pa * bs + pc;
   *dst = mset ? v : *dst + v;
 }
 
 // rhs of those rows: r - B_u U0_F - B_q Q0_F, set or added
-__global__ void hdg_boundary_rhs_kernel(int nhf, const int* __restrict__ hfaces, int dim, int nN, int nNf, int nFpc,
-                                        const int* __restrict__ face_cells, const int* __restrict__ fnodes,
+__global__ void hdg_boundary_rhs_kernel(int nhf, const int* __restrict__ hfaces, int dim, int nN, int nNf, int m,
+                                        int nFpc, const int* __restrict__ face_cells, const int* __restrict__ fnodes,
                                         const int8_t* __restrict__ invperm, const double* __restrict__ U0,
                                         const double* __restrict__ Q0, const double* __restrict__ B,
                                         const double* __restrict__ r, int rset, double* __restrict__ rhs) {
-  const int nc = nNf * (2 + dim);
+  const int bs = nNf * m, nc = bs * (2 + dim);
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)nhf * nNf) return;
-  const int ib = (int)(t / nNf), pa = (int)(t % nNf);
+  if (t >= (long long)nhf * bs) return;
+  const int ib = (int)(t / bs), pa = (int)(t % bs);
   const int F = hfaces[ib];
   const int e = face_cells[4 * (size_t)F], lf = face_cells[4 * (size_t)F + 1];
   const int8_t* ipf = invperm + ((size_t)e * nFpc + lf) * nNf;
-  const double* Bi = B + ((size_t)ib * nNf + pa) * nc;
-  double v = r[(size_t)ib * nNf + pa];
+  const double* Bi = B + ((size_t)ib * bs + pa) * nc;
+  const int mu = nN * m, mq = mu * dim;
+  double v = r[(size_t)ib * bs + pa];
   for (int pj = 0; pj < nNf; ++pj) {
     const int i = fnodes[lf * nNf + ipf[pj]];
-    v -= Bi[pj] * U0[(size_t)e * nN + i];
-    for (int k = 0; k < dim; ++k) v -= Bi[nNf + pj * dim + k] * Q0[((size_t)e * nN + i) * dim + k];
+    for (int b = 0; b < m; ++b) {
+      v -= Bi[pj * m + b] * U0[(size_t)e * mu + i * m + b];
+      for (int k = 0; k < dim; ++k) v -= Bi[bs + (pj * dim + k) * m + b] * Q0[(size_t)e * mq + (i * dim + k) * m + b];
+    }
   }
-  double* dst = rhs + (size_t)F * nNf + pa;
+  double* dst = rhs + (size_t)F * bs + pa;
   *dst = rset ? v : *dst + v;
 }
 
@@ -371,15 +380,16 @@ void launch_integrated_dirichlet(const BsrDev& A, int nbf, const int* bfaces, in
 void launch_hdg_boundary(const BsrDev& A, int nhf, const int* hfaces, int dim, int nN, int nFpc, const int* fnodes,
                          const int8_t* face_perm, const int8_t* invperm, const int* bpos, const double* U,
                          const double* Q, const double* U0, const double* Q0, const double* B, const double* r,
-                         int matrix_set, int rhs_set, double* rhs, hipStream_t s) {
+                         int matrix_set, int rhs_set, double* rhs, hipStream_t s, int m) {
   if (nhf <= 0) return;
-  const int nt = 256, bs = A.bs;
+  const int nt = 256, bs = A.bs, nNf = bs / m;
+  if (nNf * m != bs) throw std::runtime_error("HDG-type boundary: block size is not a multiple of m");
   const long long n = (long long)nhf * bs * nFpc * bs;
-  hipLaunchKernelGGL(hdg_boundary_kernel, dim3(ceil_div(n, nt)), dim3(nt), 0, s, nhf, hfaces, dim, nN, bs, nFpc,
+  hipLaunchKernelGGL(hdg_boundary_kernel, dim3(ceil_div(n, nt)), dim3(nt), 0, s, nhf, hfaces, dim, nN, nNf, m, nFpc,
                      A.face_cells.p, fnodes, face_perm, invperm, bpos, U, Q, B, matrix_set, A.vals.p);
   HFX_HIP(hipGetLastError());
   hipLaunchKernelGGL(hdg_boundary_rhs_kernel, dim3(ceil_div((long long)nhf * bs, nt)), dim3(nt), 0, s, nhf, hfaces,
-                     dim, nN, bs, nFpc, A.face_cells.p, fnodes, invperm, U0, Q0, B, r, rhs_set, rhs);
+                     dim, nN, nNf, m, nFpc, A.face_cells.p, fnodes, invperm, U0, Q0, B, r, rhs_set, rhs);
   HFX_HIP(hipGetLastError());
 }
 

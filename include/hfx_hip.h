@@ -535,6 +535,14 @@ int hfx_burgers_set_fields(hfx_burgers* h, const double* tau, int tau_sides, con
  * (IntegratedDirichletModel, the default; rows (M (x) I_m) and (M (x) I_m) g)
  * on every boundary face. */
 int hfx_burgers_set_boundary_model(hfx_burgers* h, int kind);
+/* The HDG-type boundary branch (HDGSolver.cpp:503-518) on n boundary faces,
+ * with the model's local matrices B (n x bs x bs (2 + dim), bs = nNf m, in
+ * face order: columns B_u (pj m + b), B_q ((pj dim + k) m + b), B_lambda
+ * (pj m + b)) and rhs r (n x bs); rows Set or Add.  The other boundary faces
+ * keep the Dirichlet model; a new call replaces the previous set (n = 0
+ * clears it).  For HDGBohmModel the caller runs the model per face. */
+int hfx_burgers_set_boundary_hdg(hfx_burgers* h, const int* faces, int n, const double* B, const double* r,
+                                 int matrix_set, int rhs_set);
 int hfx_burgers_set_krylov(hfx_burgers* h, int krylov);
 /* Host copies in and out of the state fields (NULL: left as they are). */
 int hfx_burgers_set_state(hfx_burgers* h, const double* sol, const double* buffer, const double* trace);

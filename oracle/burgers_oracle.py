@@ -13,7 +13,10 @@ solved with a sparse direct solver:
   (HDGSolver.cpp:531-675);
 * boundary faces: IntegratedDirichletModel rows M (x) I_m and M g
   (IntegratedDirichletModel.cpp:35-56, the Mass operator with m unknowns per
-  node) or DirichletModel rows I and g (DirichletModel.cpp);
+  node) or DirichletModel rows I and g (DirichletModel.cpp); faces of an
+  HDG-type model (HDGSolver.cpp:503-518) get B_u U_F + B_q Q_F over the
+  cell's trace columns plus B_lambda on their own block, and
+  r - B_u U0_F - B_q Q0_F, Set (rows zeroed first) or Add;
 * u = U lambda_e + U0, q = Q lambda_e + Q0 (HDGSolver.cpp:741-775);
 * NonLinearWrapper::solve (NonLinearWrapper.cpp:41-79) with the vanilla
   residual ||u - u_buf|| / ||u_buf|| (||u - u_buf|| when u_buf = 0) and the
@@ -99,9 +102,11 @@ class BurgersSystem:
     HDGnGammaModel fields G (nNodes scalars or None), b (nNodes, 2) and cs
     (sound speed); Dn is then its D field."""
 
-    def __init__(self, dim, p, mesh, tau, Dn=None, f_ip=None, dirichlet=None, boundary='integrated', ngamma=None):
+    def __init__(self, dim, p, mesh, tau, Dn=None, f_ip=None, dirichlet=None, boundary='integrated', ngamma=None,
+                 hdg=None):
         self.dim, self.p, self.m = dim, p, dim
         self.ngamma = ngamma
+        self.hdg = hdg  # dict(faces, B (n, bs, bs (2 + dim)), r (n, bs), mset, rset)
         self.s = sizes(dim, p)
         self.mesh = {k: np.asarray(mesh[k]) for k in ('coords', 'cells', 'faces', 'cell2face', 'face2cell')}
         M = self.mesh
@@ -167,7 +172,30 @@ class BurgersSystem:
         # boundary rows
         A = A.tolil()
         bs = nNf * m
+        hdg_faces = set() if self.hdg is None else set(int(F) for F in self.hdg['faces'])
+        if self.hdg is not None:
+            fmap = refel(dim, self.p)['face_nodes']
+            for ib, F in enumerate(self.hdg['faces']):
+                e = M['face2cell'][F, 0]
+                lf = list(M['cell2face'][e]).index(F)
+                node_at = {int(self.pos[e, lf, j]): int(fmap[lf][j]) for j in range(nNf)}
+                rows_u = [node_at[pj] * m + b for pj in range(nNf) for b in range(m)]
+                rows_q = [(node_at[pj] * dim + k) * m + b for pj in range(nNf) for k in range(dim) for b in range(m)]
+                Bi = np.asarray(self.hdg['B'][ib])
+                Bu, Bq, Bl = Bi[:, :bs], Bi[:, bs:bs * (1 + dim)], Bi[:, bs * (1 + dim):]
+                rowvals = Bu @ self.U[e][rows_u] + Bq @ self.Q[e][rows_q]  # bs x local trace columns
+                rhsv = np.asarray(self.hdg['r'][ib]) - Bu @ self.U0[e][rows_u] - Bq @ self.Q0[e][rows_q]
+                r0 = F * bs
+                if self.hdg.get('mset', True):
+                    A[r0:r0 + bs, :] = 0.0
+                blk = np.zeros((bs, self.ndofs))
+                np.add.at(blk, (slice(None), self.gidx[e]), rowvals)
+                blk[:, r0:r0 + bs] += Bl
+                A[r0:r0 + bs, :] = A[r0:r0 + bs, :].toarray() + blk
+                b[r0:r0 + bs] = rhsv if self.hdg.get('rset', True) else b[r0:r0 + bs] + rhsv
         for F in self.bfaces:
+            if int(F) in hdg_faces:
+                continue
             r0 = F * bs
             A[r0:r0 + bs, :] = 0.0
             gF = self.g[F].reshape(-1)

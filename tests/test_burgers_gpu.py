@@ -185,3 +185,57 @@ def test_checks():
     with pytest.raises(hf.HfxError, match='unsupported preconditioner'):
         prob.assemble()
         prob.solve(pc=hf.PC_LEARNED)
+
+
+def dense(prob):
+    rowptr, col, vals, rhs = prob.trace_system()
+    bs = prob.bs
+    A = np.zeros((prob.nb * bs, prob.nb * bs))
+    for F in range(prob.nb):
+        for k in range(rowptr[F], rowptr[F + 1]):
+            A[F * bs:(F + 1) * bs, col[k] * bs:(col[k] + 1) * bs] = vals[k]
+    return A, rhs
+
+
+@pytest.mark.parametrize('model', ['burgers', 'ngamma'])
+@pytest.mark.parametrize('mset,rset', [(1, 1), (0, 0), (1, 0)])
+def test_hdg_boundary_matches_oracle(model, mset, rset):
+    """HDG-type boundary rows (HDGSolver.cpp:503-518) with m = 2 on half of
+    the boundary faces, integrated Dirichlet on the rest."""
+    print('multi-DOF HDG boundary', model, mset, rset, flush=True)
+    p = 2
+    if model == 'burgers':
+        mesh, fields, state = workload(2, p)
+        prob = gpu_problem(2, p, mesh, fields, state)
+    else:
+        from test_ngamma_gpu import workload as ng_workload, gpu_problem as ng_problem
+        mesh, fields, state = ng_workload(p)
+        prob = ng_problem(p, mesh, fields, state)
+    a = mesh.arrays()
+    rng = np.random.default_rng(9)
+    bf = np.nonzero(a['face2cell'][:, 1] < 0)[0][::2]
+    bs = prob.bs
+    B = rng.standard_normal((bf.size, bs, bs * 4))
+    r = rng.standard_normal((bf.size, bs))
+    prob.set_boundary_hdg(bf, B, r, mset, rset)
+    prob.assemble()
+    A, rhs = dense(prob)
+    hdg = dict(faces=bf, B=B, r=r, mset=bool(mset), rset=bool(rset))
+    if model == 'burgers':
+        orc = bo.BurgersSystem(2, p, a, fields['tau'], Dn=fields['D'], f_ip=fields['source'],
+                               dirichlet=fields['dirichlet'], hdg=hdg)
+    else:
+        ng = dict(G=fields['G'], b=fields['b'], cs=fields['sound_speed'])
+        orc = bo.BurgersSystem(2, p, a, fields['tau'], Dn=fields['D'], f_ip=fields['source'],
+                               dirichlet=fields['dirichlet'], ngamma=ng, hdg=hdg)
+    Ao, bo_ = orc.assemble(state['buffer'].reshape(mesh.n_cells, -1), state['trace'])
+    assert rel(A, Ao.toarray()) < 1e-10 and rel(rhs, bo_) < 1e-10
+    prob.set_boundary_hdg([], np.zeros((0, bs, bs * 4)), np.zeros((0, bs)))
+    prob.assemble()
+    A2, rhs2 = dense(prob)
+    orc.hdg = None
+    Ao2, bo2 = orc.assemble(state['buffer'].reshape(mesh.n_cells, -1), state['trace'])
+    assert rel(A2, Ao2.toarray()) < 1e-10 and rel(rhs2, bo2) < 1e-10
+    with pytest.raises(hf.HfxError, match='not a boundary face'):
+        interior = np.nonzero(a['face2cell'][:, 1] >= 0)[0][:1]
+        prob.set_boundary_hdg(interior, B[:1], r[:1])
