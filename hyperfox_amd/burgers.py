@@ -68,6 +68,19 @@ class BurgersProblem:
         check(lib.hfx_burgers_set_boundary_hdg(self._h, _ptr(f), int(f.size), _ptr(Bd), _ptr(rd), int(bool(matrix_set)),
                                                int(bool(rhs_set))))
 
+    def set_time_scheme(self, scheme, dt=0.0):
+        """0: steady; 1 (TS_BEULER): implicit Euler with step dt."""
+        check(lib.hfx_burgers_set_time_scheme(self._h, int(scheme), float(dt)))
+
+    def start_step(self):
+        """OldSolution = BufferSolution = Solution."""
+        check(lib.hfx_burgers_start_step(self._h))
+
+    def set_dirichlet(self, dirichlet):
+        g = None if dirichlet is None else _f64(dirichlet)
+        self._keep_g = g
+        check(lib.hfx_burgers_set_dirichlet(self._h, _ptr(g)))
+
     def set_krylov(self, krylov):
         check(lib.hfx_burgers_set_krylov(self._h, int(krylov)))
 

@@ -38,6 +38,11 @@ struct BurgersArgs {
   // (nNodes, null: absent) the Gamma block's coefficient, bfield the nodal
   // b vectors (nNodes x 2), sound_speed c_s
   int model = 0;
+  // implicit Euler on the u rows (RungeKutta BEuler / Euler, the rows divided
+  // by dt): with ts_idt = 1/dt > 0, Suu + (M / dt) (x) I and F_u + (M (x) I)
+  // u_old / dt, u_old nEl x nN x m (the "OldSolution")
+  double ts_idt = 0.0;
+  const double* u_old = nullptr;
   const double* G = nullptr;
   const double* bfield = nullptr;
   double sound_speed = 1.0;

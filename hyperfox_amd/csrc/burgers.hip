@@ -192,7 +192,16 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
       const int F = a.cell2face[(size_t)e * NF + f];
       TR[t] = a.trace[((size_t)F * NNF + a.face_perm[(size_t)e * NL + fj]) * M + t % M];
     }
-    for (int t = tid; t < NIP * M; t += BT) FIP[t] = a.f_ip ? a.f_ip[(size_t)e * NIP * M + t] : 0.0;
+    for (int t = tid; t < NIP * M; t += BT) {
+      double fq = a.f_ip ? a.f_ip[(size_t)e * NIP * M + t] : 0.0;
+      if (a.u_old && a.ts_idt != 0.0) {  // implicit Euler: + u_old(x_q) / dt
+        const int q = t / M, c = t % M;
+        double uo = 0.0;
+        for (int j = 0; j < NN; ++j) uo += PHI[q * NN + j] * a.u_old[((size_t)e * NN + j) * M + c];
+        fq += a.ts_idt * uo;
+      }
+      FIP[t] = fq;
+    }
     __syncthreads();
     if constexpr (MDL == 1) {
       // HDGnGammaModel.cpp:116-124: v_00 = 0, v_01 = b, v_10 = (c^2 - (G/n)^2) b,
@@ -430,7 +439,14 @@ __global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
             v -= DV[q] * vg * PHI[q * NN + j];
           }
         }
-        if (na == mb) v -= KS[cs_of(na) * NN * NN + i * NN + j];
+        if (na == mb) {
+          v -= KS[cs_of(na) * NN * NN + i * NN + j];
+          if (a.ts_idt != 0.0) {  // implicit Euler: + M / dt
+            double mij = 0.0;
+            for (int q = 0; q < NIP; ++q) mij += DV[q] * PHI[q * NN + i] * PHI[q * NN + j];
+            v += a.ts_idt * mij;
+          }
+        }
       } else if (c < MU + ML) {
         const int l = c - MU, f = l / (NNF * M), cn = (l / M) % NNF, mb = l % M;
         const int ai = posf[f * NN + i];

@@ -35,6 +35,8 @@ struct hfx_burgers {
   bool has_mesh = false, has_fields = false, assembled = false;
   int tau_sides = 1, bc_kind = HFX_BC_INTEGRATED_DIRICHLET, krylov = HFX_KRYLOV_FGMRES;
   int model = 0;            // 0 HDGBurgersModel, 1 HDGnGammaModel
+  double ts_dt = 0.0;       // implicit Euler time step (0: steady)
+  hfx::DevBuf<double> old;  // "OldSolution" of the time step
   double sound_speed = 1.0;  // nGammaParams::soundSpeed
   hfx::DevBuf<double> tabs, coords, Dn, tau, f_ip, g, G, bfield;
   hfx::DevBuf<double> sol, buf, flux, trace;  // Solution, BufferSolution, Flux, Trace
@@ -83,6 +85,10 @@ void assemble(hfx_burgers* h) {
   a.trace = h->trace.p;
   a.f_ip = h->f_ip.p;
   a.model = h->model;
+  if (h->ts_dt > 0.0) {
+    a.ts_idt = 1.0 / h->ts_dt;
+    a.u_old = h->old.p;
+  }
   a.G = h->G.p;
   a.bfield = h->bfield.p;
   a.sound_speed = h->sound_speed;
@@ -266,6 +272,8 @@ extern "C" int hfx_burgers_set_mesh(hfx_burgers* h, const hfx_mesh* mm) {
   h->flux.alloc(h->n_sol() * h->dim);
   h->trace.alloc(h->n_trace());
   h->red.alloc(2 * 256 + 2);
+  h->old.alloc(h->n_sol());
+  HFX_HIP(hipMemsetAsync(h->old.p, 0, h->old.n * sizeof(double), s));
   HFX_HIP(hipMemsetAsync(h->sol.p, 0, h->sol.n * sizeof(double), s));
   HFX_HIP(hipMemsetAsync(h->buf.p, 0, h->buf.n * sizeof(double), s));
   HFX_HIP(hipMemsetAsync(h->flux.p, 0, h->flux.n * sizeof(double), s));
@@ -378,6 +386,43 @@ extern "C" int hfx_burgers_set_boundary_hdg(hfx_burgers* h, const int* faces, in
   h->n_dir = (int)dirl.size();
   h->is_dir.upload(dir, s);
   HFX_HIP(hipStreamSynchronize(s));
+  h->assembled = false;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_set_time_scheme(hfx_burgers* h, int scheme, double dt) {
+  HFX_TRY
+  need(h != nullptr, "hfx_burgers_set_time_scheme: null handle");
+  need(scheme == HFX_TS_NONE || scheme == HFX_TS_BEULER,
+       "hfx_burgers_set_time_scheme: the multi-DOF problems take implicit Euler (BEuler) only");
+  need(scheme == HFX_TS_NONE || (dt > 0.0 && std::isfinite(dt)),
+       "hfx_burgers_set_time_scheme: the time step needs to be set and it should not be 0");
+  h->ts_dt = scheme == HFX_TS_NONE ? 0.0 : dt;
+  h->assembled = false;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_start_step(hfx_burgers* h) {
+  HFX_TRY
+  need(h && h->has_mesh, "hfx_burgers_start_step: set the mesh first");
+  h->set_device();
+  hipStream_t s = h->stream;
+  HFX_HIP(hipMemcpyAsync(h->old.p, h->sol.p, h->n_sol() * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HFX_HIP(hipMemcpyAsync(h->buf.p, h->sol.p, h->n_sol() * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HFX_HIP(hipStreamSynchronize(s));
+  h->assembled = false;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_set_dirichlet(hfx_burgers* h, const double* dirichlet) {
+  HFX_TRY
+  need(h && h->has_fields, "hfx_burgers_set_dirichlet: set the fields first");
+  h->set_device();
+  if (dirichlet)
+    HFX_HIP(hipMemcpyAsync(h->g.p, dirichlet, h->n_trace() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  else
+    HFX_HIP(hipMemsetAsync(h->g.p, 0, h->n_trace() * sizeof(double), h->stream));
+  HFX_HIP(hipStreamSynchronize(h->stream));
   h->assembled = false;
   HFX_END
 }

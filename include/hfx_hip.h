@@ -544,6 +544,15 @@ int hfx_burgers_set_boundary_model(hfx_burgers* h, int kind);
 int hfx_burgers_set_boundary_hdg(hfx_burgers* h, const int* faces, int n, const double* B, const double* r,
                                  int matrix_set, int rhs_set);
 int hfx_burgers_set_krylov(hfx_burgers* h, int krylov);
+/* Implicit Euler (HFX_TS_BEULER: RungeKutta's BEuler, RungeKutta.cpp:113-159,
+ * as in tests/regression/HDG/TestHDGBurgers.cpp) or none: the u rows gain
+ * (M (x) I) / dt and F_u gains (M (x) I) OldSolution / dt.  OldSolution starts
+ * at zero; hfx_burgers_start_step copies Solution into OldSolution and
+ * BufferSolution (the driver's copies at the start of a step). */
+int hfx_burgers_set_time_scheme(hfx_burgers* h, int scheme, double dt);
+int hfx_burgers_start_step(hfx_burgers* h);
+/* New Dirichlet values (nFaces x nNf x m; NULL: zero), e.g. for t_{n+1}. */
+int hfx_burgers_set_dirichlet(hfx_burgers* h, const double* dirichlet);
 /* Host copies in and out of the state fields (NULL: left as they are). */
 int hfx_burgers_set_state(hfx_burgers* h, const double* sol, const double* buffer, const double* trace);
 int hfx_burgers_get_state(hfx_burgers* h, double* sol, double* buffer, double* flux, double* trace);

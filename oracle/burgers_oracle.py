@@ -17,6 +17,10 @@ solved with a sparse direct solver:
   HDG-type model (HDGSolver.cpp:503-518) get B_u U_F + B_q Q_F over the
   cell's trace columns plus B_lambda on their own block, and
   r - B_u U0_F - B_q Q0_F, Set (rows zeroed first) or Add;
+* implicit Euler (RungeKutta BEuler, RungeKutta.cpp:113-159, its first and
+  only stage): the u rows of K and F_u times dt, then (M (x) I_m) added to the
+  u-u block and (M (x) I_m) u_old to F_u, M the scalar mass (the first
+  component block of Sqq = I (x) M);
 * u = U lambda_e + U0, q = Q lambda_e + Q0 (HDGSolver.cpp:741-775);
 * NonLinearWrapper::solve (NonLinearWrapper.cpp:41-79) with the vanilla
   residual ||u - u_buf|| / ||u_buf|| (||u - u_buf|| when u_buf = 0) and the
@@ -77,6 +81,20 @@ def condense_sizes(lu, lq, ll, K, F):
     return S, S0, U, Q, U0, Q0
 
 
+def implicit_euler(dim, m, nN, K, F, dt, u_old):
+    """RungeKutta::apply for BEuler on the u rows of an m-unknown local system."""
+    lu = nN * m
+    oq = lu
+    Msc = K[oq:oq + nN * dim * m:dim * m, oq:oq + nN * dim * m:dim * m]  # Sqq = I (x) M: first component block
+    Mm = np.kron(Msc, np.eye(m))
+    K2, F2 = K.copy(), F.copy()
+    K2[:lu] *= dt
+    F2[:lu] *= dt
+    K2[:lu, :lu] += Mm
+    F2[:lu] += Mm @ u_old
+    return K2, F2
+
+
 def face_positions(dim, p, cells, faces, cell2face):
     """pos[e, f, j]: position of cell e's local face f node j in the node list
     of its face (the orientation HDGSolver reads from the mesh)."""
@@ -107,6 +125,8 @@ class BurgersSystem:
         self.dim, self.p, self.m = dim, p, dim
         self.ngamma = ngamma
         self.hdg = hdg  # dict(faces, B (n, bs, bs (2 + dim)), r (n, bs), mset, rset)
+        self.dt = None  # implicit Euler step, with self.u_old (nCells x nN m)
+        self.u_old = None
         self.s = sizes(dim, p)
         self.mesh = {k: np.asarray(mesh[k]) for k in ('coords', 'cells', 'faces', 'cell2face', 'face2cell')}
         M = self.mesh
@@ -163,6 +183,8 @@ class BurgersSystem:
                 ng = self.ngamma
                 Gn = None if ng.get('G') is None else np.asarray(ng['G'])[M['cells'][e]]
                 K, F = ngamma_local(self.p, x, Dn, Gn, np.asarray(ng['b'])[M['cells'][e]], so, taus, ng['cs'], fi)
+            if self.dt:
+                K, F = implicit_euler(dim, m, nN, K, F, self.dt, np.asarray(self.u_old).reshape(self.nC, -1)[e])
             (self.S[e], self.S0[e], self.U[e], self.Q[e], self.U0[e], self.Q0[e]) = condense_sizes(lu, lq, ll, K, F)
         rows = np.repeat(self.gidx, ll, axis=1).reshape(-1)
         cols = np.tile(self.gidx, (1, ll)).reshape(-1)

@@ -239,3 +239,60 @@ def test_hdg_boundary_matches_oracle(model, mset, rset):
     with pytest.raises(hf.HfxError, match='not a boundary face'):
         interior = np.nonzero(a['face2cell'][:, 1] >= 0)[0][:1]
         prob.set_boundary_hdg(interior, B[:1], r[:1])
+
+
+@pytest.mark.parametrize('model,dim,p', [('burgers', 2, 2), ('burgers', 3, 1), ('ngamma', 2, 2)])
+def test_implicit_euler_blocks_match_oracle(model, dim, p):
+    """Implicit Euler (RungeKutta BEuler) on the u rows, with an OldSolution
+    apart from the linearisation state."""
+    print('multi-DOF implicit Euler', model, dim, p, flush=True)
+    if model == 'burgers':
+        mesh, fields, state = workload(dim, p)
+        prob = gpu_problem(dim, p, mesh, fields, state)
+        orc = oracle_problem(dim, p, mesh, fields)
+    else:
+        from test_ngamma_gpu import workload as ng_workload, gpu_problem as ng_problem, oracle_problem as ng_oracle
+        mesh, fields, state = ng_workload(p)
+        prob = ng_problem(p, mesh, fields, state)
+        orc = ng_oracle(p, mesh, fields)
+    rng = np.random.default_rng(21)
+    u_old = state['buffer'] + 0.1 * rng.standard_normal(state['buffer'].shape)
+    dt = 0.05
+    prob.set_time_scheme(1, dt)
+    prob.set_state(sol=u_old)
+    prob.start_step()
+    prob.set_state(buffer=state['buffer'])
+    prob.assemble()
+    got = prob.condensed()
+    orc.dt, orc.u_old = dt, u_old.reshape(mesh.n_cells, -1)
+    orc.assemble(state['buffer'].reshape(mesh.n_cells, -1), state['trace'])
+    for name, g, o in zip(('S', 'S0', 'U', 'Q', 'U0', 'Q0'), got, (orc.S, orc.S0, orc.U, orc.Q, orc.U0, orc.Q0)):
+        assert rel(g, o) < 1e-10, name
+    with pytest.raises(hf.HfxError, match='implicit Euler \\(BEuler\\) only'):
+        prob.set_time_scheme(3, dt)
+
+
+def test_transient_burgers_steps_match_oracle():
+    """Three implicit-Euler steps of the regression case (TestHDGBurgers.cpp's
+    driver: copy Solution into BufferSolution and OldSolution, then the
+    nonlinear loop), from zero towards the steady Cole-Hopf field."""
+    print('transient burgers', flush=True)
+    mesh_a, orc = regression_case('2e-1', 2)
+    mesh = hf.Mesh.from_cells(2, 2, mesh_a['coords'], mesh_a['cells'])
+    prob = hf.BurgersProblem(2, 2)
+    prob.set_mesh(mesh)
+    prob.set_fields(orc.tau, D=orc.Dn, dirichlet=orc.g)
+    dt = 0.1
+    prob.set_time_scheme(1, dt)
+    nC = mesh.n_cells
+    u = np.zeros((nC, orc.s['nN'] * 2))
+    lam = np.zeros(orc.ndofs)
+    orc.dt = dt
+    for step in range(3):
+        prob.start_step()
+        its, res, lin = prob.nonlinear_solve(max_iters=12, res_tol=1e-8, pc=hf.PC_BJACOBI, rtol=1e-13)
+        orc.u_old = u.copy()
+        u, buf, q, lam, its_o, res_o = orc.newton(u, u, lam, max_iters=12, tol=1e-8)
+        sol, gbuf, flux, tr = prob.state()
+        assert its == its_o and res < 1e-8, step
+        assert rel(sol.reshape(nC, -1), u) < 1e-8 and rel(tr.reshape(-1), lam) < 1e-8, step

@@ -279,3 +279,26 @@ def test_hdg_boundary_rows_hold_after_the_solve(mset):
         else:
             own = A0[F * bs:(F + 1) * bs, :] @ lam - b0[F * bs:(F + 1) * bs]
             assert np.allclose(lhs + own, r[ib], rtol=0, atol=1e-9 * max(1.0, np.abs(r[ib]).max()))
+
+
+@pytest.mark.parametrize('dim,p', [(2, 2), (3, 1)])
+def test_implicit_euler_is_the_scalar_scheme_per_component(dim, p):
+    """With no flow the m-unknown implicit-Euler system is the scalar oracle's
+    apply_euler (pinned on the heat-equation decay) Kronecker-multiplied by I_m."""
+    s, m, lu, lq, ll = sizes_m(dim, p)
+    x = cell(dim, p, RNG)
+    D = 0.5 + RNG.random(s['nN'])
+    t = 0.5 + RNG.random(s['nF'] * s['nNf'])
+    u_old = RNG.standard_normal((s['nN'], m))
+    dt = 0.037
+    K1, F1 = orc.local_system(dim, p, 1, x, D=D, ndD=1, tau=t)
+    tau = np.einsum('j,ab->jba', t, np.eye(m)).reshape(-1)
+    K, F = bo.burgers_local(dim, p, x, D, tau, np.zeros(lu), np.zeros(ll))
+    K2, F2 = bo.implicit_euler(dim, m, s['nN'], K, F, dt, u_old.reshape(-1))
+    for a in range(m):
+        Ke, Fe = orc.apply_euler(dim, p, dt, u_old[:, a], K1, F1)
+        sc = np.abs(Ke).max()
+        assert np.allclose(K2[a::m, a::m], Ke, rtol=0, atol=1e-12 * sc)
+        assert np.allclose(F2[a::m], Fe, rtol=0, atol=1e-12 * max(np.abs(Fe).max(), 1.0))
+    assert np.allclose(K2, np.kron(orc.apply_euler(dim, p, dt, u_old[:, 0], K1, F1)[0], np.eye(m)), rtol=0,
+                       atol=1e-12 * np.abs(K2).max())
