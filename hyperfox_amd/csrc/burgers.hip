@@ -100,8 +100,23 @@ struct BLay {
   static_assert(MU <= 64, "Burgers kernel: at most 64 coupled u rows");
 };
 
+// Waves per SIMD the register allocation is held to (each workgroup puts
+// one wave on each SIMD).  Without a target the allocator takes ~170 VGPRs,
+// two waves per SIMD, and small cells wait on each other's barriers.  When
+// three or more workgroups fit a CU's 160 KB of LDS the target is four
+// (128 VGPRs, a few spills in the set-up phases): measured on gfx950 for
+// 32,768 2-D p=3 cells, Burgers 4.9 -> 3.6 ms, nGamma 3.4 -> 3.1 ms (a target
+// of three ran nGamma at 3.8 ms).  Larger cells keep their LDS-bound count.
 template <int D, int P, int MDL>
-__global__ __launch_bounds__(BT) void burgers_cell_kernel(BurgersArgs a) {
+constexpr int burgers_waves() {
+  constexpr long bytes = (long)BLay<D, P, MDL>::TOTAL * 8 + 1024;
+  constexpr long w = (160L * 1024) / bytes;
+  return w >= 3 ? 4 : (w < 1 ? 1 : (int)w);
+}
+
+template <int D, int P, int MDL>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(burgers_waves<D, P, MDL>())))
+void burgers_cell_kernel(BurgersArgs a) {
   using L = BLay<D, P, MDL>;
   constexpr int NDC = L::NDC, NV = L::NV;
   using Z = ESz<D, P>;
