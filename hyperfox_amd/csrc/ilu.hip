@@ -17,9 +17,12 @@ constexpr int AT = 256;  // sweeps: one thread per scalar row
 
 // Factor the block rows rows[0..gridDim.x) of one forward level.  Row i is
 // staged in LDS (len x bs^2), updated by the already factored rows of earlier
-// levels (lu, dinv), its diagonal block inverted by Gauss-Jordan with partial
-// pivoting, and written to lu (earlier blocks L_ik, diagonal A'_ii, later
-// blocks Uhat_ij) and dinv.  err[0] receives 1 + row on a zero pivot.
+// levels (lu, dinv; each of their blocks staged in LDS with one coalesced load
+// before its product), its diagonal block inverted by Gauss-Jordan with
+// partial pivoting (the pivot found by a wavefront argmax: the first row of
+// largest magnitude, the diagonal's own row when it is NaN), and written to lu
+// (earlier blocks L_ik, diagonal A'_ii, later blocks Uhat_ij) and dinv.
+// err[0] receives 1 + row on a zero pivot.
 __global__ __launch_bounds__(FT) void ilu_factor_kernel(int bs, const int* __restrict__ rows,
                                                         const int* __restrict__ rowptr, const double* __restrict__ vals,
                                                         const int* __restrict__ lptr, const int* __restrict__ lpos,
@@ -30,13 +33,13 @@ __global__ __launch_bounds__(FT) void ilu_factor_kernel(int bs, const int* __res
                                                         const signed char* __restrict__ kind, double* __restrict__ lu,
                                                         double* __restrict__ dinv, int* __restrict__ err) {
   extern __shared__ double sm[];
-  __shared__ int piv;
   const int i = rows[blockIdx.x];
   const int r0 = rowptr[i], len = rowptr[i + 1] - r0, B2 = bs * bs, t = threadIdx.x;
   double* R = sm;             // the row's blocks
   double* M = R + len * B2;   // diagonal block being inverted
   double* V = M + B2;         // its inverse
   double* T = V + B2;         // scratch block
+  double* W = T + B2;         // staged block of an earlier row (Uhat_kj or Dinv_k)
   for (int e = t; e < len * B2; e += FT) R[e] = vals[(size_t)r0 * B2 + e];
   __syncthreads();
   for (int l = lptr[i]; l < lptr[i + 1]; ++l) {
@@ -44,21 +47,26 @@ __global__ __launch_bounds__(FT) void ilu_factor_kernel(int bs, const int* __res
     const double* Rk = R + sl * B2;
     // A'_ij -= A'_ik Uhat_kj  (slot j != slot k: no lane reads what another writes)
     for (int q = pptr[l]; q < pptr[l + 1]; ++q) {
-      const double* U = lu + (size_t)ppos[q] * B2;
+      const double* Ug = lu + (size_t)ppos[q] * B2;
+      for (int e = t; e < B2; e += FT) W[e] = Ug[e];
+      __syncthreads();
       double* Rj = R + pslot[q] * B2;
       for (int e = t; e < B2; e += FT) {
         const int r = e / bs, c = e - r * bs;
         double acc = 0.0;
-        for (int m = 0; m < bs; ++m) acc += Rk[r * bs + m] * U[m * bs + c];
+        for (int m = 0; m < bs; ++m) acc += Rk[r * bs + m] * W[m * bs + c];
         Rj[e] -= acc;
       }
+      __syncthreads();  // W is restaged by the next product
     }
     // L_ik = A'_ik Dinv_k
     const double* Dk = dinv + (size_t)k * B2;
+    for (int e = t; e < B2; e += FT) W[e] = Dk[e];
+    __syncthreads();
     for (int e = t; e < B2; e += FT) {
       const int r = e / bs, c = e - r * bs;
       double acc = 0.0;
-      for (int m = 0; m < bs; ++m) acc += Rk[r * bs + m] * Dk[m * bs + c];
+      for (int m = 0; m < bs; ++m) acc += Rk[r * bs + m] * W[m * bs + c];
       T[e] = acc;
     }
     __syncthreads();
@@ -74,18 +82,23 @@ __global__ __launch_bounds__(FT) void ilu_factor_kernel(int bs, const int* __res
   }
   __syncthreads();
   for (int c = 0; c < bs; ++c) {
-    if (t == 0) {
-      int p = c;
-      double best = fabs(M[c * bs + c]);
-      for (int r = c + 1; r < bs; ++r) {
-        const double v = fabs(M[r * bs + c]);
-        if (v > best) { best = v; p = r; }
-      }
-      piv = p;
-      if (!(best > 0.0) || !isfinite(best)) err[0] = i + 1;
+    // pivot: the first row r >= c of largest |M[r][c]| (NaN rows never win);
+    // the diagonal's row when |M[c][c]| is NaN
+    const double dc = fabs(M[c * bs + c]);
+    double v = -1.0;
+    if (t >= c && t < bs) {
+      v = fabs(M[t * bs + c]);
+      if (v != v) v = -1.0;
     }
-    __syncthreads();
-    const int p = piv;
+    int idx = t;
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(v, off, 64);
+      const int oi = __shfl_xor(idx, off, 64);
+      if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    }
+    const int p = (dc != dc) ? c : idx;
+    const double best = (dc != dc) ? dc : v;
+    if (t == 0 && (!(best > 0.0) || !isfinite(best))) err[0] = i + 1;
     if (p != c && t < bs) {
       double a = M[c * bs + t];
       M[c * bs + t] = M[p * bs + t];
@@ -196,7 +209,7 @@ void Ilu::setup(int nb_, int bs_, const int* rowptr_host, const int* col_host, i
   nb = nb_;
   bs = bs_;
   ordering = ordering_;
-  const size_t lds = ((size_t)plan.maxlen + 3) * bs * bs * sizeof(double);
+  const size_t lds = ((size_t)plan.maxlen + 4) * bs * bs * sizeof(double);
   if (lds > 65536) throw std::runtime_error("ILU(0): a block row is too long for the factorisation kernel");
   rowptr.upload(rowptr_host, nb + 1, s);
   fwd_rows.upload(plan.fwd_rows, s);
@@ -223,7 +236,7 @@ void Ilu::factor(const BsrDev& A, hipStream_t s) {
   if (A.nb != nb || A.bs != bs || (size_t)A.nnzb * bs * bs != lu.n)
     throw std::runtime_error("ILU(0): matrix does not match the analysed pattern");
   HFX_HIP(hipMemsetAsync(err.p, 0, sizeof(int), s));
-  const size_t lds = ((size_t)plan.maxlen + 3) * bs * bs * sizeof(double);
+  const size_t lds = ((size_t)plan.maxlen + 4) * bs * bs * sizeof(double);
   for (int l = 0; l < plan.nlev_f; ++l) {
     const int n = plan.fwd_off[l + 1] - plan.fwd_off[l];
     hipLaunchKernelGGL(ilu_factor_kernel, dim3(n), dim3(FT), lds, s, bs, fwd_rows.p + plan.fwd_off[l], rowptr.p,
