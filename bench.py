@@ -350,7 +350,7 @@ def burgers_flops(p):
     return gj1 + gj2 + outs
 
 
-def burgers(hf, n=128, p=3, reps=5):
+def burgers(hf, n=128, p=3, reps=5, graphs=False):
     """The Burgers path on the reference's stationary regression case scaled
     up: 2-D viscous Burgers (m = 2) on [0, 1]^2 with the Cole-Hopf solution
     as integrated Dirichlet data, D = 1, tau = 2 D / h I, zero initial
@@ -366,6 +366,7 @@ def burgers(hf, n=128, p=3, reps=5):
     prob.set_mesh(mesh)
     prob.set_fields(tau, D=np.ones(mesh.n_nodes), dirichlet=g)
     prob.set_krylov(1)
+    prob.set_solver_graphs(graphs)
     prob.assemble()  # warm-up of the cell kernel and scatter
     t = []
     for _ in range(reps):

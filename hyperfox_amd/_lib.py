@@ -126,6 +126,7 @@ _SIGS = {
     'hfx_ngamma_create': ([_I, _I, _P], _I),
     'hfx_burgers_set_boundary_hdg': ([_P, _P, _I, _P, _P, _I, _I], _I),
     'hfx_burgers_set_time_scheme': ([_P, _I, _D], _I),
+    'hfx_burgers_set_solver_graphs': ([_P, _I], _I),
     'hfx_burgers_start_step': ([_P], _I),
     'hfx_burgers_set_dirichlet': ([_P, _P], _I),
     'hfx_ngamma_set_fields': ([_P, _P, _I, _P, _P, _P, _D, _P, _P], _I),

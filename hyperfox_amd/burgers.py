@@ -81,6 +81,9 @@ class BurgersProblem:
         self._keep_g = g
         check(lib.hfx_burgers_set_dirichlet(self._h, _ptr(g)))
 
+    def set_solver_graphs(self, on):
+        check(lib.hfx_burgers_set_solver_graphs(self._h, int(bool(on))))
+
     def set_krylov(self, krylov):
         check(lib.hfx_burgers_set_krylov(self._h, int(krylov)))
 

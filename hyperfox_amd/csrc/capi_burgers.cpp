@@ -35,6 +35,7 @@ struct hfx_burgers {
   bool has_mesh = false, has_fields = false, assembled = false;
   int tau_sides = 1, bc_kind = HFX_BC_INTEGRATED_DIRICHLET, krylov = HFX_KRYLOV_FGMRES;
   int model = 0;            // 0 HDGBurgersModel, 1 HDGnGammaModel
+  bool graphs = false;      // hfx_burgers_set_solver_graphs
   double ts_dt = 0.0;       // implicit Euler time step (0: steady)
   hfx::DevBuf<double> old;  // "OldSolution" of the time step
   double sound_speed = 1.0;  // nGammaParams::soundSpeed
@@ -152,6 +153,7 @@ void solve(hfx_burgers* h, int pc, double rtol, int maxits, int restart, hfx::Kr
   o.rtol = rtol;
   o.maxits = maxits;
   o.restart = restart;
+  o.graph = h->graphs;
   if (h->krylov == HFX_KRYLOV_BICGSTAB) hfx::bicgstab(h->A, h->rhs.p, h->trace.p, M, o, info, s);
   else hfx::fgmres(h->A, h->rhs.p, h->trace.p, M, o, info, s);
   const int mu = m.nN * h->m;
@@ -424,6 +426,13 @@ extern "C" int hfx_burgers_set_dirichlet(hfx_burgers* h, const double* dirichlet
     HFX_HIP(hipMemsetAsync(h->g.p, 0, h->n_trace() * sizeof(double), h->stream));
   HFX_HIP(hipStreamSynchronize(h->stream));
   h->assembled = false;
+  HFX_END
+}
+
+extern "C" int hfx_burgers_set_solver_graphs(hfx_burgers* h, int on) {
+  HFX_TRY
+  need(h != nullptr, "hfx_burgers_set_solver_graphs: null handle");
+  h->graphs = on != 0;
   HFX_END
 }
 

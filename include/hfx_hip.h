@@ -544,6 +544,9 @@ int hfx_burgers_set_boundary_model(hfx_burgers* h, int kind);
 int hfx_burgers_set_boundary_hdg(hfx_burgers* h, const int* faces, int n, const double* B, const double* r,
                                  int matrix_set, int rhs_set);
 int hfx_burgers_set_krylov(hfx_burgers* h, int krylov);
+/* BiCGStab's steady iterations replayed as a captured HIP graph (same
+ * kernels, same results; off by default), as hfx_hdg_set_solver_graphs. */
+int hfx_burgers_set_solver_graphs(hfx_burgers* h, int on);
 /* Implicit Euler (HFX_TS_BEULER: RungeKutta's BEuler, RungeKutta.cpp:113-159,
  * as in tests/regression/HDG/TestHDGBurgers.cpp) or none: the u rows gain
  * (M (x) I) / dt and F_u gains (M (x) I) OldSolution / dt.  OldSolution starts

@@ -296,3 +296,22 @@ def test_transient_burgers_steps_match_oracle():
         sol, gbuf, flux, tr = prob.state()
         assert its == its_o and res < 1e-8, step
         assert rel(sol.reshape(nC, -1), u) < 1e-8 and rel(tr.reshape(-1), lam) < 1e-8, step
+
+
+def test_graph_replayed_solver_gives_the_same_fields():
+    """BiCGStab's steady iterations replayed as a HIP graph: same kernels in the
+    same order, so the same trace and solution bit for bit."""
+    print('burgers solver graphs', flush=True)
+    out = []
+    for graphs in (False, True):
+        mesh, fields, state = workload(2, 3)
+        prob = gpu_problem(2, 3, mesh, fields, state)
+        prob.set_krylov(1)
+        prob.set_solver_graphs(graphs)
+        prob.assemble()
+        it, reason, rr = prob.solve(pc=hf.PC_ILU0, rtol=1e-12, maxits=2000)
+        assert reason > 0
+        out.append((it, prob.state()))
+    assert out[0][0] == out[1][0]
+    for a, b in zip(out[0][1], out[1][1]):
+        assert np.array_equal(a, b)
