@@ -73,13 +73,50 @@ def cpu_system(n, p, threads):
     return s, w.mesh.n_cells * orc.sizes(3, p)['nN'], w.mesh.n_cells
 
 
+def host_info(threads):
+    """The host the CPU legs ran on: model name (lscpu), logical CPUs,
+    OMP_NUM_THREADS and the OpenMP threads the oracle was given."""
+    model = None
+    try:
+        import subprocess
+        for line in subprocess.run(['lscpu'], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if line.startswith('Model name:'):
+                model = line.split(':', 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if model is None:
+        try:
+            with open('/proc/cpuinfo') as fh:
+                model = next((ln.split(':', 1)[1].strip() for ln in fh if ln.startswith('model name')), None)
+        except OSError:
+            pass
+    return {'cpu_model': model, 'logical_cpus': os.cpu_count(), 'OMP_NUM_THREADS': os.environ.get('OMP_NUM_THREADS'),
+            'oracle_threads': threads}
+
+
+def oracle_gmres(s, rtol=1e-10):
+    """The reference's trace solve on the oracle's system: GMRES(30) with left
+    point Jacobi (PETSc's KSPGMRES + PCJACOBI as PetscInterface configures
+    them), zero initial guess, stopping on the preconditioned residual."""
+    t = time.perf_counter()
+    lam, its, conv, relres = s.solve(method=0, rtol=rtol, maxits=100000, restart=30)
+    return lam, {'method': 'GMRES(30) + left Jacobi (PETSc semantics)', 'rtol_preconditioned': rtol,
+                 'iters': its, 'converged': conv, 'true_relres': relres, 's': time.perf_counter() - t}
+
+
 def cpu_baseline(n_cpu, p, threads, n_cpu1=8):
     """Oracle (FP64 C++ restatement of the reference path: assembly, Householder
     condensation, Dirichlet rows, CSR scatter) timed on the host cores: on all
     `threads` OpenMP threads over n_cpu^3 x 6 cells (best of 2), and on one core
-    over n_cpu1^3 x 6 cells.  Per-cell cost does not depend on the mesh size."""
+    over n_cpu1^3 x 6 cells.  Per-cell cost does not depend on the mesh size.
+    The trace solve of the same system by the reference's GMRES(30) + Jacobi
+    to 1e-10 is timed beside it."""
     s, dofs, cells = cpu_system(n_cpu, p, threads)
     best = min(s.time_assembly(cells, threads) for _ in range(2))
+    _, solve = oracle_gmres(s)
+    print('# CPU oracle: assembly %.2f s, GMRES(30)+Jacobi %d its %.1f s' % (best, solve['iters'], solve['s']),
+          file=sys.stderr, flush=True)
     del s
     s1, dofs1, cells1 = cpu_system(n_cpu1, p, 1)
     best1 = min(s1.time_assembly(cells1, 1) for _ in range(2))
@@ -88,7 +125,76 @@ def cpu_baseline(n_cpu, p, threads, n_cpu1=8):
                       '%d threads' % (n_cpu, cells, best, threads),
             'value_1core': dofs1 / best1,
             'sample_1core': '%d^3x6 = %d tets on 1 thread, best of 2: %.2f s' % (n_cpu1, cells1, best1),
-            'host_cpus': os.cpu_count()}
+            'trace_solve': solve, 'host': host_info(threads)}
+
+
+def configs_2d(hf, pr, threads, reps=10):
+    """The 2-D diffusion-source configurations at full size: 32 x 32 x 2
+    triangles at p = 2 and 128 x 128 x 2 at p = 4 (jittered 0.1 h, D = 1,
+    u = sin(pi x) sin(pi y)).  Per configuration: GPU assembly + condensation
+    + scatter (HIP events, median of `reps` after 3 warm-ups) as u-DOF/s and
+    trace-DOF/s; GPU trace solves to a true residual of 1e-10; the solution
+    error of the GPU path against the FP64 oracle's own system solved by
+    sparse LU (GPU solved to 1e-13), for lambda, u and q, and both L2 errors
+    against the exact solution; the oracle's assembly rate on the host
+    threads, and on the smaller mesh its GMRES(30) + Jacobi solve."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    sys.path.insert(0, os.path.join(ROOT, 'oracle'))
+    sys.path.insert(0, os.path.join(ROOT, 'tests'))
+    import oracle as orc
+    from hdg_helpers import l2_projection_error
+    out = {}
+    for name, n, p in (('32x32x2_p2', 32, 2), ('128x128x2_p4', 128, 4)):
+        w = pr.diffusion_source_2d(n, p)
+        prob = hf.HDGProblem(2, p)
+        f = pr.setup_gpu(w, prob)
+        prob.assemble()
+        cells, nN = w.mesh.n_cells, prob.sizes['nN']
+        ms = float(np.median(prob.time(1, warmup=3, reps=reps)))
+        kms = float(np.mean(prob.time(0, warmup=1, reps=reps)))
+        solves = {}
+        for method, pc in (('bicgstab', 'ilu0'), ('bicgstab', 'block-jacobi'), ('fgmres30', 'jacobi')):
+            solves['%s+%s' % (method, pc)] = timed_solve(hf, prob, method, pc, 100000)
+        s = orc.System(2, p, w.kind, w.mesh.arrays(), w.tau, tau_sides=w.tau_sides, f_ip=f, dirichlet=w.dirichlet,
+                       nthreads=threads)
+        rp, cols, vals, rhs = s.csr()
+        N = s.ndofs
+        t = time.perf_counter()
+        lo = spla.splu(sp.csr_matrix((vals, cols, rp), shape=(N, N)).tocsc()).solve(rhs)
+        lu_s = time.perf_counter() - t
+        uo, qo = s.reconstruct(lo)
+        prob.set_krylov(hf.KRYLOV_BICGSTAB)
+        its, reason, relres = prob.solve(pc=hf.PC_ILU0, rtol=1e-13, maxits=100000)
+        u, q, lam = prob.solution()
+        a = w.mesh.arrays()
+        ue = pr.u_sin2d(a['coords'][a['cells']])
+
+        def rl(x, y):
+            return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+        row = {'workload': 'HDG 2D diffusion-source p=%d, %d^2x2=%d jittered tris' % (p, n, cells),
+               'cells': cells, 'u_dofs': cells * nN, 'trace_dofs': int(w.mesh.n_trace),
+               'gpu': {'assembly_ms': ms, 'element_kernel_ms': kms, 'dof_per_s': cells * nN / (ms * 1e-3),
+                       'trace_dof_per_s': w.mesh.n_trace / (ms * 1e-3), 'solves_to_1e-10': solves},
+               'vs_fp64_oracle': {'oracle_solve': 'sparse LU (SuperLU) of the oracle trace system, %.1f s' % lu_s,
+                                  'gpu_solve': 'BiCGStab + ILU(0), %d its, true relres %.1e' % (its, relres),
+                                  'lambda_rel_l2': rl(lam, lo), 'u_rel_l2': rl(u, uo), 'q_rel_l2': rl(q, qo)},
+               'l2_error_vs_exact': {'gpu': l2_projection_error(2, p, a, u, ue),
+                                     'oracle': l2_projection_error(2, p, a, uo, ue)}}
+        t_asm = min(s.time_assembly(cells, threads) for _ in range(2))
+        row['cpu'] = {'assembly_s': t_asm, 'dof_per_s': cells * nN / t_asm, 'threads': threads}
+        if n <= 32:
+            _, row['cpu']['trace_solve'] = oracle_gmres(s)
+        else:
+            row['cpu']['trace_solve'] = ('not run in the bench: GMRES(30) + Jacobi needs about 21,000 iterations '
+                                         'on this system (51 s on 8 host threads)')
+        del s
+        out[name] = row
+        print('# %s: GPU %.3f ms/assembly (%.1f M DOF/s); vs oracle lambda %.1e u %.1e q %.1e; L2 err %.3e' % (
+            name, ms, cells * nN / (ms * 1e-3) / 1e6, row['vs_fp64_oracle']['lambda_rel_l2'],
+            row['vs_fp64_oracle']['u_rel_l2'], row['vs_fp64_oracle']['q_rel_l2'], row['l2_error_vs_exact']['gpu']),
+            file=sys.stderr, flush=True)
+    return out
 
 
 PC_NAMES = {'jacobi': 1, 'block-jacobi': 2, 'learned': 3, 'ilu0': 4, 'ilu0-natural': 5, 'ilu0-f32': 6}
@@ -488,6 +594,7 @@ def main():
     ap.add_argument('--no-batch', action='store_true')
     ap.add_argument('--learned-n', type=int, default=256)
     ap.add_argument('--no-burgers', action='store_true')
+    ap.add_argument('--no-2d', action='store_true', help='skip the 2-D diffusion-source configurations')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py: this build runs on exactly one GPU (multi-GPU sharding is designed, not built); '
@@ -564,8 +671,10 @@ def main():
         res['ngamma'] = ngamma(hf)
     if not args.no_batch:
         res['batched'] = batched(hf, pr)
+    threads = min(16, os.cpu_count() or 1)
+    if not args.no_2d:
+        res['configs_2d'] = configs_2d(hf, pr, threads)
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
         res['cpu_baseline'] = cpu_baseline(args.cpu_n, args.p, threads)
     print(json.dumps(res))
 

@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 
 from ._lib import check, lib
-from .hdg import _f64, _ptr
+from .hdg import _f64, _ptr, _sized
 
 BC_DIRICHLET, BC_INTEGRATED_DIRICHLET = 0, 1
 
@@ -35,6 +35,13 @@ class BurgersProblem:
             lib.hfx_burgers_destroy(self._h)
             self._h = None
 
+    def _check_tau(self, t):
+        if self.mesh is None:
+            raise ValueError('set the mesh first')
+        if t.shape[:2] != (self.mesh.n_faces, self.nNf):
+            raise ValueError('tau must be (nFaces, nNf, sides, m, m) = (%d, %d, ...), got %s'
+                             % (self.mesh.n_faces, self.nNf, t.shape))
+
     def set_mesh(self, mesh):
         check(lib.hfx_burgers_set_mesh(self._h, mesh._h))
         self.mesh = mesh
@@ -50,10 +57,12 @@ class BurgersProblem:
         t = np.asarray(tau, dtype=np.float64)
         if t.ndim != 5 or t.shape[3:] != (self.m, self.m) or t.shape[2] not in (1, 2):
             raise ValueError('tau must be (nFaces, nNf, sides, m, m) with sides 1 or 2')
+        self._check_tau(t)
         tcol = _f64(np.swapaxes(t, 3, 4))  # column-major m x m per node
-        Dd = None if D is None else _f64(D)
-        g = None if dirichlet is None else _f64(dirichlet)
-        f = None if source is None else _f64(source)
+        m = self.mesh
+        Dd = _sized(D, m.n_nodes, 'DiffusionTensor')
+        g = _sized(dirichlet, m.n_faces * self.nNf * self.m, 'Dirichlet')
+        f = _sized(source, m.n_cells * self.nIP * self.m, 'source')
         self._keep = (tcol, Dd, g, f)
         check(lib.hfx_burgers_set_fields(self._h, _ptr(tcol), t.shape[2], _ptr(Dd), _ptr(g), _ptr(f)))
 
@@ -62,8 +71,9 @@ class BurgersProblem:
 
     def set_boundary_hdg(self, faces, B, r, matrix_set=True, rhs_set=True):
         """HDG-type boundary model on `faces`: B (n, bs, bs (2 + dim)), r (n, bs)."""
-        f = np.ascontiguousarray(faces, dtype=np.int32)
-        Bd, rd = _f64(B), _f64(r)
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        Bd = _sized(B, f.size * self.bs * self.bs * (2 + self.dim), 'set_boundary_hdg B')
+        rd = _sized(r, f.size * self.bs, 'set_boundary_hdg r')
         self._keep_hdg = (f, Bd, rd)
         check(lib.hfx_burgers_set_boundary_hdg(self._h, _ptr(f), int(f.size), _ptr(Bd), _ptr(rd), int(bool(matrix_set)),
                                                int(bool(rhs_set))))
@@ -77,7 +87,7 @@ class BurgersProblem:
         check(lib.hfx_burgers_start_step(self._h))
 
     def set_dirichlet(self, dirichlet):
-        g = None if dirichlet is None else _f64(dirichlet)
+        g = _sized(dirichlet, self.mesh.n_faces * self.nNf * self.m, 'Dirichlet')
         self._keep_g = g
         check(lib.hfx_burgers_set_dirichlet(self._h, _ptr(g)))
 
@@ -88,7 +98,9 @@ class BurgersProblem:
         check(lib.hfx_burgers_set_krylov(self._h, int(krylov)))
 
     def set_state(self, sol=None, buffer=None, trace=None):
-        a = [None if v is None else _f64(v) for v in (sol, buffer, trace)]
+        nC, nF = self.mesh.n_cells, self.mesh.n_faces
+        a = [_sized(sol, nC * self.nN * self.m, 'Solution'), _sized(buffer, nC * self.nN * self.m, 'BufferSolution'),
+             _sized(trace, nF * self.nNf * self.m, 'Trace')]
         check(lib.hfx_burgers_set_state(self._h, *[_ptr(v) for v in a]))
 
     def state(self):
@@ -154,8 +166,11 @@ class NGammaProblem(BurgersProblem):
         t = np.asarray(tau, dtype=np.float64)
         if t.ndim != 5 or t.shape[3:] != (2, 2) or t.shape[2] not in (1, 2):
             raise ValueError('tau must be (nFaces, nNf, sides, 2, 2) with sides 1 or 2')
+        self._check_tau(t)
         tcol = _f64(np.swapaxes(t, 3, 4))
-        arrs = [None if v is None else _f64(v) for v in (D, G, b, dirichlet, source)]
+        m = self.mesh
+        arrs = [_sized(D, m.n_nodes, 'D'), _sized(G, m.n_nodes, 'G'), _sized(b, m.n_nodes * 2, 'Velocity'),
+                _sized(dirichlet, m.n_faces * self.nNf * 2, 'Dirichlet'), _sized(source, m.n_cells * self.nIP * 2, 'source')]
         self._keep = (tcol, *arrs)
         Dd, Gd, bd, g, f = arrs
         check(lib.hfx_ngamma_set_fields(self._h, _ptr(tcol), t.shape[2], _ptr(Dd), _ptr(Gd), _ptr(bd),

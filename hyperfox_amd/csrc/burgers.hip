@@ -589,12 +589,8 @@ template <int D, int P, int MDL>
 void launch_dp(const BurgersArgs& a, hipStream_t s) {
   const size_t lds = (size_t)BLay<D, P, MDL>::TOTAL * sizeof(double);
   if (lds > 150 * 1024) throw std::runtime_error("Burgers kernel: LDS footprint above 150 KB");
-  static bool attr = false;
-  if (!attr) {
-    HFX_HIP(hipFuncSetAttribute((const void*)burgers_cell_kernel<D, P, MDL>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr = true;
-  }
+  static PerDeviceLaunch once;
+  once.lds_attr((const void*)burgers_cell_kernel<D, P, MDL>, lds);
   const int grid = a.nEl < 65536 ? a.nEl : 65536;
   hipLaunchKernelGGL((burgers_cell_kernel<D, P, MDL>), dim3(grid), dim3(BT), lds, s, a);
   HFX_HIP(hipGetLastError());

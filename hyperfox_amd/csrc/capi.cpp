@@ -372,6 +372,10 @@ extern "C" int hfx_hdg_set_model(hfx_hdg* h, int kind) {
   HFX_TRY
   require(h, "hfx_hdg_set_model: null handle");
   require(kind >= 0 && kind <= 2, "hfx_hdg_set_model: unknown model kind");
+  // Reaction is an operator of HDGConvectionDiffusionReactionSource only
+  // (HDGConvectionDiffusionReactionSource.cpp:90-95); the other models have none
+  require(kind == HFX_MODEL_CDR_SOURCE || !h->hasR,
+          "hfx_hdg_set_model: a reaction is set; only HDGConvectionDiffusionReactionSource has a Reaction operator");
   h->kind = kind;
   h->assembled = h->solved = false;
   ++h->args_version;
@@ -421,6 +425,9 @@ extern "C" int hfx_hdg_ip_coords(hfx_hdg* h, double* out) {
 extern "C" int hfx_hdg_set_source(hfx_hdg* h, const double* f_ip, const double* r_ip) {
   HFX_TRY
   require(h && h->has_mesh, "hfx_hdg_set_source: set the mesh first");
+  require(r_ip == nullptr || h->kind == HFX_MODEL_CDR_SOURCE,
+          "hfx_hdg_set_source: only HDGConvectionDiffusionReactionSource has a Reaction operator "
+          "(set that model kind first)");
   h->set_device();
   const size_t n = (size_t)h->mesh.nCells * h->re->nip();
   h->hasF = f_ip != nullptr;

@@ -67,8 +67,8 @@ struct ALay {
   static constexpr int NSF = NS;                  // LDS face masses: tau, b.n, D
   static constexpr int O_FM = 0;
   static constexpr int O_SUQ = O_FM + NF * NSF * NNF * NNF;
-  static constexpr int O_KM = O_SUQ + D * NN * NN;  // Suu, then K^-1
-  // after the K^-1 solve, Suq and K are dead: the face traces n.q of the Q
+  static constexpr int O_KM = O_SUQ + D * NN * NN;  // Suu, then K, then its LU factors
+  // after the U solve, Suq and the K factors are dead: the face traces n.q of the Q
   // columns take their place
   static constexpr int NQN = NF * NNF * NCOL;
   static constexpr int END_SK = O_KM + NN * NN;
@@ -76,7 +76,7 @@ struct ALay {
   static constexpr int O_FG = O_FU + NN;          // per face: n (D), |f|
   static constexpr int O_VX = O_FG + NF * (D + 1);
   static constexpr int O_FWT = O_VX + (D + 1) * D;  // face-IP weights (kept for the b.n columns)
-  static constexpr int O_RD = O_FWT + NF * NIPF * NS;  // 1 / pivot of each K^-1 row
+  static constexpr int O_RD = O_FWT + NF * NIPF * NS;  // 1 / pivot of each row of U (K = P^T L U)
   // per-cell indices (ints) for the fused trace scatter, gathered with the
   // cell: face-node positions, BSR block of each face pair, face ids, side |
   // Dirichlet flags
@@ -85,7 +85,8 @@ struct ALay {
   static constexpr int IX_FACE = IX_BPOS + NF * NF;
   static constexpr int IX_FLAG = IX_FACE + NF;
   static constexpr int NIX = IX_FLAG + NF;
-  static constexpr int O_IX = O_RD + NN;
+  static constexpr int O_PL = O_RD + NN;             // pivot row of each LU step (ints)
+  static constexpr int O_IX = O_PL + (NN + 1) / 2;
   static constexpr int O_OV = O_IX + (NIX + 1) / 2;
   // overlay, phase 1
   static constexpr int O_DN = O_OV;
@@ -548,16 +549,19 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           }
     }
     wave_sync();
-    // K^-1 by Gauss-Jordan with partial pivoting in exchange form, in place:
-    // one row of K per lane in registers, the pivot row fetched with readlane.
-    // Pivoting on (p, k) with d = K[p][k] makes row p express unknown k:
-    //   K[p][k] <- 1/d, K[p][j] <- -K[p][j]/d, K[i][k] <- K[i][k]/d,
-    //   K[i][j] <- K[i][j] - K[i][k] K[p][j]/d   (i != p, j != k);
-    // after all NN pivots, K^-1[k][p(j)] = K[p(k)][j] with p(k) the pivot row
-    // of column k.  Half the row length of [K | I] elimination.  Each lane
-    // updates its row as one branch-free FMA per entry,
-    //   K[i][j] <- mlt K[p][j] + own K[i][j],
-    // with (mlt, own) = (-K[i][k]/d, 1) off the pivot row and (0, -1/d) on it.
+    // K = P^T L U by Gaussian elimination with partial pivoting, in place: one
+    // row of K per lane in registers, the pivot row fetched with readlane.  At
+    // step k the pivot p is the largest |K[i][k]| among the rows not chosen
+    // yet (DPP argmax); every other such row takes l = K[i][k] / K[p][k],
+    // K[i][j] -= l K[p][j] for j > k and keeps l in K[i][k].  Row p is then
+    // row k of U (with the multipliers of the earlier steps to its left) and
+    // is stored as row k of KM, its reciprocal pivot in RD[k].  The R columns
+    // are solved against these factors below: forward and back substitution
+    // is backward stable, whereas multiplying by an explicitly formed K^-1
+    // leaves an error in U that does not cancel in S = Slu U + Slq Q + Sll
+    // (a relative 1e-12 in S for the 2-D p = 4 cells of a 128 x 128 mesh,
+    // where cond(K) ~ 3e4).
+    int* PL = reinterpret_cast<int*>(ws + L::O_PL);
     {
       double kr[NN];
       const int rl = lane < NN ? lane : 0;
@@ -566,29 +570,28 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       static_assert(NN <= 32, "pivot search covers lanes 0..31");
       unsigned long long used = 0ull;
       int myk = 0;
-      int pl[NN];
+      double myrd = 0.0;
 #pragma unroll
       for (int k = 0; k < NN; ++k) {
         const bool cand = lane < NN && !((used >> lane) & 1ull);
         const int p = __builtin_amdgcn_readfirstlane(dpp_argmax(fabs(kr[k]), cand));
         used |= 1ull << p;
-        pl[k] = p;
-        const bool piv = lane == p;
-        if (piv) myk = k;
-        const double aik = kr[k];
-        const double dinv = 1.0 / readlane_d(aik, p);
-        const double mlt = piv ? 0.0 : -aik * dinv;
-        const double own = piv ? -dinv : 1.0;
-#pragma unroll
-        for (int j = 0; j < NN; ++j) {
-          if (j == k) continue;
-          kr[j] = fma(mlt, readlane_d(kr[j], p), own * kr[j]);
+        if (lane == 0) PL[k] = p;
+        const bool elim = cand && lane != p;
+        const double rp = 1.0 / readlane_d(kr[k], p);
+        if (lane == p) {
+          myk = k;
+          myrd = rp;
         }
-        kr[k] = piv ? dinv : aik * dinv;
+        const double l = elim ? kr[k] * rp : 0.0;
+#pragma unroll
+        for (int j = k + 1; j < NN; ++j) kr[j] = fma(-l, readlane_d(kr[j], p), kr[j]);
+        kr[k] = elim ? l : kr[k];
       }
       if (lane < NN) {
 #pragma unroll
-        for (int j = 0; j < NN; ++j) KM[myk * NN + pl[j]] = kr[j];
+        for (int j = 0; j < NN; ++j) KM[myk * NN + j] = kr[j];
+        ws[L::O_RD + myk] = myrd;
       }
     }
     wave_sync();
@@ -600,7 +603,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     const bool tr = c < NL;
     const int g = tr ? c / NNF : 0, bp = tr ? c % NNF : 0;
     double* UB = RB;               // U columns replace R columns lane by lane
-    double* QN = ws + L::O_SUQ;    // face traces of n.q, over the dead Suq and K
+    double* QN = ws + L::O_SUQ;    // face traces of n.q, over the dead Suq and K factors
     // R[:, c] = Sul[:, c] - sum_k Suq_k B_k[:, c] (or -F_u on lane NL), with
     // B_k[:, (g,b)] = -rho_g n_gk G_g[:, b] and sum_k n_gk Suq_k = sum_a nt_a SUQ_a
     if (c <= NL) {
@@ -675,46 +678,36 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           }
     }
     wave_sync();
-    // U = -K^-1 R on FP64 MFMA, in place: every tile is accumulated before
-    // the first U value overwrites R
-    {
-      // u = -K^-1 R[:, c]
-      constexpr int MT = (NN + 15) / 16, NT = (NCOL + 15) / 16, KS = (NN + 3) / 4;
-      const int li = lane & 15, lk = lane >> 4;
-      dvec4 acc[MT][NT];
+    // U = -K^-1 R: lane c <= NL solves its column against the LU factors,
+    // y = L^-1 P r, then x = U^-1 y, with the factors read at uniform
+    // (broadcast) LDS addresses; the column is read whole before it is
+    // overwritten, so the solve is in place
+    if (c <= NL) {
+      const double* RDp = ws + L::O_RD;
+      double y[NN];
 #pragma unroll
-      for (int m = 0; m < MT; ++m)
+      for (int k = 0; k < NN; ++k) y[k] = RB[PL[k] * NCOL + c];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[m][nt] = dvec4{0.0, 0.0, 0.0, 0.0};
+      for (int k = 1; k < NN; ++k) {
+        double sacc = y[k];
 #pragma unroll
-      for (int ks = 0; ks < KS; ++ks) {
-        const int j = ks * 4 + lk;
-        const bool kv = j < NN;
-        double bv[NT];
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          const int cb = nt * 16 + li;
-          bv[nt] = (kv && cb < NCOL) ? RB[j * NCOL + cb] : 0.0;
-        }
-#pragma unroll
-        for (int m = 0; m < MT; ++m) {
-          const int i = m * 16 + li;
-          const double av = (kv && i < NN) ? KM[i * NN + j] : 0.0;
-#pragma unroll
-          for (int nt = 0; nt < NT; ++nt) acc[m][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv[nt], acc[m][nt], 0, 0, 0);
-        }
+        for (int j = 0; j < NN; ++j)
+          if (j < k) sacc = fma(-KM[k * NN + j], y[j], sacc);
+        y[k] = sacc;
       }
 #pragma unroll
-      for (int m = 0; m < MT; ++m)
+      for (int kk = 0; kk < NN; ++kk) {
+        const int k = NN - 1 - kk;
+        double sacc = y[k];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
+        for (int j = 0; j < NN; ++j)
+          if (j > k) sacc = fma(-KM[k * NN + j], y[j], sacc);
+        y[k] = sacc * RDp[k];
+      }
 #pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int i = m * 16 + 4 * r + lk, cc = nt * 16 + li;
-            if (i < NN && cc < NCOL) UB[i * NCOL + cc] = -acc[m][nt][r];
-          }
+      for (int i = 0; i < NN; ++i) UB[i * NCOL + c] = -y[i];
     }
-    wave_sync();  // every lane is done with Suq and K^-1 before QN overwrites them
+    wave_sync();  // every lane is done with Suq and the K factors before QN overwrites them
     // Q_k = -A_k U - B_k on FP64 MFMA: Y_a = Ahat_a U (NN x NN times NN x NCOL),
     // then per entry q_k = -sum_a Jinv[k][a] Y_a + n_gk rho_g G_g[i][bp];
     // writes U, Q (U0, Q0 for column NL) and keeps n_f . q on the faces (QN)
@@ -926,23 +919,13 @@ void launch_aff(const ElemArgs& a, int grid_cap, hipStream_t s) {
   using L = ALay<D, P, TD>;
   const size_t lds = (size_t)(L::TAB + W * L::PER) * sizeof(double);
   auto kern = hdg_affine_kernel<D, P, TD, W, TSG>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    HFX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
+  static PerDeviceLaunch once;
+  once.lds_attr((const void*)kern, lds);
   // persistent grid by default: as many workgroups as are resident at once
   // (the LDS allows one per CU for 3-D p=3), each looping over its cells, so
   // that the reference tables are staged once per workgroup instead of once
   // per W cells (C3: 5.28 -> 4.98 ms)
-  static int resident = 0;
-  if (resident == 0) {
-    int dev = 0, cus = 0, per = 0;
-    HFX_HIP(hipGetDevice(&dev));
-    HFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HFX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)kern, 64 * W, lds));
-    resident = (per > 0 ? per : 1) * (cus > 0 ? cus : 1);
-  }
+  const int resident = once.resident_wgs((const void*)kern, 64 * W, lds);
   long long grid = ((long long)a.nEl + W - 1) / W;
   const long long cap = grid_cap > 0 ? grid_cap : resident;
   if (grid > cap) grid = cap;
@@ -1032,12 +1015,10 @@ void launch_affine_batch(int order, int waves, const ElemArgs* probs, const Batc
   if (lds > 160 * 1024) throw std::runtime_error("batched element kernel: LDS exceeds 160 KB for this wave count");
   if (grid <= 0) throw std::runtime_error("batched element kernel: empty grid");
   auto go = [&](auto kern) {
-    // the LDS limit is raised once, outside any stream capture (prepare_affine_batch)
-    static bool attr_set = false;
-    if (!attr_set) {
-      HFX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      attr_set = true;
-    }
+    // the LDS limit is raised once per device, outside any stream capture
+    // (prepare_affine_batch)
+    static PerDeviceLaunch once;
+    once.lds_attr((const void*)kern, lds);
     if (attribute_only) return;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(64 * waves), lds, s, probs, items, nitems);
     HFX_HIP(hipGetLastError());

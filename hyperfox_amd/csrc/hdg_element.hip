@@ -503,11 +503,8 @@ void launch_t(const ElemArgs& a, int grid_cap, hipStream_t s) {
   using L = Layout<D, P, NT>;
   const size_t lds = (size_t)L::TOTAL * sizeof(double);
   auto kern = hdg_element_kernel<D, P, NT>;
-  static bool attr_set = false;
-  if (!attr_set) {
-    HFX_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    attr_set = true;
-  }
+  static PerDeviceLaunch once;
+  once.lds_attr((const void*)kern, lds);
   int grid = a.nEl;
   if (grid_cap > 0 && grid > grid_cap) grid = grid_cap;
   if (grid <= 0) return;

@@ -204,7 +204,10 @@ void sweeps(const Ilu& f, const double* x, double* y, double* z, hipStream_t s) 
 }  // namespace
 
 void Ilu::setup(int nb_, int bs_, const int* rowptr_host, const int* col_host, int ordering_, hipStream_t s) {
-  if (bs_ < 1 || bs_ > 16) throw std::runtime_error("ILU(0): block size must be 1..16");
+  // the block sizes the sweeps of apply() are instantiated for; refused here,
+  // before the analysis and factorisation, rather than at the first apply
+  if (!(bs_ >= 1 && bs_ <= 12 && bs_ != 7 && bs_ != 11))
+    throw std::runtime_error("ILU(0): block size must be one of 1-6, 8, 9, 10, 12");
   plan = ilu_analyse(nb_, rowptr_host, col_host, ordering_);
   nb = nb_;
   bs = bs_;

@@ -29,6 +29,17 @@ def _i32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
 
 
+def _sized(a, n, name):
+    """a as a contiguous FP64 array of exactly n values (None stays None);
+    the C side reads n values, so any other size is refused here."""
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if a.size != n:
+        raise ValueError('%s: expected %d values, got %d (shape %s)' % (name, n, a.size, a.shape))
+    return a
+
+
 def refel_sizes(dim, order):
     out = np.zeros(7, dtype=np.int32)
     check(lib.hfx_refel_sizes(dim, order, _ptr(out)))
@@ -139,11 +150,22 @@ class HDGProblem:
         check(lib.hfx_hdg_set_mesh(self._h, mesh._h))
         self.mesh = mesh
 
+    def _need_mesh(self, what):
+        if self.mesh is None:
+            raise ValueError('%s: set the mesh first' % what)
+        return self.mesh
+
     def set_model(self, kind):
         check(lib.hfx_hdg_set_model(self._h, kind))
 
     def set_fields(self, tau, tau_sides=1, D=None, ndD=0, b=None, dirichlet=None):
-        self._keep = [_f64(tau), _f64(D), _f64(b), _f64(dirichlet)]
+        m, nNf = self._need_mesh('set_fields'), self.sizes['nNf']
+        if tau is None:
+            raise ValueError('set_fields: the field map must have a Tau field')
+        self._keep = [_sized(tau, m.n_faces * nNf * tau_sides, 'Tau'),
+                      _sized(D, m.n_nodes * ndD, 'DiffusionTensor') if D is not None and ndD > 0 else _f64(D),
+                      _sized(b, m.n_nodes * self.dim, 'Velocity'),
+                      _sized(dirichlet, m.n_faces * nNf, 'Dirichlet')]
         t, d, bb, g = self._keep
         check(lib.hfx_hdg_set_fields(self._h, _ptr(t), tau_sides, _ptr(d), ndD if d is not None else 0,
                                      _ptr(bb), _ptr(g)))
@@ -154,8 +176,9 @@ class HDGProblem:
         return out
 
     def set_source(self, f_ip=None, r_ip=None):
-        f = _f64(f_ip)
-        r = _f64(r_ip)
+        n = self._need_mesh('set_source').n_cells * self.sizes['nIP']
+        f = _sized(f_ip, n, 'source')
+        r = _sized(r_ip, n, 'reaction')
         check(lib.hfx_hdg_set_source(self._h, _ptr(f), _ptr(r)))
 
     def set_launch(self, threads=0, grid_cap=0):
@@ -262,20 +285,21 @@ class HDGProblem:
         """HDG-type boundary model on the listed boundary faces: B (n, nNf,
         nNf (2 + dim)) = [B_u | B_q | B_lambda] in face-node order, r (n,
         nNf), Set or Add assembly; an empty list clears it."""
-        f = np.ascontiguousarray(faces, dtype=np.int32)
-        B = _f64(B)
-        r = _f64(r)
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        nNf = self.sizes['nNf']
+        B = _sized(B, f.size * nNf * nNf * (2 + self.dim), 'set_boundary_hdg B')
+        r = _sized(r, f.size * nNf, 'set_boundary_hdg r')
         check(lib.hfx_hdg_set_boundary_hdg(self._h, _ptr(f), f.size, _ptr(B), _ptr(r), int(bool(matrix_set)),
                                            int(bool(rhs_set))))
 
     def set_trace(self, lam):
         """Set the trace field (nFaces x nNf)."""
-        lam = _f64(lam)
+        lam = _sized(lam, self._need_mesh('set_trace').n_trace, 'Trace')
         check(lib.hfx_hdg_set_trace(self._h, _ptr(lam)))
 
     def set_solution(self, u):
         """Set the cell-wise nodal solution field (nCells x nN)."""
-        u = _f64(u)
+        u = _sized(u, self._need_mesh('set_solution').n_cells * self.sizes['nN'], 'Solution')
         check(lib.hfx_hdg_set_solution(self._h, _ptr(u)))
 
     def set_solver_type(self, kind):
@@ -284,7 +308,7 @@ class HDGProblem:
 
     def set_flux(self, q):
         """Set the cell-wise nodal flux field (nCells x nN*dim)."""
-        q = _f64(q)
+        q = _sized(q, self._need_mesh('set_flux').n_cells * self.sizes['nN'] * self.dim, 'Flux')
         check(lib.hfx_hdg_set_flux(self._h, _ptr(q)))
 
     def set_fused_scatter(self, on):
@@ -309,7 +333,7 @@ class HDGProblem:
 
     def pc_apply(self, pc, x):
         """y = M^-1 x for preconditioner `pc` set up on the assembled system."""
-        x = _f64(x)
+        x = _sized(x, self._need_mesh('pc_apply').n_trace, 'pc_apply x')
         y = np.zeros_like(x)
         check(lib.hfx_hdg_pc_apply(self._h, pc, _ptr(x), _ptr(y)))
         return y

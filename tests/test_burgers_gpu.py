@@ -150,7 +150,7 @@ def test_stationary_regression_case_matches_oracle(p, damp):
 @pytest.mark.parametrize('pc', [hf.PC_BJACOBI, hf.PC_JACOBI])
 def test_3d_p2_solve_matches_oracle(pc):
     """Block size 18 (3-D p = 2, m = 3): the Jacobi preconditioners and the
-    Krylov kernels at that size; ILU(0) stops at block size 16 with an error."""
+    Krylov kernels at that size; ILU(0) refuses the block size at set-up."""
     print('burgers solve 3-D p=2', pc, flush=True)
     mesh, fields, state = workload(3, 2)
     prob = gpu_problem(3, 2, mesh, fields, state)
@@ -164,7 +164,7 @@ def test_3d_p2_solve_matches_oracle(pc):
     u, q = orc.reconstruct(lam)
     assert rel(tr.reshape(-1), lam) < 1e-9 and rel(sol.reshape(mesh.n_cells, -1), u) < 1e-9
     assert rel(flux.reshape(mesh.n_cells, -1), q) < 1e-9
-    with pytest.raises(hf.HfxError, match='block size must be 1..16'):
+    with pytest.raises(hf.HfxError, match='block size must be one of'):
         prob.solve(pc=hf.PC_ILU0)
 
 
@@ -300,11 +300,13 @@ def test_transient_burgers_steps_match_oracle():
 
 def test_graph_replayed_solver_gives_the_same_fields():
     """BiCGStab's steady iterations replayed as a HIP graph: same kernels in the
-    same order, so the same trace and solution bit for bit."""
+    same order, so the same trace and solution bit for bit.  The mesh is
+    large enough that the solve runs past the first residual check (8
+    iterations) and restarts, so graphs are captured and replayed."""
     print('burgers solver graphs', flush=True)
     out = []
     for graphs in (False, True):
-        mesh, fields, state = workload(2, 3)
+        mesh, fields, state = workload(2, 3, n=12)
         prob = gpu_problem(2, 3, mesh, fields, state)
         prob.set_krylov(1)
         prob.set_solver_graphs(graphs)
@@ -312,6 +314,6 @@ def test_graph_replayed_solver_gives_the_same_fields():
         it, reason, rr = prob.solve(pc=hf.PC_ILU0, rtol=1e-12, maxits=2000)
         assert reason > 0
         out.append((it, prob.state()))
-    assert out[0][0] == out[1][0]
+    assert out[0][0] == out[1][0] and out[0][0] > 16, out[0][0]
     for a, b in zip(out[0][1], out[1][1]):
         assert np.array_equal(a, b)

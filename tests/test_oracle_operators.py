@@ -23,6 +23,15 @@ PMAX = {2: 5, 3: 3}
 CASES = [(d, p, e) for d in (2, 3) for p in range(1, PMAX[d] + 1) for e in ELEMS[d]]
 
 
+def nonsymmetric_D(dim):
+    """A constant diffusion tensor with D != D^T (positive definite symmetric
+    part), so that reading the column-major node values transposed
+    (HDGConvectionDiffusionReactionSource.cpp:122-124) changes every D term."""
+    Dm = np.diag(np.arange(1, dim + 1, dtype=float))
+    Dm += 0.3 * np.triu(np.ones((dim, dim)), 1) - 0.2 * np.tril(np.ones((dim, dim)), -1)
+    return Dm
+
+
 def _tau(dim, p, val=1.0):
     s = orc.sizes(dim, p)
     return np.full(s['nF'] * s['nNf'], val)
@@ -57,8 +66,7 @@ def test_face_blocks(dim, p, el):
     """tau, normal and D.n face terms (HDGBase.cpp:106-137, HDGDiffusion.cpp:109-128)."""
     V = ELEMS[dim][el]
     X = element_nodes(dim, p, V)
-    Dm = np.diag(np.arange(1, dim + 1, dtype=float)) + 0.3
-    Dm = (Dm + Dm.T) / 2
+    Dm = nonsymmetric_D(dim)
     Dnodes = np.tile(Dm.T.reshape(-1), (X.shape[0], 1))  # column-major per node
     # constant tau on each face keeps every integrand within the face rule's
     # exactness (degree 2p); variable tau is covered by the GPU parity tests
@@ -167,3 +175,87 @@ def test_condensation_against_monolithic_solve(dim, p, el):
     assert np.allclose(c['Q'] @ lam + c['Q0'], uq[nN:], rtol=1e-11, atol=1e-11)
     lrow = K[nUQ:, :nUQ] @ uq + K[nUQ:, nUQ:] @ lam - F[nUQ:]
     assert np.allclose(c['S'] @ lam - c['S0'], lrow, rtol=1e-10, atol=1e-11)
+
+
+@pytest.mark.parametrize('dim,p,el', CASES)
+def test_nonsymmetric_diffusion_volume(dim, p, el):
+    """Suq_k = int (D grad u)_k v - sum_f int (D n)_k u v with a constant,
+    non-symmetric D given column-major per node (HDGDiffusion.cpp:129-144 with
+    the field layout of HDGConvectionDiffusionReactionSource.cpp:122-124):
+    the transposed tensor gives different values, which this bar separates."""
+    V = ELEMS[dim][el]
+    X = element_nodes(dim, p, V)
+    Dm = nonsymmetric_D(dim)
+    Dnodes = np.tile(Dm.T.reshape(-1), (X.shape[0], 1))
+    K, _ = orc.local_system(dim, p, 1, X, D=Dnodes, ndD=dim * dim, tau=_tau(dim, p, 0.0))
+    B = Blocks(dim, p)
+    o = 2 * p
+    mons = monomials(dim, p)
+    gap = scale = 0.0
+    for eu in (mons[-1], mons[1]):
+        for ev in mons[1:]:
+            u, v = mono(eu)(X), mono(ev)(X)
+            for k in range(dim):
+                Suq = K[np.ix_(B.u(), B.q(k))]
+
+                def exact(D):
+                    ex = integrate(V, lambda x: sum(D[k, c] * dmono(eu, c)(x) for c in range(dim)) * mono(ev)(x), o)
+                    for f in range(B.nF):
+                        fv, _ = face_vertex_coords(dim, p, V, f)
+                        n = outward_normal(V, fv)
+                        ex -= integrate_face(fv, lambda x: (D @ n)[k] * mono(eu)(x) * mono(ev)(x), o)
+                    return ex
+                ex, ex_t = exact(Dm), exact(Dm.T)
+                assert abs(u @ Suq @ v - ex) < 1e-11 * max(1, abs(ex))
+                gap = max(gap, abs(ex - ex_t))
+                scale = max(scale, abs(ex))
+    assert gap > 1e-3 * scale   # the transposed tensor fails the bar above somewhere
+
+
+@pytest.mark.parametrize('dim,p,el', CASES)
+def test_reaction_block(dim, p, el):
+    """Reaction (Reaction.cpp:5-36) is a Mass with the measure r dV, added to
+    the u-u block of HDGConvectionDiffusionReactionSource only
+    (HDGConvectionDiffusionReactionSource.cpp:90-95).  With r(x) = r0 + g.x
+    evaluated at the element IPs: the local system with r minus the one
+    without is zero outside Suu and in F, u^T dSuu v = int r u v for
+    monomials with deg(u v) <= 2p - 1 (the integrand then stays within the
+    rule's degree 2p), and a constant r gives r0 times the mass block.  The
+    other models have no Reaction operator and refuse r."""
+    V = ELEMS[dim][el]
+    X = element_nodes(dim, p, V)
+    nN = X.shape[0]
+    rng = np.random.default_rng(7 * dim + p)
+    s = orc.sizes(dim, p)
+    tau = rng.uniform(0.5, 2.0, s['nF'] * s['nNf'])
+    b = rng.normal(size=(nN, dim)) * 0.3
+    Dn = rng.uniform(0.5, 1.5, nN)
+    f = rng.normal(size=s['nIP'])
+    xip = orc.ip_coords(dim, p, X)
+    g = np.array([0.6, -0.9, 0.4][:dim])
+    r0 = 1.3
+    args = dict(D=Dn, ndD=1, b=b, tau=tau, f_ip=f)
+    K0, F0 = orc.local_system(dim, p, 2, X, **args)
+    K1, F1 = orc.local_system(dim, p, 2, X, r_ip=r0 + xip @ g, **args)
+    Kc, _ = orc.local_system(dim, p, 2, X, r_ip=np.full(s['nIP'], r0), **args)
+    B = Blocks(dim, p)
+    dK = K1 - K0
+    outside = dK.copy()
+    outside[np.ix_(B.u(), B.u())] = 0.0
+    assert np.all(outside == 0.0) and np.array_equal(F1, F0)
+    dS = dK[np.ix_(B.u(), B.u())]
+    M = K0[np.ix_(B.q(0), B.q(0))]
+    uu = np.ix_(B.u(), B.u())
+    # the difference cancels the (larger) convection and tau entries of Suu:
+    # compare at their round-off
+    assert np.allclose((Kc - K0)[uu], r0 * M, rtol=0, atol=1e-14 * np.abs(Kc[uu]).max())
+    mons = monomials(dim, p)
+    for eu in mons:
+        for ev in mons:
+            if eu.sum() + ev.sum() > 2 * p - 1:
+                continue
+            ex = integrate(V, lambda x: (r0 + x @ g) * mono(eu)(x) * mono(ev)(x), 2 * p)
+            assert abs(mono(eu)(X) @ dS @ mono(ev)(X) - ex) < 1e-12 * max(1, abs(ex))
+    for kind in (0, 1):
+        with pytest.raises(orc.OracleError):
+            orc.local_system(dim, p, kind, X, tau=tau, r_ip=r0 + xip @ g)
