@@ -49,6 +49,7 @@ _SIGS = {
     'hfx_hdg_trace_sizes': ([_P, _P], _I),
     'hfx_hdg_get_trace_csr': ([_P] * 5, _I),
     'hfx_hdg_solve': ([_P, _I, _D, _I, _I, _P, _P], _I),
+    'hfx_hdg_solve_noalloc': ([_P, _I, _D, _I, _P, _P], _I),
     'hfx_hdg_pc_apply': ([_P, _I, _P, _P], _I),
     'hfx_hdg_set_fused_scatter': ([_P, _I], _I),
     'hfx_hdg_set_time_scheme': ([_P, _I, _D], _I),

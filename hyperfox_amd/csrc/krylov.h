@@ -49,11 +49,18 @@ void bsr_inv_point_diag(const BsrDev& A, double* dinv, hipStream_t s);
 void apply_block_diag(int nb, int bs, const double* dinv, const double* x, double* y, hipStream_t s);
 void apply_point_diag(long long n, const double* dinv, const double* x, double* y, hipStream_t s);
 
-// Deterministic FP64 dot / norm helpers (two-stage reductions).
+// Deterministic FP64 dot / norm helpers (two-stage reductions).  The
+// partials live in buffers of its own (init(n)) or in caller memory of
+// Reducer::doubles(n) values (init(n, mem)).
 struct Reducer {
-  DevBuf<double> partial, out;
+  DevBuf<double> partial_buf, out_buf;
+  double* partial = nullptr;  // grid * 66
+  double* out = nullptr;      // 132
   int grid = 0;
+  static int grid_for(long long n);
+  static size_t doubles(long long n) { return (size_t)grid_for(n) * 66 + 132; }
   void init(long long n);
+  void init(long long n, double* mem);
   // out[j] = V_j . w for j < k (V column-major, leading dim ldv); result on device in out.p
   void multidot(long long n, int k, const double* V, long long ldv, const double* w, hipStream_t s);
   // w -= sum_j h[j] V_j ; then out[k] = ||w||^2 partial-reduced (device h, k <= 64)
@@ -71,7 +78,13 @@ void fgmres(const BsrDev& A, const double* b, double* x, const std::function<voi
 // trip per iteration; the recurrence residual is read back every 8
 // iterations).  Convergence is confirmed on the true FP64 residual; a
 // recurrence that converges early or breaks down restarts from it.
+//
+// ws: null (the solver allocates its vectors), or caller device memory of
+// bicgstab_ws_doubles(n) values, 256-byte aligned, which then holds every
+// vector, scalar and partial sum of the solve (nothing is allocated; the
+// graph option must be off).
 void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<void(const double*, double*)>& precond,
-              const KrylovOpts& o, KrylovInfo& info, hipStream_t s);
+              const KrylovOpts& o, KrylovInfo& info, hipStream_t s, double* ws = nullptr);
+size_t bicgstab_ws_doubles(long long n);
 
 }  // namespace hfx

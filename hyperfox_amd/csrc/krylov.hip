@@ -385,36 +385,52 @@ void apply_point_diag(long long n, const double* dinv, const double* x, double* 
   HFX_HIP(hipGetLastError());
 }
 
-void Reducer::init(long long n) {
+// round a count of doubles up to 256 bytes (caller workspaces are carved in
+// 256-byte aligned pieces)
+static inline size_t al32(size_t k) { return (k + 31) & ~(size_t)31; }
+
+int Reducer::grid_for(long long n) {
   long long g = (n + RT * 8 - 1) / (RT * 8);
   if (g < 1) g = 1;
   if (g > 1024) g = 1024;
-  grid = (int)g;
-  partial.alloc((size_t)grid * 66);
-  out.alloc(132);
+  return (int)g;
+}
+
+void Reducer::init(long long n) {
+  grid = grid_for(n);
+  partial_buf.alloc((size_t)grid * 66);
+  out_buf.alloc(132);
+  partial = partial_buf.p;
+  out = out_buf.p;
+}
+
+void Reducer::init(long long n, double* mem) {
+  grid = grid_for(n);
+  partial = mem;
+  out = mem + (size_t)grid * 66;
 }
 
 void Reducer::multidot(long long n, int k, const double* V, long long ldv, const double* w, hipStream_t s) {
   if (k > 64) throw std::runtime_error("multidot: at most 64 vectors");
-  hipLaunchKernelGGL(multidot_kernel, dim3(grid, k), dim3(RT), 0, s, n, V, ldv, w, partial.p);
+  hipLaunchKernelGGL(multidot_kernel, dim3(grid, k), dim3(RT), 0, s, n, V, ldv, w, partial);
   HFX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(reduce_rows_kernel, dim3(k), dim3(RT), 0, s, grid, partial.p, out.p);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(k), dim3(RT), 0, s, grid, partial, out);
   HFX_HIP(hipGetLastError());
 }
 
 void Reducer::update_norm(long long n, int k, const double* V, long long ldv, const double* h, double* w,
                           double* out_norm2, hipStream_t s) {
   if (k > 64) throw std::runtime_error("update: at most 64 vectors");
-  hipLaunchKernelGGL(update_norm_kernel, dim3(grid), dim3(RT), 0, s, n, k, V, ldv, h, w, partial.p);
+  hipLaunchKernelGGL(update_norm_kernel, dim3(grid), dim3(RT), 0, s, n, k, V, ldv, h, w, partial);
   HFX_HIP(hipGetLastError());
-  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(RT), 0, s, grid, partial.p, out_norm2);
+  hipLaunchKernelGGL(reduce_rows_kernel, dim3(1), dim3(RT), 0, s, grid, partial, out_norm2);
   HFX_HIP(hipGetLastError());
 }
 
 double Reducer::norm(long long n, const double* x, hipStream_t s) {
   multidot(n, 1, x, 0, x, s);
   double v = 0.0;
-  HFX_HIP(hipMemcpyAsync(&v, out.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  HFX_HIP(hipMemcpyAsync(&v, out, sizeof(double), hipMemcpyDeviceToHost, s));
   HFX_HIP(hipStreamSynchronize(s));
   return std::sqrt(v);
 }
@@ -464,10 +480,10 @@ void fgmres(const BsrDev& A, const double* b, double* x, const std::function<voi
       bsr_spmv(A, zk, w.p, nullptr, s);
       // classical Gram-Schmidt, twice
       red.multidot(n, k + 1, V.p, n, w.p, s);
-      HFX_HIP(hipMemcpyAsync(hdev.p, red.out.p, (k + 1) * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HFX_HIP(hipMemcpyAsync(hdev.p, red.out, (k + 1) * sizeof(double), hipMemcpyDeviceToDevice, s));
       red.update_norm(n, k + 1, V.p, n, hdev.p, w.p, hdev.p + 128, s);
       red.multidot(n, k + 1, V.p, n, w.p, s);
-      HFX_HIP(hipMemcpyAsync(hdev.p + 64, red.out.p, (k + 1) * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HFX_HIP(hipMemcpyAsync(hdev.p + 64, red.out, (k + 1) * sizeof(double), hipMemcpyDeviceToDevice, s));
       red.update_norm(n, k + 1, V.p, n, hdev.p + 64, w.p, hdev.p + 128, s);
       hipLaunchKernelGGL(scale_by_inv_sqrt_kernel, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, hdev.p + 128, w.p,
                          V.p + (size_t)(k + 1) * n);
@@ -516,17 +532,47 @@ void fgmres(const BsrDev& A, const double* b, double* x, const std::function<voi
 
 namespace hfx {
 
+size_t bicgstab_ws_doubles(long long n) {
+  const size_t G = (size_t)Reducer::grid_for(n), nblk_rows = (size_t)ceil_div(n, RT);
+  return al32(Reducer::doubles(n)) + 8 * al32((size_t)n) + al32(SC_N) + al32(2 * G) + al32(2 * nblk_rows);
+}
+
 void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<void(const double*, double*)>& precond,
-              const KrylovOpts& o, KrylovInfo& info, hipStream_t s) {
+              const KrylovOpts& o, KrylovInfo& info, hipStream_t s, double* ws) {
   const long long n = A.nrows();
   info = KrylovInfo();
+  if (ws && o.graph) throw std::runtime_error("bicgstab: a caller workspace excludes the graph option");
   Reducer red;
-  red.init(n);
-  const int G = red.grid;
-  DevBuf<double> r, rh, p, v, ph, sv, sh, tv, sc, part;
-  for (DevBuf<double>* q : {&r, &rh, &p, &v, &ph, &sv, &sh, &tv}) q->alloc(n);
-  sc.alloc(SC_N);
-  part.alloc((size_t)2 * G);
+  const int G = Reducer::grid_for(n);
+  const int nblk_rows = ceil_div(n, RT);
+  // r, rhat, p, v, p^, s, s^, t, the device scalars, the dot partials and the
+  // fused-kernel partials: owned buffers, or carved from the caller's memory
+  DevBuf<double> own[11];
+  double* vec[8];
+  double *sc, *part, *fpart;
+  if (ws) {
+    red.init(n, ws);
+    double* q = ws + al32(Reducer::doubles(n));
+    for (int k = 0; k < 8; ++k, q += al32((size_t)n)) vec[k] = q;
+    sc = q;
+    q += al32(SC_N);
+    part = q;
+    q += al32((size_t)2 * G);
+    fpart = q;
+  } else {
+    red.init(n);
+    for (int k = 0; k < 8; ++k) {
+      own[k].alloc(n);
+      vec[k] = own[k].p;
+    }
+    own[8].alloc(SC_N);
+    own[9].alloc((size_t)2 * G);
+    own[10].alloc((size_t)2 * nblk_rows);
+    sc = own[8].p;
+    part = own[9].p;
+    fpart = own[10].p;
+  }
+  double *r = vec[0], *rh = vec[1], *p = vec[2], *v = vec[3], *ph = vec[4], *sv = vec[5], *sh = vec[6], *tv = vec[7];
   const double bnorm = red.norm(n, b, s);
   if (bnorm == 0.0) {
     HFX_HIP(hipMemsetAsync(x, 0, n * sizeof(double), s));
@@ -541,20 +587,17 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
     else HFX_HIP(hipMemcpyAsync(out, in, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   };
   auto dots = [&](int nd, const double* a0, const double* b0, const double* a1, const double* b1, int op) {
-    hipLaunchKernelGGL(dots_kernel, dim3(G), dim3(RT), 0, s, n, nd, a0, b0, a1, b1, part.p);
+    hipLaunchKernelGGL(dots_kernel, dim3(G), dim3(RT), 0, s, n, nd, a0, b0, a1, b1, part);
     HFX_HIP(hipGetLastError());
-    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(ST), 0, s, G, nd, op, part.p, sc.p);
+    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(ST), 0, s, G, nd, op, part, sc);
     HFX_HIP(hipGetLastError());
   };
   // fused path: SpMV / update kernels leave one partial per 256-row block
-  const int nblk_rows = ceil_div(n, RT);
-  DevBuf<double> fpart;
-  fpart.alloc((size_t)2 * nblk_rows);
   auto spmv_dot = [&](const double* xin, double* yout, const double* w1, bool both) {
 #define HFX_BS(B)                                                                                          \
   case B:                                                                                                  \
     hipLaunchKernelGGL((spmv_dot_kernel<B>), dim3(nblk_rows), dim3(RT), 0, s, A.nb, A.rowptr.p, A.col.p,   \
-                       A.vals.p, xin, yout, w1, both ? 1 : 0, fpart.p);                                    \
+                       A.vals.p, xin, yout, w1, both ? 1 : 0, fpart);                                    \
     break;
     switch (A.bs) {
       HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(12) HFX_BS(18)
@@ -564,7 +607,7 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
     HFX_HIP(hipGetLastError());
   };
   auto scalars = [&](int nd, int op) {
-    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(ST), 0, s, nblk_rows, nd, op, fpart.p, sc.p);
+    hipLaunchKernelGGL(scalars_kernel, dim3(1), dim3(ST), 0, s, nblk_rows, nd, op, fpart, sc);
     HFX_HIP(hipGetLastError());
   };
   const int check_every = 8;
@@ -572,18 +615,18 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
   // (rhat, v), alpha, s, M s, A s^ with (s, t) and (t, t), omega, x, r with
   // (rhat, r) and (r, r)
   auto steady = [&]() {
-    hipLaunchKernelGGL(bicg_p_kernel, dim3(nblk), dim3(256), 0, s, n, 0, sc.p, r.p, p.p, v.p);
+    hipLaunchKernelGGL(bicg_p_kernel, dim3(nblk), dim3(256), 0, s, n, 0, sc, r, p, v);
     HFX_HIP(hipGetLastError());
-    apply_M(p.p, ph.p);
-    spmv_dot(ph.p, v.p, rh.p, false);
+    apply_M(p, ph);
+    spmv_dot(ph, v, rh, false);
     scalars(1, 1);
-    hipLaunchKernelGGL(bicg_s_kernel, dim3(nblk), dim3(256), 0, s, n, sc.p, r.p, v.p, sv.p);
+    hipLaunchKernelGGL(bicg_s_kernel, dim3(nblk), dim3(256), 0, s, n, sc, r, v, sv);
     HFX_HIP(hipGetLastError());
-    apply_M(sv.p, sh.p);
-    spmv_dot(sh.p, tv.p, sv.p, true);
+    apply_M(sv, sh);
+    spmv_dot(sh, tv, sv, true);
     scalars(2, 2);
-    hipLaunchKernelGGL(bicg_xr_dot_kernel, dim3(nblk), dim3(RT), 0, s, n, sc.p, ph.p, sh.p, sv.p, tv.p, rh.p, x, r.p,
-                       fpart.p);
+    hipLaunchKernelGGL(bicg_xr_dot_kernel, dim3(nblk), dim3(RT), 0, s, n, sc, ph, sh, sv, tv, rh, x, r,
+                       fpart);
     HFX_HIP(hipGetLastError());
     scalars(2, 0);
   };
@@ -622,7 +665,7 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
       graph_block();
       it += check_every;
       double hs[SC_N];
-      HFX_HIP(hipMemcpyAsync(hs, sc.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+      HFX_HIP(hipMemcpyAsync(hs, sc, sizeof(hs), hipMemcpyDeviceToHost, s));
       HFX_HIP(hipStreamSynchronize(s));
       const double rr = std::sqrt(std::fabs(hs[SC_RR]));
       if (!std::isfinite(hs[SC_RR])) { info.reason = DIVERGED_NANORINF; break; }
@@ -631,30 +674,30 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
     }
     if (restart) {
       // r = b - A x (true FP64 residual), rhat = r, rho = (r, r)
-      bsr_spmv(A, x, r.p, b, s);
-      const double rn = red.norm(n, r.p, s);
+      bsr_spmv(A, x, r, b, s);
+      const double rn = red.norm(n, r, s);
       info.true_relres = rn / bnorm;
       if (!std::isfinite(rn)) { info.reason = DIVERGED_NANORINF; break; }
       if (rn <= target) { info.reason = CONVERGED_RTOL; break; }
       if (it >= o.maxits) { info.reason = DIVERGED_ITS; break; }
-      HFX_HIP(hipMemcpyAsync(rh.p, r.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-      hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(64), 0, s, sc.p);
+      HFX_HIP(hipMemcpyAsync(rh, r, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL(set_scalars_kernel, dim3(1), dim3(64), 0, s, sc);
       HFX_HIP(hipGetLastError());
-      dots(2, rh.p, r.p, r.p, r.p, 0);
+      dots(2, rh, r, r, r, 0);
       ++info.restarts;
       restart = false;
-      hipLaunchKernelGGL(bicg_p_kernel, dim3(nblk), dim3(256), 0, s, n, 1, sc.p, r.p, p.p, v.p);
+      hipLaunchKernelGGL(bicg_p_kernel, dim3(nblk), dim3(256), 0, s, n, 1, sc, r, p, v);
       HFX_HIP(hipGetLastError());
-      apply_M(p.p, ph.p);
-      spmv_dot(ph.p, v.p, rh.p, false);          // v = A p^, (rhat, v)
+      apply_M(p, ph);
+      spmv_dot(ph, v, rh, false);          // v = A p^, (rhat, v)
       scalars(1, 1);
-      hipLaunchKernelGGL(bicg_s_kernel, dim3(nblk), dim3(256), 0, s, n, sc.p, r.p, v.p, sv.p);
+      hipLaunchKernelGGL(bicg_s_kernel, dim3(nblk), dim3(256), 0, s, n, sc, r, v, sv);
       HFX_HIP(hipGetLastError());
-      apply_M(sv.p, sh.p);
-      spmv_dot(sh.p, tv.p, sv.p, true);          // t = A s^, (s, t), (t, t)
+      apply_M(sv, sh);
+      spmv_dot(sh, tv, sv, true);          // t = A s^, (s, t), (t, t)
       scalars(2, 2);
-      hipLaunchKernelGGL(bicg_xr_dot_kernel, dim3(nblk), dim3(RT), 0, s, n, sc.p, ph.p, sh.p, sv.p, tv.p, rh.p, x,
-                         r.p, fpart.p);
+      hipLaunchKernelGGL(bicg_xr_dot_kernel, dim3(nblk), dim3(RT), 0, s, n, sc, ph, sh, sv, tv, rh, x,
+                         r, fpart);
       HFX_HIP(hipGetLastError());
       scalars(2, 0);                             // (rhat, r), (r, r)
     } else {
@@ -663,7 +706,7 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
     ++it;
     if (it % check_every == 0 || it >= o.maxits) {
       double hs[SC_N];
-      HFX_HIP(hipMemcpyAsync(hs, sc.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+      HFX_HIP(hipMemcpyAsync(hs, sc, sizeof(hs), hipMemcpyDeviceToHost, s));
       HFX_HIP(hipStreamSynchronize(s));
       const double rr = std::sqrt(std::fabs(hs[SC_RR]));
       if (!std::isfinite(hs[SC_RR])) { info.reason = DIVERGED_NANORINF; break; }

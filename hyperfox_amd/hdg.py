@@ -331,6 +331,14 @@ class HDGProblem:
         check(lib.hfx_hdg_solve(self._h, pc, rtol, maxits, restart, _ptr(info), _ptr(rel)))
         return int(info[0]), int(info[1]), float(rel[0])
 
+    def solve_noalloc(self, pc=2, rtol=1e-10, maxits=10000):
+        """The BiCGStab trace solve of solve() with no device allocation
+        (hfx_hdg_solve_noalloc: a fixed per-GPU workspace; pc 0, 1 or 2)."""
+        info = np.zeros(2, dtype=np.int32)
+        rel = np.zeros(1)
+        check(lib.hfx_hdg_solve_noalloc(self._h, pc, rtol, maxits, _ptr(info), _ptr(rel)))
+        return int(info[0]), int(info[1]), float(rel[0])
+
     def pc_apply(self, pc, x):
         """y = M^-1 x for preconditioner `pc` set up on the assembled system."""
         x = _sized(x, self._need_mesh('pc_apply').n_trace, 'pc_apply x')

@@ -285,6 +285,22 @@ int hfx_hdg_set_solver_graphs(hfx_hdg* h, int on);
  * -9 nan); relres[0] = final true relative residual. */
 int hfx_hdg_solve(hfx_hdg* h, int pc, double rtol, int maxits, int restart, int* info, double* relres);
 
+/* The trace solve of hfx_hdg_solve with no device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  BiCGStab (HFX_KRYLOV_BICGSTAB) with `pc` HFX_PC_NONE,
+ * HFX_PC_JACOBI or HFX_PC_BJACOBI, zero initial guess, stopping on the true
+ * residual ||b - A lambda||_2 / ||b||_2 <= rtol, then the reconstruction of u
+ * and q; the same kernels in the same order as hfx_hdg_solve with BiCGStab, so
+ * the same iterations and fields bit for bit.  The workspace is a fixed
+ * 64 MiB device array per GPU, held until the process ends (about
+ * (8 + nNf) doubles per trace DOF with block Jacobi: up to ~0.6M trace DOFs);
+ * a larger system is refused with an error.  Calls that overlap on one GPU
+ * (several host threads) race on that array and must be serialised.
+ * info[0] = iterations, info[1] = converged reason; relres[0] = final true
+ * relative residual. */
+int hfx_hdg_solve_noalloc(hfx_hdg* h, int pc, double rtol, int maxits, int* info, double* relres);
+
 /* Set up preconditioner `pc` for the assembled trace matrix and apply it
  * once: y = M^-1 x (host vectors of trace DOFs).  For tests and callers that
  * drive their own iteration. */
