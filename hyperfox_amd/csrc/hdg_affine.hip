@@ -687,22 +687,19 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       double y[NN];
 #pragma unroll
       for (int k = 0; k < NN; ++k) y[k] = RB[PL[k] * NCOL + c];
+      // column-oriented (right-looking) substitution: once y[k] is final,
+      // every later row takes its update at once, so each step issues
+      // independent FMAs and the dependent chain is NN long, not NN^2 / 2
 #pragma unroll
-      for (int k = 1; k < NN; ++k) {
-        double sacc = y[k];
+      for (int k = 0; k < NN - 1; ++k)
 #pragma unroll
-        for (int j = 0; j < NN; ++j)
-          if (j < k) sacc = fma(-KM[k * NN + j], y[j], sacc);
-        y[k] = sacc;
-      }
+        for (int i = k + 1; i < NN; ++i) y[i] = fma(-KM[i * NN + k], y[k], y[i]);
 #pragma unroll
       for (int kk = 0; kk < NN; ++kk) {
         const int k = NN - 1 - kk;
-        double sacc = y[k];
+        y[k] *= RDp[k];
 #pragma unroll
-        for (int j = 0; j < NN; ++j)
-          if (j > k) sacc = fma(-KM[k * NN + j], y[j], sacc);
-        y[k] = sacc * RDp[k];
+        for (int i = 0; i < k; ++i) y[i] = fma(-KM[i * NN + k], y[k], y[i]);
       }
 #pragma unroll
       for (int i = 0; i < NN; ++i) UB[i * NCOL + c] = -y[i];
