@@ -126,6 +126,8 @@ _SIGS = {
     'hfx_burgers_set_boundary_model': ([_P, _I], _I),
     'hfx_ngamma_create': ([_I, _I, _P], _I),
     'hfx_burgers_set_boundary_hdg': ([_P, _P, _I, _P, _P, _I, _I], _I),
+    'hfx_ngamma_set_boundary_bohm': ([_P, _P, _I, _P, _I], _I),
+    'hfx_burgers_boundary_rows': ([_P, _P, _P], _I),
     'hfx_burgers_set_time_scheme': ([_P, _I, _D], _I),
     'hfx_burgers_set_solver_graphs': ([_P, _I], _I),
     'hfx_burgers_start_step': ([_P], _I),

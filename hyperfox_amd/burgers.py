@@ -75,8 +75,18 @@ class BurgersProblem:
         Bd = _sized(B, f.size * self.bs * self.bs * (2 + self.dim), 'set_boundary_hdg B')
         rd = _sized(r, f.size * self.bs, 'set_boundary_hdg r')
         self._keep_hdg = (f, Bd, rd)
+        self._n_hdg = f.size
         check(lib.hfx_burgers_set_boundary_hdg(self._h, _ptr(f), int(f.size), _ptr(Bd), _ptr(rd), int(bool(matrix_set)),
                                                int(bool(rhs_set))))
+
+    def boundary_rows(self):
+        """(B, r) of the HDG-type boundary faces at the last assembly
+        (caller-set, or built by the Bohm model): (n, bs, bs (2 + dim)), (n, bs)."""
+        n = int(getattr(self, '_n_hdg', 0))
+        B = np.zeros((n, self.bs, self.bs * (2 + self.dim)))
+        r = np.zeros((n, self.bs))
+        check(lib.hfx_burgers_boundary_rows(self._h, _ptr(B), _ptr(r)))
+        return B, r
 
     def set_time_scheme(self, scheme, dt=0.0):
         """0: steady; 1 (TS_BEULER): implicit Euler with step dt."""
@@ -175,3 +185,13 @@ class NGammaProblem(BurgersProblem):
         Dd, Gd, bd, g, f = arrs
         check(lib.hfx_ngamma_set_fields(self._h, _ptr(tcol), t.shape[2], _ptr(Dd), _ptr(Gd), _ptr(bd),
                                         float(sound_speed), _ptr(g), _ptr(f)))
+
+    def set_boundary_bohm(self, faces, normals, transfer=2):
+        """HDGBohmModel on the boundary `faces` with the exterior normals
+        (n, nNf, 2) in face-node order; transfer 0 neumann, 1 dirichlet,
+        2 heaviside, 3 tanh (the reference regression's functions)."""
+        f = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1)
+        nr = _sized(normals, f.size * self.nNf * 2, 'set_boundary_bohm normals')
+        self._keep_bohm = (f, nr)
+        check(lib.hfx_ngamma_set_boundary_bohm(self._h, _ptr(f), int(f.size), _ptr(nr), int(transfer)))
+        self._n_hdg = f.size

@@ -6,6 +6,8 @@
 
 #include <cstdint>
 
+#include "../../include/hfx_hip.h"
+
 namespace hfx {
 
 // Arguments of one Burgers cell-kernel launch (device pointers).
@@ -67,5 +69,30 @@ void launch_expand_perm(long long nblk, int nNf, int m, const int8_t* perm, int8
 void burgers_residual(long long n, const double* cur, const double* prev, double* partial, double* out2,
                       hipStream_t s);
 void burgers_damp(long long n, double damp, double* cur, double* prev, hipStream_t s);
+
+// HDGBohmModel rows of the 2-D (n, Gamma) problem (HDGBohmModel.cpp:97-268)
+// on the boundary faces faces[nfaces]: B (nfaces x 2 nNf x 8 nNf, columns
+// u (j 2 + a), q ((j 2 + k) 2 + a), lambda (j 2 + a)) and r (nfaces x 2 nNf)
+// from the face-local fields, as HDGSolver::applyBoundaryConditions gathers
+// them: Solution and Flux of the face's cell at the face nodes, Trace, Tau
+// (side 0) and the exterior normals in face-node order, b, D and G at the
+// face nodes; fphi, fdphi, fw the face element's tables at its cubature.
+struct BohmArgs {
+  int nNf = 0, nipf = 0, nN = 0, nFpc = 0, tau_sides = 1, kind = HFX_BOHM_HEAVISIDE;
+  const int* faces = nullptr;
+  const double* normals = nullptr;   // nfaces x nNf x 2
+  const int* face2cell = nullptr;
+  const int* cell2face = nullptr;
+  const int* faceNodes = nullptr;
+  const int8_t* invperm = nullptr;
+  const int* cells = nullptr;
+  const double* coords = nullptr;
+  const double *fphi = nullptr, *fdphi = nullptr, *fw = nullptr;
+  const double *sol = nullptr, *flux = nullptr, *trace = nullptr, *tau = nullptr;
+  const double *D = nullptr, *G = nullptr, *bfield = nullptr;
+  double sound_speed = 1.0;
+  double *B = nullptr, *r = nullptr;
+};
+void launch_bohm(const BohmArgs& a, int nfaces, hipStream_t s);
 
 }  // namespace hfx

@@ -49,6 +49,11 @@ struct hfx_burgers {
   hfx::DevBuf<int> hdg_faces;
   hfx::DevBuf<double> hdg_B, hdg_r;
   int n_hdg = 0, n_dir = 0, hdg_mset = 1, hdg_rset = 1;
+  // HDGBohmModel on the HDG-type faces (hfx_ngamma_set_boundary_bohm): B and r
+  // rebuilt on the GPU before every assembly
+  bool bohm = false;
+  int bohm_kind = HFX_BOHM_HEAVISIDE;
+  hfx::DevBuf<double> bohm_normals;
   hfx::DevBuf<uint8_t> is_dir;
   hfx::BsrPattern pat;
   hfx::BsrDev A;
@@ -106,6 +111,43 @@ void assemble(hfx_burgers* h) {
     hfx::launch_integrated_dirichlet(h->A, h->n_dir, h->bfaces.p, d, nN, nipf, m.nFpc, h->faceNodes.p, h->perm.p,
                                      h->cells.p, h->coords.p, h->tabs.p + t_fphi, h->tabs.p + t_fdphi,
                                      h->tabs.p + t_fw, h->g.p, h->rhs.p, s, h->m);
+  }
+  // HDGBohmModel::compute on its faces, from the current state
+  if (h->bohm && h->n_hdg > 0) {
+    need(h->Dn.p && h->G.p, "hfx_burgers_assemble: the Bohm model needs the D and G fields");
+    const hfx::RefEl& re = *h->re;
+    const int nN = re.nN, nNf = re.nNf(), nip = re.nip(), nipf = re.face->nip(), d = h->dim;
+    const size_t t_fphi = (size_t)nip * nN * (1 + d), t_fdphi = t_fphi + (size_t)nipf * nNf;
+    const size_t t_fw = t_fdphi + (size_t)nipf * nNf * (d - 1) + nip;
+    hfx::BohmArgs ba;
+    ba.nNf = nNf;
+    ba.nipf = nipf;
+    ba.nN = nN;
+    ba.nFpc = m.nFpc;
+    ba.tau_sides = h->tau_sides;
+    ba.kind = h->bohm_kind;
+    ba.faces = h->hdg_faces.p;
+    ba.normals = h->bohm_normals.p;
+    ba.face2cell = h->f2c.p;
+    ba.cell2face = h->c2f.p;
+    ba.faceNodes = h->faceNodes.p;
+    ba.invperm = h->invperm.p;
+    ba.cells = h->cells.p;
+    ba.coords = h->coords.p;
+    ba.fphi = h->tabs.p + t_fphi;
+    ba.fdphi = h->tabs.p + t_fdphi;
+    ba.fw = h->tabs.p + t_fw;
+    ba.sol = h->sol.p;
+    ba.flux = h->flux.p;
+    ba.trace = h->trace.p;
+    ba.tau = h->tau.p;
+    ba.D = h->Dn.p;
+    ba.G = h->G.p;
+    ba.bfield = h->bfield.p;
+    ba.sound_speed = h->sound_speed;
+    ba.B = h->hdg_B.p;
+    ba.r = h->hdg_r.p;
+    hfx::launch_bohm(ba, h->n_hdg, s);
   }
   // HDG-type boundary rows (HDGSolver.cpp:503-518) on the faces of that model
   if (h->n_hdg > 0)
@@ -351,32 +393,26 @@ extern "C" int hfx_burgers_set_boundary_model(hfx_burgers* h, int kind) {
   HFX_END
 }
 
-extern "C" int hfx_burgers_set_boundary_hdg(hfx_burgers* h, const int* faces, int n, const double* B,
-                                            const double* r, int matrix_set, int rhs_set) {
-  HFX_TRY
-  need(h && h->has_mesh, "hfx_burgers_set_boundary_hdg: set the mesh first");
-  need(n >= 0 && (n == 0 || (faces && B && r)), "hfx_burgers_set_boundary_hdg: null argument");
+namespace {
+
+// The faces of the HDG-type boundary model (checked: boundary faces, each
+// once); the Dirichlet models keep the other boundary faces, and the scatter
+// leaves the cell's own rows on these.  B and r are left to the caller.
+void set_hdg_faces(hfx_burgers* h, const char* who, const int* faces, int n, int matrix_set, int rhs_set) {
   const hfx::Mesh& m = h->mesh;
-  const int bs = h->bs(), nc = bs * (2 + h->dim);
   std::vector<char> bnd(m.nFaces, 0), seen(m.nFaces, 0);
   for (int f : m.boundary) bnd[f] = 1;
   for (int i = 0; i < n; ++i) {
-    need(faces[i] >= 0 && faces[i] < m.nFaces && bnd[faces[i]], "hfx_burgers_set_boundary_hdg: not a boundary face");
-    need(!seen[faces[i]], "hfx_burgers_set_boundary_hdg: a face is listed twice");
+    need(faces[i] >= 0 && faces[i] < m.nFaces && bnd[faces[i]], (std::string(who) + ": not a boundary face").c_str());
+    need(!seen[faces[i]], (std::string(who) + ": a face is listed twice").c_str());
     seen[faces[i]] = 1;
   }
-  for (size_t i = 0; i < (size_t)n * bs * nc; ++i)
-    need(std::isfinite(B[i]), "hfx_burgers_set_boundary_hdg: local matrices must be finite");
   h->set_device();
   hipStream_t s = h->stream;
   h->hdg_faces.upload(std::vector<int>(faces, faces + n), s);
-  h->hdg_B.upload(std::vector<double>(B, B + (size_t)n * bs * nc), s);
-  h->hdg_r.upload(std::vector<double>(r, r + (size_t)n * bs), s);
   h->n_hdg = n;
   h->hdg_mset = matrix_set != 0;
   h->hdg_rset = rhs_set != 0;
-  // the Dirichlet models keep the other boundary faces; the scatter leaves
-  // the cell's own rows on these
   std::vector<int> dirl;
   std::vector<uint8_t> dir(m.nFaces, 0);
   for (int f : m.boundary)
@@ -387,8 +423,57 @@ extern "C" int hfx_burgers_set_boundary_hdg(hfx_burgers* h, const int* faces, in
   h->bfaces.upload(dirl, s);
   h->n_dir = (int)dirl.size();
   h->is_dir.upload(dir, s);
-  HFX_HIP(hipStreamSynchronize(s));
+  h->bohm = false;
   h->assembled = false;
+}
+
+}  // namespace
+
+extern "C" int hfx_burgers_set_boundary_hdg(hfx_burgers* h, const int* faces, int n, const double* B,
+                                            const double* r, int matrix_set, int rhs_set) {
+  HFX_TRY
+  need(h && h->has_mesh, "hfx_burgers_set_boundary_hdg: set the mesh first");
+  need(n >= 0 && (n == 0 || (faces && B && r)), "hfx_burgers_set_boundary_hdg: null argument");
+  const int bs = h->bs(), nc = bs * (2 + h->dim);
+  for (size_t i = 0; i < (size_t)n * bs * nc; ++i)
+    need(std::isfinite(B[i]), "hfx_burgers_set_boundary_hdg: local matrices must be finite");
+  set_hdg_faces(h, "hfx_burgers_set_boundary_hdg", faces, n, matrix_set, rhs_set);
+  hipStream_t s = h->stream;
+  h->hdg_B.upload(std::vector<double>(B, B + (size_t)n * bs * nc), s);
+  h->hdg_r.upload(std::vector<double>(r, r + (size_t)n * bs), s);
+  HFX_HIP(hipStreamSynchronize(s));
+  HFX_END
+}
+
+extern "C" int hfx_ngamma_set_boundary_bohm(hfx_burgers* h, const int* faces, int n, const double* normals,
+                                            int transfer) {
+  HFX_TRY
+  need(h && h->has_mesh, "hfx_ngamma_set_boundary_bohm: set the mesh first");
+  need(h->model == 1, "hfx_ngamma_set_boundary_bohm: the Bohm model belongs to the nGamma problem");
+  need(n >= 0 && (n == 0 || (faces && normals)), "hfx_ngamma_set_boundary_bohm: null argument");
+  need(transfer >= HFX_BOHM_NEUMANN && transfer <= HFX_BOHM_TANH, "hfx_ngamma_set_boundary_bohm: unknown transfer function");
+  const int nNf = h->mesh.nNf, bs = h->bs(), nc = bs * (2 + h->dim);
+  for (size_t i = 0; i < (size_t)n * nNf * 2; ++i)
+    need(std::isfinite(normals[i]), "hfx_ngamma_set_boundary_bohm: the normals must be finite");
+  set_hdg_faces(h, "hfx_ngamma_set_boundary_bohm", faces, n, 1, 1);  // Set / Set (HDGBohmModel.cpp:89-92)
+  hipStream_t s = h->stream;
+  h->hdg_B.alloc((size_t)std::max(n, 1) * bs * nc);
+  h->hdg_r.alloc((size_t)std::max(n, 1) * bs);
+  if (n > 0) h->bohm_normals.upload(std::vector<double>(normals, normals + (size_t)n * nNf * 2), s);
+  h->bohm = n > 0;
+  h->bohm_kind = transfer;
+  HFX_HIP(hipStreamSynchronize(s));
+  HFX_END
+}
+
+extern "C" int hfx_burgers_boundary_rows(hfx_burgers* h, double* B, double* r) {
+  HFX_TRY
+  need(h && h->has_mesh, "hfx_burgers_boundary_rows: set the mesh first");
+  h->set_device();
+  const int bs = h->bs(), nc = bs * (2 + h->dim);
+  if (B && h->n_hdg > 0) h->hdg_B.download(B, (size_t)h->n_hdg * bs * nc, h->stream);
+  if (r && h->n_hdg > 0) h->hdg_r.download(r, (size_t)h->n_hdg * bs, h->stream);
+  HFX_HIP(hipStreamSynchronize(h->stream));
   HFX_END
 }
 

@@ -556,9 +556,31 @@ int hfx_burgers_set_boundary_model(hfx_burgers* h, int kind);
  * face order: columns B_u (pj m + b), B_q ((pj dim + k) m + b), B_lambda
  * (pj m + b)) and rhs r (n x bs); rows Set or Add.  The other boundary faces
  * keep the Dirichlet model; a new call replaces the previous set (n = 0
- * clears it).  For HDGBohmModel the caller runs the model per face. */
+ * clears it).  hfx_ngamma_set_boundary_bohm builds HDGBohmModel's rows on the GPU. */
 int hfx_burgers_set_boundary_hdg(hfx_burgers* h, const int* faces, int n, const double* B, const double* r,
                                  int matrix_set, int rhs_set);
+/* HDGBohmModel (src/model/HDGBohmModel.cpp:97-268), the Bohm-Chodura
+ * boundary model of the nGamma problem (a handle of hfx_ngamma_create), on
+ * the n boundary faces `faces` (n = 0 removes it).  Before every assembly the
+ * GPU rebuilds each face's HDG-type rows B = [B_u | B_q | B_lambda] and r from
+ * the current Solution, Flux and Trace, the model's b, D, G (both required)
+ * and sound speed, the face's Tau (side 0) and the exterior normals
+ * `normals` (n x nNf x 2 in face-node order: the model's ExteriorNormals
+ * field), and applies them with Set assembly, as the model's AssemblyType
+ * says (HDGBohmModel.cpp:89-92).  `transfer` picks the transfer function of
+ * the reference's Bohm regression (tests/regression/HDG/TestHDGBohm.cpp:37-130):
+ * HFX_BOHM_NEUMANN (0), HFX_BOHM_DIRICHLET (1), HFX_BOHM_HEAVISIDE (the step
+ * with |y| tolerance 1e-5 and zero derivative) or HFX_BOHM_TANH (the tanh
+ * smoothed step, width 1e-3, with its derivative); a std::function of the
+ * reference cannot cross to the device.  Replaces an
+ * hfx_burgers_set_boundary_hdg set; the other boundary faces keep the
+ * Dirichlet model. */
+enum { HFX_BOHM_NEUMANN = 0, HFX_BOHM_DIRICHLET = 1, HFX_BOHM_HEAVISIDE = 2, HFX_BOHM_TANH = 3 };
+int hfx_ngamma_set_boundary_bohm(hfx_burgers* h, const int* faces, int n, const double* normals, int transfer);
+/* The HDG-type rows of the last assembly: B (n x bs x bs (2 + dim)) and r
+ * (n x bs) of the faces of hfx_burgers_set_boundary_hdg or
+ * hfx_ngamma_set_boundary_bohm, in their order (either may be NULL). */
+int hfx_burgers_boundary_rows(hfx_burgers* h, double* B, double* r);
 int hfx_burgers_set_krylov(hfx_burgers* h, int krylov);
 /* BiCGStab's steady iterations replayed as a captured HIP graph (same
  * kernels, same results; off by default), as hfx_hdg_set_solver_graphs. */

@@ -125,6 +125,7 @@ class BurgersSystem:
         self.dim, self.p, self.m = dim, p, dim
         self.ngamma = ngamma
         self.hdg = hdg  # dict(faces, B (n, bs, bs (2 + dim)), r (n, bs), mset, rset)
+        self.bohm = None  # dict(faces, normals (n, nNf, 2), kind): HDGBohmModel rows rebuilt before each assembly
         self.dt = None  # implicit Euler step, with self.u_old (nCells x nN m)
         self.u_old = None
         self.s = sizes(dim, p)
@@ -242,15 +243,48 @@ class BurgersSystem:
         q = np.einsum('eij,ej->ei', self.Q, L) + self.Q0
         return u, q
 
-    def newton(self, sol, buf, trace, max_iters=20, tol=1e-6, damp=0.0):
-        """NonLinearWrapper::solve: returns (sol, buf, flux, trace, iterations, residual)."""
+    def bohm_rows(self, sol, flux, trace):
+        """HDGBohmModel::compute on every face of self.bohm with the face-local
+        fields of HDGSolver::applyBoundaryConditions (HDGSolver.cpp:404-476):
+        Solution and Flux of the face's cell at the face nodes, Trace, Tau of
+        side 0 and the ExteriorNormals in face-node order, b, D, G at the face
+        nodes; returns the HDG-type rows as self.hdg takes them (Set / Set)."""
+        import bohm_oracle
+        M, nN, nNf = self.mesh, self.s['nN'], self.s['nNf']
+        fmap = refel(self.dim, self.p)['face_nodes']
+        ng = self.ngamma
+        sol = np.asarray(sol).reshape(self.nC, nN, 2)
+        flux = np.asarray(flux).reshape(self.nC, nN, 4)
+        trace = np.asarray(trace).reshape(self.nFaces, nNf, 2)
+        Bs, rs = [], []
+        for ib, F in enumerate(self.bohm['faces']):
+            e = M['face2cell'][F, 0]
+            lf = list(M['cell2face'][e]).index(F)
+            node_at = {int(self.pos[e, lf, j]): int(fmap[lf][j]) for j in range(nNf)}
+            cn = [node_at[j] for j in range(nNf)]
+            nodes = M['faces'][F]
+            B, r, _ = bohm_oracle.bohm_local(
+                self.p, M['coords'][nodes], np.asarray(self.bohm['normals'][ib]), np.asarray(ng['b'])[nodes],
+                trace[F], flux[e][cn], np.stack([self.tau[F, j, 0].T.reshape(-1) for j in range(nNf)]),
+                self.Dn[nodes], np.asarray(ng['G'])[nodes], sol[e][cn], np.full(nNf, ng['cs']), self.bohm['kind'])
+            Bs.append(B)
+            rs.append(r)
+        return dict(faces=list(self.bohm['faces']), B=np.array(Bs), r=np.array(rs), mset=True, rset=True)
+
+    def newton(self, sol, buf, trace, max_iters=20, tol=1e-6, damp=0.0, flux=None):
+        """NonLinearWrapper::solve: returns (sol, buf, flux, trace, iterations,
+        residual).  With a Bohm model, its rows come from the Solution, Flux
+        (flux: the starting Flux field) and Trace of each iteration."""
         sol = np.array(sol, dtype=np.float64).reshape(self.nC, -1)
         buf = np.array(buf, dtype=np.float64).reshape(self.nC, -1)
         trace = np.array(trace, dtype=np.float64).reshape(-1)
-        flux = None
+        if flux is None:
+            flux = np.zeros((self.nC, self.s['nN'] * self.dim * self.m))
         res = np.inf
         it = 0
         for it in range(1, max_iters + 1):
+            if self.bohm is not None:
+                self.hdg = self.bohm_rows(sol, flux, trace)
             self.assemble(buf, trace)
             trace = self.solve()
             sol, flux = self.reconstruct(trace)
