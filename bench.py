@@ -340,10 +340,10 @@ def transient(hf, pr, prob, w, steps=3, dt=1e-2):
             'sexplicit_steps': ex}
 
 
-def learned_vs_plain(hf, pr, n, fgmres_maxits=2000):
+def learned_vs_plain(hf, pr, n, fgmres_maxits=2000, seeds=12):
     """Learned-preconditioner workload: 2-D HDG diffusion, n x n x 2 jittered
-    triangles, p = 3, k = exp(GRF(ell 0.1, sigma 1)), f = 1, held-out GRF
-    seeds.  BiCGStab to a true residual of 1e-10 with block Jacobi, with the
+    triangles, p = 3, k = exp(GRF(ell 0.1, sigma 1)), f = 1, `seeds` held-out
+    GRF seeds (disjoint from the training seeds).  BiCGStab to a true residual of 1e-10 with block Jacobi, with the
     trained network's correction of block Jacobi (BF16 MFMA inference; set-up
     time included) and with block ILU(0); FGMRES(30) with the same three
     preconditioners, capped at `fgmres_maxits` iterations (on these
@@ -358,7 +358,7 @@ def learned_vs_plain(hf, pr, n, fgmres_maxits=2000):
     prob = hf.HDGProblem(2, 3)
     net.attach(prob, bf16=True)
     rows, fg = [], []
-    for i, seed in enumerate((900000 + n * 1000 + 500, 900000 + n * 1000 + 501)):
+    for i, seed in enumerate(900000 + n * 1000 + 500 + k for k in range(seeds)):
         w = pr.diffusion_grf_2d(n, 3, seed, mesh=mesh)
         if i == 0:
             pr.setup_gpu(w, prob)
@@ -391,6 +391,9 @@ def learned_vs_plain(hf, pr, n, fgmres_maxits=2000):
     def ratio(a, b, key):
         return float(np.mean([r[a][key] / r[b][key] for r in rows]))
 
+    def total(pc, key):
+        return float(np.sum([r[pc][key] for r in rows]))
+
     return {'workload': 'HDG 2D diffusion, %dx%dx2=%d tris, p=3, k=exp(GRF ell=0.1 sigma=1), f=1' % (n, n, 2 * n * n),
             'trace_dofs': int(mesh.n_trace), 'method': 'BiCGStab FP64, true relres 1e-10', 'plain_pc': 'block-jacobi',
             'learned_pc': 'block-jacobi + GNN correction (BF16 MFMA inference)', 'runs': rows,
@@ -398,6 +401,10 @@ def learned_vs_plain(hf, pr, n, fgmres_maxits=2000):
             'time_ratio': ratio('block-jacobi', 'learned', 'ms'),
             'ilu0_iters_ratio': ratio('block-jacobi', 'ilu0', 'iters'),
             'ilu0_time_ratio': ratio('block-jacobi', 'ilu0', 'ms'),
+            'held_out_seeds': len(rows),
+            'total_ms': {pc: total(pc, 'ms') for pc in ('block-jacobi', 'learned', 'ilu0')},
+            'total_iters': {pc: total(pc, 'iters') for pc in ('block-jacobi', 'learned', 'ilu0')},
+            'learned_vs_ilu0_time': total('ilu0', 'ms') / total('learned', 'ms'),
             'fgmres30_capped': fg}
 
 
