@@ -254,25 +254,34 @@ def test_hdg_trace_assembly_through_the_interface_matches_oracle(dim, p, n):
 
 def test_cpp_adapter_compiles_against_the_reference_interface(tmp_path):
     """HipLinAlgebra.h implements the reference's abstract class; compiled and
-    driven through its call-ordering checks (no solve: no GPU here)."""
+    driven through its call-ordering checks (no solve: no GPU here), which
+    throw the reference's ErrorHandle as PetscInterface's do, with the
+    class and member function in its message (ErrorHandle.cpp: "class :
+    function : message")."""
     iface = '/root/reference/src/resolution/LinAlgebraInterface.h'
-    if not os.path.exists(iface):
-        pytest.skip('reference interface header not present')
+    errh = '/root/reference/src/globals/ErrorHandle.cpp'
+    if not os.path.exists(iface) or not os.path.exists(errh):
+        pytest.skip('reference interface headers not present')
     if shutil.which('g++') is None:
         pytest.skip('no g++')
     src = tmp_path / 'drive.cpp'
     src.write_text(r'''
 #include <cstdio>
+#include <string>
 #include <vector>
 #include "HipLinAlgebra.h"
 using namespace hfox;
-template <class F> static int throws(F f) { try { f(); } catch (const std::exception&) { return 1; } return 0; }
+template <class F> static int throws(F f) { try { f(); } catch (const ErrorHandle&) { return 1; } return 0; }
 int main() {
   HipOpts o; o.rtol = 1e-16;
   LinAlgebraInterface* la = new HipLinAlgebra(o);
   std::vector<double> dv = {1}; std::vector<int> iv = {0};
   int bad = 0;
   bad += !throws([&] { la->assemble(); });
+  try { la->assemble(); } catch (const ErrorHandle& e) {
+    std::printf("%s\n", e.what());
+    bad += std::string(e.what()).rfind("HipLinAlgebra : assemble : ", 0) != 0;
+  }
   bad += !throws([&] { la->setValMatrix(0, 0, 1.0); });
   bad += !throws([&] { la->configure(); });
   la->initialize();
@@ -296,8 +305,9 @@ int main() {
 ''')
     exe = tmp_path / 'drive'
     libdir = os.path.join(ROOT, 'hyperfox_amd')
-    cmd = ['g++', '-std=c++17', '-O1', str(src), '-I', os.path.join(ROOT, 'include'), '-I',
-           os.path.dirname(iface), '-L', libdir, '-lhfx_hip', '-Wl,-rpath,' + libdir, '-o', str(exe)]
+    cmd = ['g++', '-std=c++17', '-O1', str(src), errh, '-I', os.path.join(ROOT, 'include'), '-I',
+           os.path.dirname(iface), '-I', os.path.dirname(errh), '-L', libdir, '-lhfx_hip', '-Wl,-rpath,' + libdir,
+           '-o', str(exe)]
     subprocess.run(cmd, check=True, capture_output=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
