@@ -39,6 +39,21 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Scheduling hint for a fully unrolled block of NM MFMAs fed by ND LDS
+// reads: alternate one MFMA with its share of the reads and a few VALU
+// instructions, so that the operands of the next MFMAs are fetched while the
+// current ones run (one wave per SIMD: nothing else hides the LDS latency).
+template <int NM, int ND>
+__device__ __forceinline__ void mfma_interleave() {
+  constexpr int RD = (ND + NM - 1) / NM;
+#pragma unroll
+  for (int i = 0; i < NM; ++i) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+    __builtin_amdgcn_sched_group_barrier(0x100, RD, 0);
+    __builtin_amdgcn_sched_group_barrier(0x002, 4, 0);
+  }
+}
+
 // value of v in lane l (l uniform across the wave)
 __device__ inline double readlane_d(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -103,6 +118,7 @@ struct ALay {
   static constexpr int O_RB = O_OV;
   static constexpr int END2 = O_RB + NN * NCOL;
   static constexpr int PER = END1 > END2 ? END1 : END2;
+  static_assert(NF * NIPF * NS >= NCOL, "the face-IP weights hold one dead row of trace columns");
 };
 
 // Wave-wide argmax of a non-negative value over the valid lanes among 0..31;
@@ -422,10 +438,12 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll
           for (int ks = 0; ks < KS; ++ks) {
             const int q = ks * 4 + lk;
-            const double bv = (q < NIP && jb < NN) ? PHI[q * NN + jb] : 0.0;
+            const double y = PHI[(q < NIP ? q : 0) * NN + (jb < NN ? jb : 0)];
+            const double bv = (q < NIP && jb < NN) ? y : 0.0;
 #pragma unroll
             for (int x = 0; x < NX; ++x) acc[x] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks][x], bv, acc[x], 0, 0, 0);
           }
+          mfma_interleave<KS * NX, KS>();
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = m * 16 + 4 * r + lk, j = jb;
@@ -519,25 +537,28 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
         for (int m = 0; m < MT; ++m) acc[nt][m] = dvec4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 3
+      // operands loaded at clamped (in-range) addresses and zeroed on the
+      // padding lanes afterwards: no exec-masked loads, so they pipeline
+#pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         const int kk = ks * 4 + lk;
         const bool kv = kk < KD;
         const int a2 = kv ? kk / NN : 0, l = kv ? kk % NN : 0;
-        double av[MT];
+        double av[MT], bv[MT];
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
           const int i = m * 16 + li;
-          av[m] = (kv && i < NN) ? SUQ[(a2 * NN + i) * NN + l] : 0.0;
+          const double x = SUQ[(a2 * NN + (i < NN ? i : 0)) * NN + l];
+          av[m] = (kv && i < NN) ? x : 0.0;
+          const double y = AH[(a2 * NN + l) * NN + (i < NN ? i : 0)];
+          bv[m] = (kv && i < NN) ? y : 0.0;
         }
 #pragma unroll
-        for (int nt = 0; nt < MT; ++nt) {
-          const int jb = nt * 16 + li;
-          const double bv = (kv && jb < NN) ? AH[(a2 * NN + l) * NN + jb] : 0.0;
+        for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
-          for (int m = 0; m < MT; ++m) acc[nt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv, acc[nt][m], 0, 0, 0);
-        }
+          for (int m = 0; m < MT; ++m) acc[nt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[nt], acc[nt][m], 0, 0, 0);
       }
+      mfma_interleave<KS * MT * MT, KS * 2 * MT>();
 #pragma unroll
       for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
@@ -648,25 +669,35 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
         for (int m = 0; m < MT; ++m) acc[nt][m] = dvec4{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 3
+      // operands at clamped addresses, padding zeroed after the load
+#pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         const int kk = ks * 4 + lk;
         const bool kv = kk < KD;
         const int a2 = kv ? kk / NN : 0, j = kv ? kk % NN : 0;
-        double av[MT];
+        double av[MT], bv[NT];
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
           const int i = m * 16 + li;
-          av[m] = (kv && i < NN) ? SUQ[(a2 * NN + i) * NN + j] : 0.0;
+          const double x = SUQ[(a2 * NN + (i < NN ? i : 0)) * NN + j];
+          av[m] = (kv && i < NN) ? x : 0.0;
         }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           const double ntv = a2 == 0 ? ntc[nt][0] : (a2 == 1 ? ntc[nt][1] : ntc[nt][2]);
-          const double bv = kv ? ntv * GT[(ggs[nt] * NN + j) * NNF + bbs[nt]] : 0.0;
-#pragma unroll
-          for (int m = 0; m < MT; ++m) acc[nt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv, acc[nt][m], 0, 0, 0);
+          const double y = GT[(ggs[nt] * NN + j) * NNF + bbs[nt]];
+          bv[nt] = kv ? ntv * y : 0.0;
         }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+          for (int m = 0; m < MT; ++m) acc[nt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[nt], acc[nt][m], 0, 0, 0);
       }
+      mfma_interleave<KS * MT * NT, KS * (MT + NT)>();
+      // R += product: all the R entries are read before any is written (the
+      // compiler cannot tell the rows apart and would otherwise wait on every
+      // read-modify-write)
+      double rv[NT][MT][4];
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
@@ -674,7 +705,16 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = m * 16 + 4 * r + lk, cc = nt * 16 + li;
-            if (cvs[nt] && i < NN) RB[i * NCOL + cc] += acc[nt][m][r];
+            rv[nt][m][r] = RB[(i < NN ? i : 0) * NCOL + (cc < NCOL ? cc : 0)];
+          }
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = m * 16 + 4 * r + lk, cc = nt * 16 + li;
+            if (cvs[nt] && i < NN) RB[i * NCOL + cc] = rv[nt][m][r] + acc[nt][m][r];
           }
     }
     wave_sync();
@@ -718,32 +758,61 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       for (int f = 0; f < NF; ++f)
 #pragma unroll
         for (int k = 0; k < D; ++k) nfk[f][k] = FG[f * (D + 1) + k];
+      // the Ahat operands are the same for every column tile: loaded once,
+      // at clamped addresses with the padding zeroed after the load
+      double av[KS][MT][D];
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const int j = ks * 4 + lk, jc = j < NN ? j : 0;
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+          const int i = m * 16 + li, ic = i < NN ? i : 0;
+#pragma unroll
+          for (int a2 = 0; a2 < D; ++a2) {
+            const double x = AH[(a2 * NN + ic) * NN + jc];
+            av[ks][m][a2] = (j < NN && i < NN) ? x : 0.0;
+          }
+        }
+      }
 #pragma unroll 1
       for (int nt = 0; nt < NT; ++nt) {
-        const int cc = nt * 16 + li;
+        const int cc = nt * 16 + li, ccc = cc < NCOL ? cc : NCOL - 1;
+        double bv[KS];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const int j = ks * 4 + lk;
+          const double x = UB[(j < NN ? j : 0) * NCOL + ccc];
+          bv[ks] = (j < NN && cc < NCOL) ? x : 0.0;
+        }
+        // epilogue inputs, fetched while the MFMAs run: G_g at each row, the
+        // U column, each row's face positions
+        const bool trc = cc < NL;
+        const int gg = trc ? cc / NNF : 0, bb = trc ? cc % NNF : 0;
+        double gbv[MT][4], ubv[MT][4];
+        unsigned spw[MT][4];
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = m * 16 + 4 * r + lk, ic = i < NN ? i : 0;
+            gbv[m][r] = GT[(gg * NN + ic) * NNF + bb];
+            ubv[m][r] = UB[ic * NCOL + ccc];
+            spw[m][r] = (unsigned)s_pos[ic];
+          }
         dvec4 y[MT][D];
 #pragma unroll
         for (int m = 0; m < MT; ++m)
 #pragma unroll
           for (int a2 = 0; a2 < D; ++a2) y[m][a2] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-          const int j = ks * 4 + lk;
-          const bool kv = j < NN;
-          const double bv = (kv && cc < NCOL) ? UB[j * NCOL + cc] : 0.0;
+        for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-          for (int m = 0; m < MT; ++m) {
-            const int i = m * 16 + li;
+          for (int m = 0; m < MT; ++m)
 #pragma unroll
-            for (int a2 = 0; a2 < D; ++a2) {
-              const double av = (kv && i < NN) ? AH[(a2 * NN + i) * NN + j] : 0.0;
-              y[m][a2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, y[m][a2], 0, 0, 0);
-            }
-          }
-        }
+            for (int a2 = 0; a2 < D; ++a2)
+              y[m][a2] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks][m][a2], bv[ks], y[m][a2], 0, 0, 0);
+        mfma_interleave<KS * MT * D, KS>();
         if (cc <= NL) {
-          const bool trc = cc < NL;
-          const int gg = trc ? cc / NNF : 0, bb = trc ? cc % NNF : 0;
           const double rg = FG[gg * (D + 1) + D] / det;
           double ng[D];
 #pragma unroll
@@ -756,7 +825,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
             for (int r = 0; r < 4; ++r) {
               const int i = m * 16 + 4 * r + lk;
               if (i >= NN) continue;
-              const double gb = trc ? rg * GT[(gg * NN + i) * NNF + bb] : 0.0;
+              const double gb = trc ? rg * gbv[m][r] : 0.0;
               double qk[D];
               for (int k = 0; k < D; ++k) {
                 double s = 0.0;
@@ -765,20 +834,23 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
               }
               if (trc) {
                 // streaming stores: U and Q are read again only by the reconstruction
-                __builtin_nontemporal_store(UB[i * NCOL + cc], &Ue[(size_t)i * NL + cc]);
+                __builtin_nontemporal_store(ubv[m][r], &Ue[(size_t)i * NL + cc]);
                 for (int k = 0; k < D; ++k) __builtin_nontemporal_store(qk[k], &Qe[(size_t)(i * D + k) * NL + cc]);
               } else {
-                Ue[i] = UB[i * NCOL + cc];
+                Ue[i] = ubv[m][r];
                 for (int k = 0; k < D; ++k) Qe[i * D + k] = qk[k];
               }
-              const unsigned pw = (unsigned)s_pos[i];
+              // n_f . q on every face of node i; a node off face f writes its
+              // value to a dead row instead (the face-IP weights, unused after
+              // the face masses), which keeps the stores free of branches
+              const unsigned pw = spw[m][r];
+#pragma unroll
               for (int f = 0; f < NF; ++f) {
                 const int pf = (int)(pw >> (8 * f) & 0xFFu);
-                if (pf != 0xFF) {
-                  double sq = 0.0;
-                  for (int k = 0; k < D; ++k) sq += nfk[f][k] * qk[k];
-                  QN[(f * NNF + pf) * NCOL + cc] = sq;
-                }
+                double sq = 0.0;
+                for (int k = 0; k < D; ++k) sq += nfk[f][k] * qk[k];
+                double* dst = pf != 0xFF ? QN + (f * NNF + pf) * NCOL + cc : FWT + cc;
+                *dst = sq;
               }
             }
         }
@@ -810,61 +882,94 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       // fused trace scatter: no S; each face block goes to its place in the
       // BSR values in face-node order (off-diagonal blocks have this cell as
       // their only contributor; Dirichlet rows get zeros), the diagonal block
-      // to this cell's side of dside, summed by the diagonal pass
+      // to this cell's side of dside, summed by the diagonal pass.
+      // Every LDS operand of a face is loaded before its MFMAs, at an
+      // in-range (clamped) address with padding lanes zeroed afterwards, so
+      // that the loads issue back to back instead of one wait per MFMA.
+      const int lic = li < NNF ? li : 0;
+      int ccl[NT], gcol[NT], bcol[NT], pcs[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int cc = nt * 16 + li;
+        ccl[nt] = cc < NCOL ? cc : NCOL - 1;
+        gcol[nt] = cc < NL ? cc / NNF : 0;
+        bcol[nt] = cc < NL ? cc % NNF : 0;
+        pcs[nt] = IX[L::IX_PERM + gcol[nt] * NNF + bcol[nt]];  // block column of a trace column
+      }
 #pragma unroll 1
       for (int f = 0; f < NF; ++f) {
         const double* fmt = FM + (f * NSF + 0) * NNF * NNF;
         const double* fmb = FM + (f * NSF + 1) * NNF * NNF;
         const double* fmd = FM + (f * NSF + 2) * NNF * NNF;
+        double av[KS], bv[KS][NT];
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+          const int kk = ks * 4 + lk;
+          const bool kv = kk < KF, isu = kk < NNF;
+          const int b = kv ? (isu ? kk : kk - NNF) : 0;
+          const double x = (isu ? fmt : fmd)[lic * NNF + b];
+          av[ks] = (kv && li < NNF) ? (isu ? x : -x) : 0.0;
+          const double* row = isu ? UB + s_fn[f * NNF + b] * NCOL : QN + (f * NNF + b) * NCOL;
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt) {
+            const double y = row[ccl[nt]];
+            bv[ks][nt] = (kv && nt * 16 + li < NCOL) ? y : 0.0;
+          }
+        }
+        // epilogue inputs, loaded while the MFMAs run: the face's flags and
+        // node permutation, each column's block, the Sll = -FMt + FMb terms
         const int Ff = fused ? IX[L::IX_FACE + f] : 0;
         const int flg = fused ? IX[L::IX_FLAG + f] : 0;
-        const bool dirf = (flg & 2) != 0;
-        const int sidef = flg & 1;
-        const int* permf = IX + L::IX_PERM + f * NNF;
+        int prm[4], bpo[NT];
+        double sll[NT][4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int aa = 4 * r + lk;
+          prm[r] = IX[L::IX_PERM + f * NNF + (aa < NNF ? aa : 0)];
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          bpo[nt] = fused ? IX[L::IX_BPOS + f * NF + gcol[nt]] : 0;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int aa = 4 * r + lk, ac = aa < NNF ? aa : 0;
+            sll[nt][r] = -fmt[ac * NNF + bcol[nt]] + fmb[ac * NNF + bcol[nt]];
+          }
+        }
         // the NT column tiles of face f accumulated together (independent
         // MFMA chains sharing the A operand)
         dvec4 accs[NT];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) accs[nt] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-          const int kk = ks * 4 + lk;
-          const bool kv = kk < KF;
-          const int b = kv ? (kk < NNF ? kk : kk - NNF) : 0;
-          double av = 0.0;
-          if (kv && li < NNF) av = kk < NNF ? fmt[li * NNF + b] : -fmd[li * NNF + b];
-          const int rowb = kk < NNF ? s_fn[f * NNF + b] * NCOL : -1;
+        for (int ks = 0; ks < KS; ++ks)
 #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) {
-            const int cc = nt * 16 + li;
-            double bv = 0.0;
-            if (kv && cc < NCOL) bv = kk < NNF ? UB[rowb + cc] : QN[(f * NNF + b) * NCOL + cc];
-            accs[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, accs[nt], 0, 0, 0);
-          }
-        }
+          for (int nt = 0; nt < NT; ++nt)
+            accs[nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv[ks][nt], accs[nt], 0, 0, 0);
+        mfma_interleave<KS * NT, KS * (NT + 1)>();
+        const bool dirf = (flg & 2) != 0;
+        const int sidef = flg & 1;
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           const int cc = nt * 16 + li;
           const dvec4 acc = accs[nt];
           if (cc <= NL) {
             const bool trc = cc < NL;
-            const int gg = trc ? cc / NNF : 0, bb = trc ? cc % NNF : 0;
+            const int gg = gcol[nt];
             double* Se = trc ? a.S + (size_t)e * NL * NL : a.S0 + (size_t)e * NL;
             double* blk = nullptr;  // fused: this lane's block, column already placed
-            if (fused && trc) {
-              const int pc = IX[L::IX_PERM + gg * NNF + bb];
-              blk = (gg == f) ? a.dside + (size_t)(2 * Ff + sidef) * NNF * NNF + pc
-                              : a.bvals + (size_t)IX[L::IX_BPOS + f * NF + gg] * NNF * NNF + pc;
-            }
+            if (fused && trc)
+              blk = (gg == f) ? a.dside + (size_t)(2 * Ff + sidef) * NNF * NNF + pcs[nt]
+                              : a.bvals + (size_t)bpo[nt] * NNF * NNF + pcs[nt];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int aa = 4 * r + lk;
               if (aa >= NNF) continue;
               double sv = (expl && trc) ? 0.0 : acc[r];
               if (trc) {
-                if (f == gg) sv += -fmt[aa * NNF + bb] + fmb[aa * NNF + bb];
+                if (f == gg) sv += sll[nt][r];
                 if (fused)  // streaming store (measured faster, the diagonal sides included)
-                  __builtin_nontemporal_store((dirf && gg != f) ? 0.0 : sv, &blk[(size_t)permf[aa] * NNF]);
+                  __builtin_nontemporal_store((dirf && gg != f) ? 0.0 : sv, &blk[(size_t)prm[r] * NNF]);
                 else Se[(size_t)(f * NNF + aa) * NL + cc] = sv;
               } else {
                 Se[f * NNF + aa] = -sv;
