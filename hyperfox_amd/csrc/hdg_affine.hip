@@ -109,8 +109,8 @@ struct ALay {
   static constexpr int O_TAU = O_BN + NN * D;
   static constexpr int O_FQ = O_TAU + NF * NNF;
   static constexpr int O_RQ = O_FQ + NIP;
-  static constexpr int O_EJ = O_RQ + NIP;
-  static constexpr int O_BJ = O_EJ + NIP * D * D;
+  static constexpr int O_SD = O_RQ + NIP;          // w det D at each cubature point
+  static constexpr int O_BJ = O_SD + NIP;
   static constexpr int O_WR = O_BJ + NIP * D;
   static constexpr int O_WF = O_WR + NIP;
   static constexpr int END1 = O_WF + NIP;
@@ -199,7 +199,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
   double* TAU = ws + L::O_TAU;
   double* FQ = ws + L::O_FQ;
   double* RQ = ws + L::O_RQ;
-  double* EJ = ws + L::O_EJ;
+  double* SD = ws + L::O_SD;
   double* BJ = ws + L::O_BJ;
   double* WR = ws + L::O_WR;
   double* WF = ws + L::O_WF;
@@ -306,17 +306,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         for (int c = 0; c < D; ++c) bq[c] += ph * BN[i * D + c];
       }
       const double wd = WQ[q] * det;
-      for (int k = 0; k < D; ++k)
-        for (int r = 0; r < D; ++r) {
-          double s;
-          if constexpr (TD) {
-            s = 0.0;
-            for (int l = 0; l < D; ++l) s += dq[k * D + l] * JI[l * D + r];
-          } else {
-            s = dq[0] * JI[k * D + r];
-          }
-          EJ[(q * D + k) * D + r] = wd * s;
-        }
+      SD[q] = wd * dq[0];
       for (int r = 0; r < D; ++r) {
         double s = 0.0;
         for (int k = 0; k < D; ++k) s += bq[k] * JI[k * D + r];
@@ -397,7 +387,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     wave_sync();
     // ----------------------------------------- Suq_k and Suu (entries)
     // On FP64 MFMA over the cubature points: Suq_k = G_k PHI^T and
-    // Suu = G_u PHI^T with G_k[i][q] = sum_c EJ[q][k][c] dphi_c(i, q) and
+    // Suu = G_u PHI^T with G_k[i][q] = SD[q] sum_c Jinv[k][c] dphi_c(i, q) and
     // G_u[i][q] = WR[q] phi_i(q) - sum_k BJ[q][k] dphi_k(i, q), K = NIP;
     // then the face terms, and Suq moved to the reference frame
     // SUQ_a = sum_k Jinv[k][a] Suq_k.
@@ -413,21 +403,25 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       for (int m = 0; m < MT; ++m) {
         const int ia = m * 16 + li;  // A row this lane supplies
         double av[KS][NX];
+        // table and per-point reads at clamped indices (no exec-masked loads)
+        const int ic = ia < NN ? ia : 0;
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
           const int q = ks * 4 + lk;
           const bool v = q < NIP && ia < NN;
-          double g[D];
-          for (int c2 = 0; c2 < D; ++c2) g[c2] = v ? DPHI[(q * NN + ia) * D + c2] : 0.0;
-          const int qq = v ? q : 0;
+          const int qq = q < NIP ? q : 0;
+          double g[D], bj[D];
+          for (int c2 = 0; c2 < D; ++c2) g[c2] = DPHI[(qq * NN + ic) * D + c2];
+          for (int k = 0; k < D; ++k) bj[k] = BJ[qq * D + k];
+          const double sd = SD[qq], wr = WR[qq], ph = PHI[qq * NN + ic];
           double bgs = 0.0;
           for (int k = 0; k < D; ++k) {
             double s2 = 0.0;
-            for (int c2 = 0; c2 < D; ++c2) s2 += EJ[qq * D * D + k * D + c2] * g[c2];
-            av[ks][k] = v ? s2 : 0.0;
-            bgs += BJ[qq * D + k] * g[k];
+            for (int c2 = 0; c2 < D; ++c2) s2 += JI[k * D + c2] * g[c2];
+            av[ks][k] = v ? sd * s2 : 0.0;
+            bgs += bj[k] * g[k];
           }
-          av[ks][D] = v ? WR[qq] * PHI[qq * NN + ia] - bgs : 0.0;
+          av[ks][D] = v ? wr * ph - bgs : 0.0;
         }
 #pragma unroll 1
         for (int nt = 0; nt < MT; ++nt) {
