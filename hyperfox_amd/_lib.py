@@ -71,6 +71,7 @@ _SIGS = {
     'hfx_hdg_get_solution': ([_P] * 4, _I),
     'hfx_hdg_time': ([_P, _I, _I, _I, _P], _I),
     'hfx_element_lds_bytes': ([_I, _I, _P], _I),
+    'hfx_element_workspace_bytes': ([_I, _I, _P], _I),
     'hfx_hdg_set_kernel': ([_P, _I], _I),
     'hfx_hdg_set_krylov': ([_P, _I], _I),
     'hfx_hdg_kernel_info': ([_P, _P], _I),

@@ -1025,6 +1025,12 @@ extern "C" int hfx_element_lds_bytes(int dim, int order, long long* bytes) {
   HFX_END
 }
 
+extern "C" int hfx_element_workspace_bytes(int dim, int order, long long* bytes) {
+  HFX_TRY
+  *bytes = (long long)(hfx::element_workspace_doubles(dim, order) * sizeof(double));
+  HFX_END
+}
+
 extern "C" int hfx_hdg_set_kernel(hfx_hdg* h, int which) {
   HFX_TRY
   require(h, "hfx_hdg_set_kernel: null handle");

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <memory>
 #include <stdexcept>
 #include <vector>
@@ -177,8 +178,20 @@ struct hfx_hdg {
       a.u_old = rk_old_u.p;
       a.ts_w = ts_w.p;
     }
+    // orders whose element working set is kept in device memory: one slab
+    // per workgroup, as many slabs as workgroups fill the GPU twice over
+    const size_t wsd = hfx::element_workspace_doubles(dim, order);
+    if (wsd > 0 && mesh.nCells > 0) {
+      if (num_cu <= 0) HFX_HIP(hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, device));
+      const int slots = (int)std::min<long long>(mesh.nCells, 2LL * num_cu);
+      if (gws.n < (size_t)slots * wsd) gws.alloc((size_t)slots * wsd);
+      a.gws = gws.p;
+      a.gws_slots = (int)(gws.n / wsd);
+    }
     return a;
   }
+  int num_cu = 0;
+  hfx::DevBuf<double> gws;  // element workspace (element_workspace_doubles per slab)
   int rk_stages() const { return rk_n - 1; }
   double rk_entry(int r, int c) const { return rk_table[(size_t)r * rk_n + c]; }
   double rk_diag() const { return rk_entry(rk_stage, 1 + rk_stage); }
@@ -215,9 +228,11 @@ struct hfx_hdg {
   }
   bool use_affine() const {
     if (kernel_choice == 1) return false;
-    const bool ok = affine && !(hasD && ndD == dim * dim);
+    // the affine kernel keeps a cell's working set in LDS: 2-D p <= 5, 3-D p <= 3
+    const bool ok = affine && !(hasD && ndD == dim * dim) && (dim == 2 || order <= 3);
     if (kernel_choice == 2 && !ok)
-      throw std::runtime_error("affine element kernel needs straight-sided cells and a scalar diffusion field");
+      throw std::runtime_error("affine element kernel needs straight-sided cells, a scalar diffusion field and "
+                               "2-D p <= 5 or 3-D p <= 3");
     return ok;
   }
   int affine_waves() const {

@@ -91,14 +91,23 @@ struct ElemArgs {
   int hdg_explicit = 0;
   const double* sol_old = nullptr;
   const double* flux_old = nullptr;
+  // orders whose element working set exceeds the LDS (3-D p = 4, 5): a
+  // device workspace of element_workspace_doubles() doubles per workgroup,
+  // gws_slots of them; the grid is capped at gws_slots
+  double* gws = nullptr;
+  int gws_slots = 0;
 };
 
 // Launch the element kernel for (dim, order) on `count` elements starting at
 // a.el_offset; `nt` threads per element (64, 128 or 256).  Throws on error.
 void launch_element(int dim, int order, const ElemArgs& a, int nt, int grid_cap, hipStream_t s);
 
-// Dynamic LDS bytes the kernel needs for (dim, order).
+// Dynamic LDS bytes the kernel needs for (dim, order): 0 for the orders
+// that keep the element's working set in a device workspace instead.
 size_t element_lds_bytes(int dim, int order);
+// Doubles of device workspace per workgroup for (dim, order); 0 when the
+// working set is in LDS.
+size_t element_workspace_doubles(int dim, int order);
 
 // Affine-cell kernel (hdg_affine.hip): one wavefront per cell, `waves`
 // wavefronts per workgroup; scalar or identity diffusion only.

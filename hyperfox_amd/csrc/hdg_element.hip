@@ -82,13 +82,24 @@ __device__ inline void inv_small(const double* J, double dt, double* Ji) {
   }
 }
 
+// Orders whose working set does not fit the 160 KB of LDS keep it in a
+// device workspace, one slab per workgroup (the same layout, read and written
+// through the caches); the workgroup barriers order those accesses as they
+// order the LDS ones.
+template <int D, int P>
+constexpr bool element_in_workspace() {
+  return (size_t)Layout<D, P, 64>::TOTAL * sizeof(double) > 160 * 1024;
+}
+
 template <int D, int P, int NT>
 __global__ __launch_bounds__(NT) void hdg_element_kernel(ElemArgs a) {
   using L = Layout<D, P, NT>;
   using Z = ESz<D, P>;
   constexpr int NN = L::NN, NNF = L::NNF, NIP = L::NIP, NIPF = L::NIPF, NF = L::NF, NL = L::NL, NS = L::NS;
   constexpr int NC1 = L::NC1, NC2 = L::NC2;
-  extern __shared__ double sm[];
+  extern __shared__ double lds_ws[];
+  double* sm = lds_ws;
+  if constexpr (element_in_workspace<D, P>()) sm = a.gws + (size_t)blockIdx.x * L::TOTAL;
   __shared__ int fn[NF * NNF];
   __shared__ int pos[NF * NN];
   __shared__ int vopp[NF];
@@ -501,12 +512,17 @@ __global__ void ip_coords_kernel(int nEl, const double* __restrict__ tabs, const
 template <int D, int P, int NT>
 void launch_t(const ElemArgs& a, int grid_cap, hipStream_t s) {
   using L = Layout<D, P, NT>;
-  const size_t lds = (size_t)L::TOTAL * sizeof(double);
+  constexpr bool gws = element_in_workspace<D, P>();
+  const size_t lds = gws ? 0 : (size_t)L::TOTAL * sizeof(double);
   auto kern = hdg_element_kernel<D, P, NT>;
   static PerDeviceLaunch once;
-  once.lds_attr((const void*)kern, lds);
+  if (!gws) once.lds_attr((const void*)kern, lds);
   int grid = a.nEl;
   if (grid_cap > 0 && grid > grid_cap) grid = grid_cap;
+  if (gws) {
+    if (!a.gws || a.gws_slots <= 0) throw std::runtime_error("element kernel: this order needs its device workspace");
+    if (grid > a.gws_slots) grid = a.gws_slots;
+  }
   if (grid <= 0) return;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(NT), lds, s, a);
   HFX_HIP(hipGetLastError());
@@ -525,10 +541,20 @@ void launch_dp(const ElemArgs& a, int nt, int grid_cap, hipStream_t s) {
 }  // namespace
 
 size_t element_lds_bytes(int dim, int order) {
-#define HFX_CASE(D, P) \
-  if (dim == D && order == P) return (size_t)Layout<D, P, 64>::TOTAL * sizeof(double);
+#define HFX_CASE(D, P)                                                                             \
+  if (dim == D && order == P)                                                                      \
+    return element_in_workspace<D, P>() ? 0 : (size_t)Layout<D, P, 64>::TOTAL * sizeof(double);
   HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
-  HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3)
+  HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3) HFX_CASE(3, 4) HFX_CASE(3, 5)
+#undef HFX_CASE
+  throw std::runtime_error("element kernel: (dim, order) not supported on the GPU path");
+}
+
+size_t element_workspace_doubles(int dim, int order) {
+#define HFX_CASE(D, P) \
+  if (dim == D && order == P) return element_in_workspace<D, P>() ? (size_t)Layout<D, P, 64>::TOTAL : 0;
+  HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
+  HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3) HFX_CASE(3, 4) HFX_CASE(3, 5)
 #undef HFX_CASE
   throw std::runtime_error("element kernel: (dim, order) not supported on the GPU path");
 }
@@ -537,7 +563,7 @@ void launch_element(int dim, int order, const ElemArgs& a, int nt, int grid_cap,
 #define HFX_CASE(D, P) \
   if (dim == D && order == P) { launch_dp<D, P>(a, nt, grid_cap, s); return; }
   HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
-  HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3)
+  HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3) HFX_CASE(3, 4) HFX_CASE(3, 5)
 #undef HFX_CASE
   throw std::runtime_error("element kernel: (dim, order) not supported on the GPU path");
 }
@@ -553,7 +579,7 @@ void launch_ip_coords(int dim, int order, int nEl, const double* tabs, const dou
     return;                                                                                              \
   }
   HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
-  HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3)
+  HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3) HFX_CASE(3, 4) HFX_CASE(3, 5)
 #undef HFX_CASE
   throw std::runtime_error("ip coords: (dim, order) not supported on the GPU path");
 }

@@ -329,8 +329,16 @@ int hfx_hdg_get_solution(hfx_hdg* h, double* u, double* q, double* lambda);
  * 2 trace solve with the last solve's options.  ms receives `reps` values. */
 int hfx_hdg_time(hfx_hdg* h, int what, int warmup, int reps, double* ms);
 
-/* Dynamic LDS bytes and threads used by the element kernel of (dim, order). */
+/* Dynamic LDS bytes used by the general element kernel of (dim, order); 0
+ * for the orders (3-D p = 4, 5) whose element working set does not fit the
+ * LDS and lives in a device workspace instead (hfx_element_workspace_bytes).
+ * The reference supports 2-D p <= 10 and 3-D p <= 5
+ * (ReferenceElement.cpp:620-623); the GPU path 2-D p <= 5, 3-D p <= 5. */
 int hfx_element_lds_bytes(int dim, int order, long long* bytes);
+/* Device workspace bytes per workgroup of the general element kernel of
+ * (dim, order) (0 when its working set is in LDS); a problem allocates it for
+ * up to twice as many workgroups as the GPU has compute units. */
+int hfx_element_workspace_bytes(int dim, int order, long long* bytes);
 
 /* ------------------------------------------- linear-algebra plug point (la) */
 

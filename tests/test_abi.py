@@ -4,6 +4,8 @@ import ctypes
 import os
 import re
 
+import pytest
+
 import hyperfox_amd as hf
 from conftest import ROOT
 
@@ -33,5 +35,18 @@ def test_errors_are_reported_not_raised_through_the_abi():
 
 def test_unsupported_gpu_orders_rejected_before_any_device_call():
     v = ctypes.c_longlong()
-    assert hf.lib.hfx_element_lds_bytes(3, 4, ctypes.byref(v)) == -1
+    assert hf.lib.hfx_element_lds_bytes(3, 6, ctypes.byref(v)) == -1
+    assert hf.lib.hfx_element_lds_bytes(2, 6, ctypes.byref(v)) == -1
+    assert hf.lib.hfx_element_workspace_bytes(3, 6, ctypes.byref(v)) == -1
     assert hf.lib.hfx_element_lds_bytes(3, 3, ctypes.byref(v)) == 0 and v.value > 0
+    assert hf.lib.hfx_element_workspace_bytes(3, 3, ctypes.byref(v)) == 0 and v.value == 0
+
+
+@pytest.mark.parametrize('p', [4, 5])
+def test_3d_high_orders_keep_the_element_in_a_device_workspace(p):
+    # 3-D p = 4, 5 (ReferenceElement.cpp:623): the element's working set
+    # exceeds the 160 KB of LDS, so the kernel takes a per-workgroup slab of
+    # device memory instead
+    v = ctypes.c_longlong()
+    assert hf.lib.hfx_element_lds_bytes(3, p, ctypes.byref(v)) == 0 and v.value == 0
+    assert hf.lib.hfx_element_workspace_bytes(3, p, ctypes.byref(v)) == 0 and v.value > 160 * 1024

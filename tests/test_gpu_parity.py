@@ -17,6 +17,10 @@ from hdg_helpers import l2_projection_error, refined_sparse_solve, rel
 pytestmark = pytest.mark.gpu
 
 CASES = [(2, p) for p in range(1, 6)] + [(3, p) for p in range(1, 4)]
+# 3-D p = 4, 5: the general kernel with the element's working set in a device
+# workspace (it exceeds the LDS); no affine kernel at these orders
+HIGH = [(3, 4), (3, 5)]
+CASES_ALL = CASES + HIGH
 
 
 def small_workload(dim, p, kind, seed=5):
@@ -79,7 +83,7 @@ def oracle_local(w, f, e):
     return orc.local_system(w.dim, w.p, w.kind, X, D=D, ndD=w.ndD, b=b, tau=tau.reshape(-1), f_ip=fe, r_ip=re)
 
 
-@pytest.mark.parametrize('dim,p', CASES)
+@pytest.mark.parametrize('dim,p', CASES_ALL)
 def test_ip_coords(dim, p):
     w = small_workload(dim, p, 1)
     prob, _ = run_gpu(w)
@@ -90,7 +94,7 @@ def test_ip_coords(dim, p):
 
 
 @pytest.mark.parametrize('kind', [0, 1, 2])
-@pytest.mark.parametrize('dim,p', CASES)
+@pytest.mark.parametrize('dim,p', CASES_ALL)
 def test_local_matrices_match_oracle(dim, p, kind):
     w = small_workload(dim, p, kind)
     prob, f = run_gpu(w)
@@ -106,7 +110,7 @@ def test_local_matrices_match_oracle(dim, p, kind):
 
 @pytest.mark.parametrize('kernel', [1, 0])
 @pytest.mark.parametrize('kind', [0, 1, 2])
-@pytest.mark.parametrize('dim,p', CASES)
+@pytest.mark.parametrize('dim,p', CASES_ALL)
 def test_condensed_blocks_match_oracle(dim, p, kind, kernel):
     """kernel 1 forces the general (per-IP Jacobian) kernel; 0 picks the
     affine-cell kernel on these straight-sided meshes (general for tensor D)."""
@@ -115,7 +119,7 @@ def test_condensed_blocks_match_oracle(dim, p, kind, kernel):
     prob.set_kernel(kernel)
     info = prob.kernel_info()
     assert info['mesh_affine']
-    assert info['kernel'] == ('general' if (kernel == 1 or kind == 1) else 'affine')
+    assert info['kernel'] == ('general' if (kernel == 1 or kind == 1 or (dim, p) in HIGH) else 'affine')
     prob.assemble()
     g = prob.condensed()
     for e in range(w.mesh.n_cells):
@@ -128,7 +132,7 @@ def test_condensed_blocks_match_oracle(dim, p, kind, kernel):
 
 
 @pytest.mark.parametrize('kind', [0, 1, 2])
-@pytest.mark.parametrize('dim,p', CASES)
+@pytest.mark.parametrize('dim,p', CASES_ALL)
 def test_trace_system_matches_oracle(dim, p, kind):
     w = small_workload(dim, p, kind)
     prob, f = run_gpu(w)
@@ -167,7 +171,7 @@ def test_fused_scatter_equals_stored_scatter(dim, p, kind):
 
 @pytest.mark.parametrize('pc', [hf.PC_NONE, hf.PC_JACOBI, hf.PC_BJACOBI])
 @pytest.mark.parametrize('kind', [1, 2])
-@pytest.mark.parametrize('dim,p', [(2, 1), (2, 3), (2, 5), (3, 1), (3, 3)])
+@pytest.mark.parametrize('dim,p', [(2, 1), (2, 3), (2, 5), (3, 1), (3, 3), (3, 4), (3, 5)])
 def test_solution_matches_oracle(dim, p, kind, pc):
     w = small_workload(dim, p, kind)
     prob, f = run_gpu(w)
@@ -181,9 +185,25 @@ def test_solution_matches_oracle(dim, p, kind, pc):
     s = oracle_system(w, f)
     lo, _, _, _ = s.solve(method=1)
     uo, qo = s.reconstruct(lo)
-    assert rel(lam, lo) < 1e-10
-    assert rel(u, uo) < 1e-10
-    assert rel(q, qo) < 1e-10
+    bar = solution_bar(s, relres)
+    assert rel(lam, lo) < bar
+    assert rel(u, uo) < bar
+    assert rel(q, qo) < bar
+
+
+def solution_bar(s, relres):
+    """1e-10, or the forward-error bound 2 cond(A) relres where that is
+    larger: the 3-D p = 5 CDR system has cond(A) = 1.6e3, so a solve stopped
+    at a relative residual of 1e-13 is only good to ~1.6e-10 (measured 1.1e-10
+    against the oracle's dense LU)."""
+    rp, c, v, _ = s.csr()
+    n = len(rp) - 1
+    if n > 4000:
+        return 1e-10
+    A = np.zeros((n, n))
+    for i in range(n):
+        A[i, c[rp[i]:rp[i + 1]]] = v[rp[i]:rp[i + 1]]
+    return max(1e-10, 2.0 * np.linalg.cond(A) * max(relres, 1e-16))
 
 
 @pytest.mark.parametrize('pc', [hf.PC_NONE, hf.PC_BJACOBI])
@@ -205,6 +225,49 @@ def test_bicgstab_solution_matches_oracle(dim, p, kind, pc):
     assert rel(lam, lo) < 1e-10
     assert rel(u, uo) < 1e-10
     assert rel(q, qo) < 1e-10
+
+
+def curved_3d(p, seed=5):
+    """The jittered 2^3 x 6 tetrahedral mesh with every high-order node
+    moved by a smooth field that vanishes on the boundary (curved interior
+    faces, per-point Jacobians)."""
+    m = hf.Mesh.structured(3, p, 2, 0.05, seed)
+    a = m.arrays()
+    x = a['coords'].copy()
+    ho = np.setdiff1d(np.arange(x.shape[0]), np.unique(a['cells'][:, :4]))
+    bump = np.prod(np.sin(np.pi * x[ho]), axis=1)
+    for k in range(3):
+        x[ho, k] += 0.04 * bump * np.cos(2.0 * x[ho, (k + 1) % 3])
+    return hf.Mesh.from_cells(3, p, x, a['cells'])
+
+
+@pytest.mark.parametrize('kind', [1, 2])
+@pytest.mark.parametrize('p', [4, 5])
+def test_curved_3d_high_order_matches_oracle(p, kind):
+    """3-D p = 4, 5 on curved cells: the condensed blocks of every cell and
+    the solution against the oracle."""
+    print('curved 3-D', p, kind, flush=True)
+    base = small_workload(3, p, kind)
+    m = curved_3d(p)
+    w = pr.Workload(base.name, 3, p, m, base.kind, base.tau, base.tau_sides, D=base.D, ndD=base.ndD, b=base.b,
+                    dirichlet=base.dirichlet, f_fn=base.f_fn, r_fn=base.r_fn)
+    prob, f = run_gpu(w)
+    info = prob.kernel_info()
+    assert not info['mesh_affine'] and info['kernel'] == 'general'
+    prob.assemble()
+    g = prob.condensed()
+    for e in range(m.n_cells):
+        Ko, Fo = oracle_local(w, f, e)
+        c = orc.condense(3, p, Ko, Fo)
+        for k in ('S', 'U', 'Q'):
+            assert rel(g[k][e], c[k]) < 1e-11, (e, k, rel(g[k][e], c[k]))
+    its, reason, relres = prob.solve(pc=hf.PC_BJACOBI, rtol=1e-13, maxits=20000)
+    assert reason == 2 and relres <= 1e-13
+    u, q, lam = prob.solution()
+    s = oracle_system(w, f)
+    lo, _, _, _ = s.solve(method=1)
+    uo, qo = s.reconstruct(lo)
+    assert rel(lam, lo) < 1e-10 and rel(u, uo) < 1e-10 and rel(q, qo) < 1e-10
 
 
 def test_constant_dirichlet_light_tri2():

@@ -8,7 +8,7 @@ import pytest
 import hyperfox_amd as hf
 import oracle as orc
 from hdg_helpers import rel
-from test_gpu_parity import CASES, oracle_local, oracle_system, run_gpu, small_workload
+from test_gpu_parity import CASES_ALL, oracle_local, oracle_system, run_gpu, small_workload
 from test_hdg_modes_oracle import linear_case
 
 pytestmark = pytest.mark.gpu
@@ -22,7 +22,7 @@ def old_fields(w, seed=4):
 
 @pytest.mark.parametrize('kernel', [1, 0])
 @pytest.mark.parametrize('kind', [0, 2])
-@pytest.mark.parametrize('dim,p', CASES)
+@pytest.mark.parametrize('dim,p', CASES_ALL)
 def test_explicit_condensed_blocks_match_oracle(dim, p, kind, kernel):
     w = small_workload(dim, p, kind)
     prob, f = run_gpu(w)
