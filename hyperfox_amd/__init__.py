@@ -9,7 +9,7 @@ from ._lib import lib, LIB_PATH, HfxError  # noqa: F401
 from .hdg import Mesh, HDGProblem, Batch, refel_sizes, refel_tables, cubature, ilu_analyse  # noqa: F401
 from .hdg import RK_NAMES, RK_CUSTOM, rk_table  # noqa: F401
 from .linalg import HipLinAlgebra, HipOpts  # noqa: F401
-from .burgers import BurgersProblem, NGammaProblem  # noqa: F401
+from .burgers import BurgersProblem, MultiProblem, NGammaProblem  # noqa: F401
 
 MODEL_LAPLACE = 0
 MODEL_DIFFUSION_SOURCE = 1

@@ -126,6 +126,7 @@ _SIGS = {
     'hfx_burgers_set_fields': ([_P, _P, _I, _P, _P, _P], _I),
     'hfx_burgers_set_boundary_model': ([_P, _I], _I),
     'hfx_ngamma_create': ([_I, _I, _P], _I),
+    'hfx_multi_create': ([_I, _I, _I, _P], _I),
     'hfx_burgers_set_boundary_hdg': ([_P, _P, _I, _P, _P, _I, _I], _I),
     'hfx_ngamma_set_boundary_bohm': ([_P, _P, _I, _P, _I], _I),
     'hfx_burgers_boundary_rows': ([_P, _P, _P], _I),

@@ -1,6 +1,7 @@
-"""The multi-DOF nonlinear path on the GPU: HDGBurgersModel
-(src/model/HDGBurgersModel.cpp, m = dim unknowns per node) and HDGnGammaModel
-(src/model/HDGnGammaModel.cpp, 2-D, m = 2) on HDGSolver's IMPLICIT assembly,
+"""The multi-DOF path on the GPU: HDGBurgersModel
+(src/model/HDGBurgersModel.cpp, m = dim unknowns per node), HDGnGammaModel
+(src/model/HDGnGammaModel.cpp, 2-D, m = 2) and the scalar models at m = dim
+DOFs per node (MultiProblem) on HDGSolver's IMPLICIT assembly,
 and NonLinearWrapper's loop (src/solver/NonLinearWrapper.cpp), through the
 hfx_burgers_* / hfx_ngamma_* entry points of libhfx_hip.so.
 
@@ -158,6 +159,23 @@ class BurgersProblem:
                                               float(rtol), int(maxits), int(restart), ctypes.byref(it),
                                               ctypes.byref(res), ctypes.byref(lin)))
         return it.value, res.value, lin.value
+
+
+class MultiProblem(BurgersProblem):
+    """A scalar HDG model at m = dim DOFs per node on one GPU
+    (HDGOperator::multiplyDOFs, src/operator/HDGOperator.cpp:17-28, as
+    tests/regression/HDG/TestHDGLaplaceMulti.cpp runs HDGLaplaceModel):
+    set_fields(tau, D=None, dirichlet, source) with D None for
+    HDGLaplaceModel (D = I) or a scalar nodal D; tau (nFaces, nNf, sides, m,
+    m) couples the components on the faces.  Linear: assemble(), solve(),
+    state() give the solution."""
+
+    def __init__(self, dim, order, device=0):
+        h = ctypes.c_void_p()
+        check(lib.hfx_multi_create(int(device), int(dim), int(order), ctypes.byref(h)))
+        self._h = h
+        self.dim, self.order, self.m = dim, order, dim
+        self.mesh = None
 
 
 class NGammaProblem(BurgersProblem):

@@ -38,7 +38,9 @@ struct BurgersArgs {
   const double* f_ip = nullptr;
   // model 1: HDGnGammaModel (2-D, m = 2): Dn is its D field (n block), G
   // (nNodes, null: absent) the Gamma block's coefficient, bfield the nodal
-  // b vectors (nNodes x 2), sound_speed c_s
+  // b vectors (nNodes x 2), sound_speed c_s.  Model 2: a scalar model at
+  // m = dim DOFs per node (HDGLaplaceModel, HDGDiffusionSource): diffusion
+  // Dn on every component, no convection; sol and trace are not used.
   int model = 0;
   // implicit Euler on the u rows (RungeKutta BEuler / Euler, the rows divided
   // by dt): with ts_idt = 1/dt > 0, Suu + (M / dt) (x) I and F_u + (M (x) I)

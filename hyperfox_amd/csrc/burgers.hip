@@ -55,7 +55,11 @@ __device__ inline void binv(const double* J, double dt, double* Ji) {
 
 // MDL 0: HDGBurgersModel (one diffusion coefficient for every component,
 // HDGUNabU); MDL 1: HDGnGammaModel (D on n, G on Gamma, four HDGConvection
-// velocities, no HDGUNabU).
+// velocities, no HDGUNabU); MDL 2: a scalar model with m = dim DOFs per node
+// (HDGLaplaceModel / HDGDiffusionSource, HDGOperator::multiplyDOFs,
+// HDGOperator.cpp:17-28): the MDL 0 system without HDGUNabU, i.e. one
+// diffusion operator repeated on every component, coupled only by the
+// matrix tau.
 template <int D, int P, int MDL>
 struct BLay {
   using Z = ESz<D, P>;
@@ -445,7 +449,7 @@ void burgers_cell_kernel(BurgersArgs a) {
             }
             v -= DV[q] * w;
           }
-        } else {
+        } else if constexpr (MDL == 1) {
           for (int q = 0; q < NIP; ++q) {  // HDGConvection of v_(na mb): Suu -= C^T
             const double* gi = GR + (q * NN + i) * D;
             const double* vq = VQ + (q * NV + na * M + mb) * D;
@@ -474,7 +478,7 @@ void burgers_cell_kernel(BurgersArgs a) {
               for (int k = 0; k < D; ++k) ln += LAM[o * M + k] * FN[o * D + k];
               w += USF[o * M + na] * FN[o * D + mb];
               if (na == mb) w += ln;
-            } else {  // HDGConvection: Sul += (v.n) face mass
+            } else if constexpr (MDL == 1) {  // HDGConvection: Sul += (v.n) face mass
               w += VFN[o * (NV > 0 ? NV : 1) + na * M + mb];
             }
             v += FDS[o] * w * FPHI[q * NNF + ai] * FPHI[q * NNF + cn];
@@ -569,7 +573,7 @@ void burgers_cell_kernel(BurgersArgs a) {
               for (int k = 0; k < D; ++k) ln += LAM[o * M + k] * FN[o * D + k];
               w += USF[o * M + na] * FN[o * D + mb];
               if (na == mb) w += ln;
-            } else {
+            } else if constexpr (MDL == 1) {
               w += VFN[o * (NV > 0 ? NV : 1) + na * M + mb];
             }
             v += FDS[o] * w * FPHI[q * NNF + ai] * FPHI[q * NNF + cn];
@@ -671,6 +675,21 @@ void launch_burgers(int dim, int order, const BurgersArgs& a, hipStream_t s) {
       case 3: launch_dp<2, 3, 1>(a, s); return;
       case 4: launch_dp<2, 4, 1>(a, s); return;
       case 5: launch_dp<2, 5, 1>(a, s); return;
+    }
+  } else if (a.model == 2) {
+    if (dim == 2) {
+      switch (order) {
+        case 1: launch_dp<2, 1, 2>(a, s); return;
+        case 2: launch_dp<2, 2, 2>(a, s); return;
+        case 3: launch_dp<2, 3, 2>(a, s); return;
+        case 4: launch_dp<2, 4, 2>(a, s); return;
+        case 5: launch_dp<2, 5, 2>(a, s); return;
+      }
+    } else if (dim == 3) {
+      switch (order) {
+        case 1: launch_dp<3, 1, 2>(a, s); return;
+        case 2: launch_dp<3, 2, 2>(a, s); return;
+      }
     }
   } else if (dim == 2) {
     switch (order) {

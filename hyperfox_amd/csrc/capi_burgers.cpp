@@ -34,7 +34,7 @@ struct hfx_burgers {
   hfx::Mesh mesh;
   bool has_mesh = false, has_fields = false, assembled = false;
   int tau_sides = 1, bc_kind = HFX_BC_INTEGRATED_DIRICHLET, krylov = HFX_KRYLOV_FGMRES;
-  int model = 0;            // 0 HDGBurgersModel, 1 HDGnGammaModel
+  int model = 0;            // 0 HDGBurgersModel, 1 HDGnGammaModel, 2 scalar model at m = dim DOFs
   bool graphs = false;      // hfx_burgers_set_solver_graphs
   double ts_dt = 0.0;       // implicit Euler time step (0: steady)
   hfx::DevBuf<double> old;  // "OldSolution" of the time step
@@ -72,7 +72,7 @@ void need(bool c, const char* msg) {
 
 void assemble(hfx_burgers* h) {
   need(h->has_mesh && h->has_fields, "hfx_burgers_assemble: set the mesh and the fields first");
-  need(h->model == 0 || h->bfield.p != nullptr, "hfx_burgers_assemble: the nGamma b field is not set");
+  need(h->model != 1 || h->bfield.p != nullptr, "hfx_burgers_assemble: the nGamma b field is not set");
   const hfx::Mesh& m = h->mesh;
   hipStream_t s = h->stream;
   hfx::BurgersArgs a;
@@ -240,6 +240,14 @@ extern "C" int hfx_burgers_create(int device, int dim, int order, hfx_burgers** 
   HFX_END
 }
 
+extern "C" int hfx_multi_create(int device, int dim, int order, hfx_burgers** out) {
+  HFX_TRY
+  need(out != nullptr, "hfx_multi_create: null output");
+  need(hfx::burgers_supported(dim, order), "hfx_multi_create: the multi-DOF kernel covers 2-D orders 1-5 and 3-D orders 1-2");
+  *out = create(device, dim, order, 2);
+  HFX_END
+}
+
 extern "C" int hfx_ngamma_create(int device, int order, hfx_burgers** out) {
   HFX_TRY
   need(out != nullptr, "hfx_ngamma_create: null output");
@@ -344,6 +352,7 @@ void set_common_fields(hfx_burgers* h, const double* tau, int tau_sides, const d
   h->tau_sides = tau_sides;
   h->tau.upload(tau, (size_t)m.nFaces * m.nNf * tau_sides * mm, s);
   if (D) h->Dn.upload(D, (size_t)m.nNodes, s);
+  else if (h->model == 2) h->Dn.upload(std::vector<double>((size_t)m.nNodes, 1.0), s);  // HDGLaplaceModel: D = I
   else h->Dn.release();
   h->g.alloc(h->n_trace());
   if (dirichlet) HFX_HIP(hipMemcpyAsync(h->g.p, dirichlet, h->n_trace() * sizeof(double), hipMemcpyHostToDevice, s));
@@ -360,7 +369,7 @@ extern "C" int hfx_burgers_set_fields(hfx_burgers* h, const double* tau, int tau
                                       const double* dirichlet, const double* f_ip) {
   HFX_TRY
   need(h != nullptr, "hfx_burgers_set_fields: null handle");
-  need(h->model == 0, "hfx_burgers_set_fields: an nGamma problem takes hfx_ngamma_set_fields");
+  need(h->model != 1, "hfx_burgers_set_fields: an nGamma problem takes hfx_ngamma_set_fields");
   set_common_fields(h, tau, tau_sides, D, dirichlet, f_ip);
   HFX_END
 }

@@ -542,6 +542,17 @@ void hfx_burgers_destroy(hfx_burgers* h);
  * hfx_burgers_* call below applies, with hfx_ngamma_set_fields in place of
  * hfx_burgers_set_fields. */
 int hfx_ngamma_create(int device, int order, hfx_burgers** out);
+/* A scalar HDG model at m = dim DOFs per node (HDGModel::allocate with
+ * nDOFsPerNode > 1, HDGOperator::multiplyDOFs, HDGOperator.cpp:17-28; the
+ * reference's TestHDGLaplaceMulti.cpp): HDGLaplaceModel when the
+ * DiffusionTensor passed to hfx_burgers_set_fields is NULL (D = I), the
+ * diffusion-source model with that scalar nodal D otherwise, the source (m
+ * values per IP) on every component, and the m x m Tau field coupling the
+ * components on the faces.  Linear: hfx_burgers_assemble and
+ * hfx_burgers_solve give the solution (the state it linearises about is not
+ * read).  The same handle type and calls as hfx_burgers_create; 2-D orders
+ * 1-5, 3-D orders 1-2. */
+int hfx_multi_create(int device, int dim, int order, hfx_burgers** out);
 /* HDGnGammaModel::setFieldMap (HDGnGammaModel.cpp:5-44): Tau, Dirichlet and
  * the source values as for Burgers; D (n block) and G (Gamma block) nNodes
  * scalars or NULL (absent); b nNodes x 2 (required); sound_speed

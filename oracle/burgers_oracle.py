@@ -295,3 +295,24 @@ class BurgersSystem:
             sol = (1.0 - damp) * sol + damp * buf
             buf = sol.copy()
         return sol, buf, flux, trace, it, res
+
+
+class MultiSystem(BurgersSystem):
+    """A scalar HDG model at m = dim DOFs per node (HDGModel::allocate with
+    nDOFsPerNode = m, HDGOperator::multiplyDOFs, HDGOperator.cpp:17-28):
+    HDGLaplaceModel when Dn is None (D = 1), the diffusion-source model with
+    the scalar nodal Dn otherwise, one source per component and the m x m tau
+    on the faces.  It is the BurgersSystem with no solution and no trace,
+    where HDGUNabU vanishes: the no-flow local system is the scalar oracle's
+    times I_m and a tau matrix enters as kron(scalar tau part, tau), both
+    pinned in test_burgers_oracle.py.  Boundary rows: DirichletModel
+    ('dirichlet', as TestHDGLaplaceMulti.cpp) or 'integrated'."""
+
+    def __init__(self, dim, p, mesh, tau, Dn=None, f_ip=None, dirichlet=None, boundary='dirichlet'):
+        nn = np.asarray(mesh['coords']).shape[0]
+        super().__init__(dim, p, mesh, tau, Dn=np.ones(nn) if Dn is None else Dn, f_ip=f_ip, dirichlet=dirichlet,
+                         boundary=boundary)
+
+    def assemble_linear(self):
+        nN, m = self.s['nN'], self.m
+        return self.assemble(np.zeros((self.nC, nN * m)), np.zeros(self.ndofs))
