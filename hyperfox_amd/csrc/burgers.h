@@ -51,12 +51,18 @@ struct BurgersArgs {
   const double* bfield = nullptr;
   double sound_speed = 1.0;
   double *S = nullptr, *S0 = nullptr, *U = nullptr, *Q = nullptr, *U0 = nullptr, *Q0 = nullptr;
+  // 3-D p = 3 (working set above the LDS): a device workspace of
+  // burgers_workspace_doubles() per workgroup, gws_slots of them
+  double* gws = nullptr;
+  int gws_slots = 0;
 };
 
 // True when (dim, order) has a Burgers cell kernel: 2-D orders 1..5 and
-// 3-D orders 1..2 (m nN <= 64 rows for the workgroup Gauss-Jordan, the LDS
-// footprint within 150 KB, trace blocks of the solver's block sizes).
+// 3-D orders 1..3 (m nN <= 64 rows for the workgroup Gauss-Jordan; 3-D p = 3
+// keeps its working set in a device workspace).
 bool burgers_supported(int dim, int order);
+// Doubles of device workspace per workgroup (0: the cell fits the LDS).
+size_t burgers_workspace_doubles(int dim, int order, int model);
 
 // Launch over all a.nEl cells (one workgroup of 256 threads per cell,
 // grid-stride); throws on error.

@@ -118,6 +118,14 @@ constexpr int burgers_waves() {
   return w >= 3 ? 4 : (w < 1 ? 1 : (int)w);
 }
 
+// Cells whose working set does not fit the LDS (3-D p = 3: 279 KB) keep it
+// in a device workspace, one slab per workgroup in the same layout; the
+// workgroup barriers order those accesses as they order the LDS ones.
+template <int D, int P, int MDL>
+constexpr bool burgers_in_workspace() {
+  return (size_t)BLay<D, P, MDL>::TOTAL * sizeof(double) > 150 * 1024;
+}
+
 template <int D, int P, int MDL>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(burgers_waves<D, P, MDL>())))
 void burgers_cell_kernel(BurgersArgs a) {
@@ -126,7 +134,9 @@ void burgers_cell_kernel(BurgersArgs a) {
   using Z = ESz<D, P>;
   constexpr int M = L::M, NN = L::NN, NNF = L::NNF, NIP = L::NIP, NIPF = L::NIPF, NF = L::NF, NL = L::NL;
   constexpr int MU = L::MU, ML = L::ML, MM = L::MM, NFQ = L::NFQ, NC1 = L::NC1, NC2 = L::NC2;
-  extern __shared__ double sm[];
+  extern __shared__ double lds_ws[];
+  double* sm = lds_ws;
+  if constexpr (burgers_in_workspace<D, P, MDL>()) sm = a.gws + (size_t)blockIdx.x * L::TOTAL;
   __shared__ int fn[NF * NNF];
   __shared__ int posf[NF * NN];
   __shared__ int vopp[NF];
@@ -591,11 +601,15 @@ void burgers_cell_kernel(BurgersArgs a) {
 
 template <int D, int P, int MDL>
 void launch_dp(const BurgersArgs& a, hipStream_t s) {
-  const size_t lds = (size_t)BLay<D, P, MDL>::TOTAL * sizeof(double);
-  if (lds > 150 * 1024) throw std::runtime_error("Burgers kernel: LDS footprint above 150 KB");
+  constexpr bool gws = burgers_in_workspace<D, P, MDL>();
+  const size_t lds = gws ? 0 : (size_t)BLay<D, P, MDL>::TOTAL * sizeof(double);
   static PerDeviceLaunch once;
-  once.lds_attr((const void*)burgers_cell_kernel<D, P, MDL>, lds);
-  const int grid = a.nEl < 65536 ? a.nEl : 65536;
+  if (!gws) once.lds_attr((const void*)burgers_cell_kernel<D, P, MDL>, lds);
+  int grid = a.nEl < 65536 ? a.nEl : 65536;
+  if (gws) {
+    if (!a.gws || a.gws_slots <= 0) throw std::runtime_error("Burgers kernel: this order needs its device workspace");
+    if (grid > a.gws_slots) grid = a.gws_slots;
+  }
   hipLaunchKernelGGL((burgers_cell_kernel<D, P, MDL>), dim3(grid), dim3(BT), lds, s, a);
   HFX_HIP(hipGetLastError());
 }
@@ -662,7 +676,15 @@ __global__ void damp_kernel(long long n, double damp, double* __restrict__ cur, 
 }  // namespace
 
 bool burgers_supported(int dim, int order) {
-  return (dim == 2 && order >= 1 && order <= 5) || (dim == 3 && order >= 1 && order <= 2);
+  return (dim == 2 && order >= 1 && order <= 5) || (dim == 3 && order >= 1 && order <= 3);
+}
+
+size_t burgers_workspace_doubles(int dim, int order, int model) {
+  if (model == 1) return 0;
+  if (dim == 3 && order == 3)
+    return model == 2 ? (burgers_in_workspace<3, 3, 2>() ? BLay<3, 3, 2>::TOTAL : 0)
+                      : (burgers_in_workspace<3, 3, 0>() ? BLay<3, 3, 0>::TOTAL : 0);
+  return 0;
 }
 
 void launch_burgers(int dim, int order, const BurgersArgs& a, hipStream_t s) {
@@ -689,6 +711,7 @@ void launch_burgers(int dim, int order, const BurgersArgs& a, hipStream_t s) {
       switch (order) {
         case 1: launch_dp<3, 1, 2>(a, s); return;
         case 2: launch_dp<3, 2, 2>(a, s); return;
+        case 3: launch_dp<3, 3, 2>(a, s); return;
       }
     }
   } else if (dim == 2) {
@@ -703,6 +726,7 @@ void launch_burgers(int dim, int order, const BurgersArgs& a, hipStream_t s) {
     switch (order) {
       case 1: launch_dp<3, 1, 0>(a, s); return;
       case 2: launch_dp<3, 2, 0>(a, s); return;
+      case 3: launch_dp<3, 3, 0>(a, s); return;
     }
   }
   throw std::runtime_error("Burgers kernel: unsupported dimension/order");

@@ -3,6 +3,7 @@
 // loop, resident on one GPU.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <memory>
@@ -35,6 +36,8 @@ struct hfx_burgers {
   bool has_mesh = false, has_fields = false, assembled = false;
   int tau_sides = 1, bc_kind = HFX_BC_INTEGRATED_DIRICHLET, krylov = HFX_KRYLOV_FGMRES;
   int model = 0;            // 0 HDGBurgersModel, 1 HDGnGammaModel, 2 scalar model at m = dim DOFs
+  int num_cu = 0;
+  hfx::DevBuf<double> gws;  // cell workspace of the orders above the LDS (3-D p = 3)
   bool graphs = false;      // hfx_burgers_set_solver_graphs
   double ts_dt = 0.0;       // implicit Euler time step (0: steady)
   hfx::DevBuf<double> old;  // "OldSolution" of the time step
@@ -99,6 +102,15 @@ void assemble(hfx_burgers* h) {
   a.bfield = h->bfield.p;
   a.sound_speed = h->sound_speed;
   a.S = h->S.p; a.S0 = h->S0.p; a.U = h->U.p; a.Q = h->Q.p; a.U0 = h->U0.p; a.Q0 = h->Q0.p;
+  const size_t wsd = hfx::burgers_workspace_doubles(h->dim, h->order, h->model);
+  if (wsd > 0) {
+    // one slab per workgroup, as many as fill the GPU twice over
+    if (h->num_cu <= 0) HFX_HIP(hipDeviceGetAttribute(&h->num_cu, hipDeviceAttributeMultiprocessorCount, h->device));
+    const int slots = (int)std::min<long long>(m.nCells, 2LL * h->num_cu);
+    if (h->gws.n < (size_t)slots * wsd) h->gws.alloc((size_t)slots * wsd);
+    a.gws = h->gws.p;
+    a.gws_slots = (int)(h->gws.n / wsd);
+  }
   hfx::launch_burgers(h->dim, h->order, a, s);
   // trace system: HDGSolver's scatter (Dirichlet rows identity / g), then the
   // IntegratedDirichletModel rows M (x) I and (M (x) I) g
@@ -235,7 +247,7 @@ extern "C" int hfx_burgers_create(int device, int dim, int order, hfx_burgers** 
   HFX_TRY
   need(out != nullptr, "hfx_burgers_create: null output");
   need(hfx::burgers_supported(dim, order),
-       "hfx_burgers_create: the Burgers kernel covers 2-D orders 1-5 and 3-D orders 1-2");
+       "hfx_burgers_create: the Burgers kernel covers 2-D orders 1-5 and 3-D orders 1-3");
   *out = create(device, dim, order, 0);
   HFX_END
 }
@@ -243,7 +255,7 @@ extern "C" int hfx_burgers_create(int device, int dim, int order, hfx_burgers** 
 extern "C" int hfx_multi_create(int device, int dim, int order, hfx_burgers** out) {
   HFX_TRY
   need(out != nullptr, "hfx_multi_create: null output");
-  need(hfx::burgers_supported(dim, order), "hfx_multi_create: the multi-DOF kernel covers 2-D orders 1-5 and 3-D orders 1-2");
+  need(hfx::burgers_supported(dim, order), "hfx_multi_create: the multi-DOF kernel covers 2-D orders 1-5 and 3-D orders 1-3");
   *out = create(device, dim, order, 2);
   HFX_END
 }

@@ -533,7 +533,8 @@ int hfx_hdg_pc_setup_ms(hfx_hdg* h, double* ms);
  * HDGBurgersModel.cpp:118-128).  State: Solution and BufferSolution
  * nCells x nN x m, Flux nCells x nN x dim x m, Trace nFaces x nNf x m; the
  * model linearises about BufferSolution and Trace.  2-D orders 1-5 and 3-D
- * orders 1-2. */
+ * orders 1-3 (3-D p = 3 keeps each cell's working set in a device
+ * workspace: it exceeds the LDS). */
 typedef struct hfx_burgers hfx_burgers;
 int hfx_burgers_create(int device, int dim, int order, hfx_burgers** out);
 void hfx_burgers_destroy(hfx_burgers* h);
@@ -551,7 +552,7 @@ int hfx_ngamma_create(int device, int order, hfx_burgers** out);
  * components on the faces.  Linear: hfx_burgers_assemble and
  * hfx_burgers_solve give the solution (the state it linearises about is not
  * read).  The same handle type and calls as hfx_burgers_create; 2-D orders
- * 1-5, 3-D orders 1-2. */
+ * 1-5, 3-D orders 1-3. */
 int hfx_multi_create(int device, int dim, int order, hfx_burgers** out);
 /* HDGnGammaModel::setFieldMap (HDGnGammaModel.cpp:5-44): Tau, Dirichlet and
  * the source values as for Burgers; D (n block) and G (Gamma block) nNodes

@@ -21,7 +21,8 @@ from test_burgers_oracle import cole_hopf, regression_case  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-CASES = [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2)]
+# 3-D p = 3: the cell's working set (279 KB) is kept in a device workspace
+CASES = [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2), (3, 3)]
 
 
 def rel(a, b):
@@ -148,6 +149,25 @@ def test_stationary_regression_case_matches_oracle(p, damp):
 
 
 @pytest.mark.parametrize('pc', [hf.PC_BJACOBI, hf.PC_JACOBI])
+def test_3d_p3_solve_matches_oracle(pc):
+    """3-D p = 3 (cell working set in a device workspace, block size 30):
+    trace solve and reconstruction against the oracle."""
+    print('burgers solve 3-D p=3', pc, flush=True)
+    mesh, fields, state = workload(3, 3)
+    prob = gpu_problem(3, 3, mesh, fields, state)
+    prob.assemble()
+    it, reason, rr = prob.solve(pc=pc, rtol=1e-13, maxits=5000)
+    assert reason > 0 and rr <= 1e-13
+    sol, buf, flux, tr = prob.state()
+    orc = oracle_problem(3, 3, mesh, fields)
+    orc.assemble(state['buffer'].reshape(mesh.n_cells, -1), state['trace'])
+    lam = orc.solve()
+    u, q = orc.reconstruct(lam)
+    assert rel(tr.reshape(-1), lam) < 1e-9 and rel(sol.reshape(mesh.n_cells, -1), u) < 1e-9
+    assert rel(flux.reshape(mesh.n_cells, -1), q) < 1e-9
+
+
+@pytest.mark.parametrize('pc', [hf.PC_BJACOBI, hf.PC_JACOBI])
 def test_3d_p2_solve_matches_oracle(pc):
     """Block size 18 (3-D p = 2, m = 3): the Jacobi preconditioners and the
     Krylov kernels at that size; ILU(0) refuses the block size at set-up."""
@@ -169,8 +189,8 @@ def test_3d_p2_solve_matches_oracle(pc):
 
 
 def test_checks():
-    with pytest.raises(hf.HfxError, match='2-D orders 1-5 and 3-D orders 1-2'):
-        hf.BurgersProblem(3, 3)
+    with pytest.raises(hf.HfxError, match='2-D orders 1-5 and 3-D orders 1-3'):
+        hf.BurgersProblem(3, 4)
     prob = hf.BurgersProblem(2, 1)
     with pytest.raises(hf.HfxError, match='set the mesh and the fields first'):
         prob.assemble()

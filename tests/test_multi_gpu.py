@@ -48,7 +48,7 @@ def gpu(dim, p, m, tau, D, g, f):
 
 
 @pytest.mark.parametrize('laplace', [False, True])
-@pytest.mark.parametrize('dim,p', [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2)])
+@pytest.mark.parametrize('dim,p', [(2, 1), (2, 2), (2, 3), (2, 4), (2, 5), (3, 1), (3, 2), (3, 3)])
 def test_blocks_and_solution_match_oracle(dim, p, laplace):
     print('multi', dim, p, laplace, flush=True)
     m, tau, D, g, f = workload(dim, p, laplace=laplace)
@@ -113,8 +113,8 @@ def test_linear_fields_exact_with_coupling_tau(dim, p):
 
 
 def test_multi_checks():
-    with pytest.raises(hf.HfxError, match='2-D orders 1-5 and 3-D orders 1-2'):
-        hf.MultiProblem(3, 3)
+    with pytest.raises(hf.HfxError, match='2-D orders 1-5 and 3-D orders 1-3'):
+        hf.MultiProblem(3, 4)
     m, tau, D, g, f = workload(2, 1)
     prob = hf.MultiProblem(2, 1)
     prob.set_mesh(m)
