@@ -191,58 +191,67 @@ int cubature(int dim, int order, vec* pts, vec* w) {
 // Lagrange functions of the node set through a monomial basis inverted in
 // long double; independent of the modal basis the reference uses
 // (ReferenceElement.cpp:269-480), which spans the same P_p.
+// Lagrange basis on the given nodes through the monomials: phi_j = sum_m
+// mono_m W[m][j], W the inverse Vandermonde.  The monomial Vandermonde of the
+// 2-D p = 10 nodes has a condition number near 1e9, so the inversion and the
+// sums run in IEEE quadruple precision (__float128, 113-bit mantissa): the
+// tables then carry full double accuracy at every order (long double left
+// 1e-10 in the p = 10 derivatives).
+typedef __float128 qf;
+static inline qf qabs(qf v) { return v < 0 ? -v : v; }
+
 struct Basis {
   int dim, p, n;
   std::vector<int> ex;             // n x dim exponents
-  std::vector<long double> W;      // n x n, phi_j = sum_m mono_m W[m][j]
+  std::vector<qf> W;               // n x n, phi_j = sum_m mono_m W[m][j]
   void init(int d, int pp, const vec& nodes) {
     dim = d; p = pp; ex.clear();
     for (int a = 0; a <= p; ++a)
       for (int b = 0; b <= (d > 1 ? p - a : 0); ++b)
         for (int c = 0; c <= (d > 2 ? p - a - b : 0); ++c) { ex.push_back(a); if (d > 1) ex.push_back(b); if (d > 2) ex.push_back(c); }
     n = (int)ex.size() / d;
-    std::vector<long double> V(n * n), I(n * n, 0);
+    std::vector<qf> V(n * n), I(n * n, 0);
     for (int i = 0; i < n; ++i) for (int m = 0; m < n; ++m) V[i * n + m] = mono(m, &nodes[i * d]);
     for (int i = 0; i < n; ++i) I[i * n + i] = 1;
     for (int k = 0; k < n; ++k) {   // Gauss-Jordan, partial pivoting
       int pv = k;
-      for (int i = k + 1; i < n; ++i) if (fabsl(V[i * n + k]) > fabsl(V[pv * n + k])) pv = i;
+      for (int i = k + 1; i < n; ++i) if (qabs(V[i * n + k]) > qabs(V[pv * n + k])) pv = i;
       for (int j = 0; j < n; ++j) { std::swap(V[k * n + j], V[pv * n + j]); std::swap(I[k * n + j], I[pv * n + j]); }
-      long double dk = 1.0L / V[k * n + k];
+      qf dk = 1 / V[k * n + k];
       for (int j = 0; j < n; ++j) { V[k * n + j] *= dk; I[k * n + j] *= dk; }
       for (int i = 0; i < n; ++i) if (i != k) {
-        long double f = V[i * n + k];
+        qf f = V[i * n + k];
         if (f != 0) for (int j = 0; j < n; ++j) { V[i * n + j] -= f * V[k * n + j]; I[i * n + j] -= f * I[k * n + j]; }
       }
     }
     W = I;
   }
-  long double mono(int m, const double* x) const {
-    long double v = 1;
-    for (int k = 0; k < dim; ++k) for (int e = 0; e < ex[m * dim + k]; ++e) v *= x[k];
+  qf mono(int m, const double* x) const {
+    qf v = 1;
+    for (int k = 0; k < dim; ++k) for (int e = 0; e < ex[m * dim + k]; ++e) v *= (qf)x[k];
     return v;
   }
   void eval(const double* x, double* phi, double* dphi) const {
-    std::vector<long double> mv(n), mg(n * dim);
+    std::vector<qf> mv(n), mg(n * dim);
     for (int m = 0; m < n; ++m) {
       mv[m] = mono(m, x);
       for (int k = 0; k < dim; ++k) {
         int e = ex[m * dim + k];
-        long double g = e;
+        qf g = e;
         if (e == 0) g = 0;
         for (int l = 0; l < dim; ++l) {
           int el = ex[m * dim + l] - (l == k ? 1 : 0);
-          for (int t = 0; t < el; ++t) g *= x[l];
+          for (int t = 0; t < el; ++t) g *= (qf)x[l];
         }
         mg[m * dim + k] = g;
       }
     }
     for (int j = 0; j < n; ++j) {
-      long double s = 0;
+      qf s = 0;
       for (int m = 0; m < n; ++m) s += mv[m] * W[m * n + j];
       phi[j] = (double)s;
       if (dphi) for (int k = 0; k < dim; ++k) {
-        long double g = 0;
+        qf g = 0;
         for (int m = 0; m < n; ++m) g += mg[m * dim + k] * W[m * n + j];
         dphi[j * dim + k] = (double)g;
       }
@@ -270,7 +279,8 @@ const Ref& ref(int d, int p) {
   std::lock_guard<std::mutex> lk(g_ref_mu);
   {
     if (!g_refs.count(key)) {
-      if (d < 2 || d > 3 || p < 1 || p > 5) throw std::runtime_error("oracle supports dim 2-3, order 1-5");
+      if (d < 2 || d > 3 || p < 1 || p > (d == 2 ? 10 : 5))
+        throw std::runtime_error("oracle supports 2-D orders 1-10 and 3-D orders 1-5");
       auto r = std::make_shared<Ref>();
       r->d = d; r->p = p; r->nF = d + 1;
       r->nodes = nodes_of(d, p);
@@ -670,7 +680,8 @@ void explicit_blocks(const Ref& R, const double* K, const double* F, const doubl
 }
 
 void check_dim(int dim, int p) {
-  if (dim < 2 || dim > 3 || p < 1 || p > 5) throw std::runtime_error("oracle supports dim 2-3 and order 1-5");
+  if (dim < 2 || dim > 3 || p < 1 || p > (dim == 2 ? 10 : 5))
+    throw std::runtime_error("oracle supports 2-D orders 1-10 and 3-D orders 1-5");
 }
 
 }  // namespace

@@ -19,7 +19,7 @@ ELEMS = {
         'rot': [[0.1, 0.2, 0.0], [1.0, 0.4, 0.1], [0.2, 1.1, 0.3], [0.3, 0.2, 0.9]],
         'jit': [[0.01, 0.0, 0.02], [0.26, 0.02, 0.0], [0.24, 0.23, 0.01], [0.26, 0.24, 0.27]]},
 }
-PMAX = {2: 5, 3: 3}
+PMAX = {2: 10, 3: 3}
 CASES = [(d, p, e) for d in (2, 3) for p in range(1, PMAX[d] + 1) for e in ELEMS[d]]
 
 

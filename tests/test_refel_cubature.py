@@ -71,9 +71,16 @@ def test_gll_golden_values():
             # edge {0,1} of the triangle carries the 1-D nodes on y = -1
             edge = tabs['nodes'][3:3 + p - 1, 0]
             assert np.allclose(edge, vals, atol=1e-15, rtol=0)
+    # p = 6..10 (the 2-D orders up to ReferenceElement.cpp:620-623): the
+    # interior Lobatto nodes are the roots of P_p', computed here
+    for p in range(6, 11):
+        gll = np.sort(np.polynomial.legendre.Legendre.basis(p).deriv().roots().real)
+        for tabs in (hf.refel_tables(2, p), orc.refel(2, p)):
+            edge = tabs['nodes'][3:3 + p - 1, 0]
+            assert np.allclose(edge, gll, atol=1e-13, rtol=0), p
 
 
-@pytest.mark.parametrize('dim,pmax', [(2, 5), (3, 5)])
+@pytest.mark.parametrize('dim,pmax', [(2, 10), (3, 5)])
 def test_node_counts_and_faces(dim, pmax):
     for p in range(1, pmax + 1):
         t = hf.refel_tables(dim, p)
@@ -94,7 +101,7 @@ def _monomials(dim, p):
 
 
 @pytest.mark.parametrize('impl', ['product', 'oracle'])
-@pytest.mark.parametrize('dim,pmax', [(2, 5), (3, 5)])
+@pytest.mark.parametrize('dim,pmax', [(2, 10), (3, 5)])
 def test_lagrange_reproduces_monomials(impl, dim, pmax):
     """TestReferenceElement.cpp:146-236: sum_j m(x_j) phi_j = m and
     sum_j m(x_j) grad phi_j = grad m at every IP, for deg(m) <= p."""
@@ -115,10 +122,11 @@ def test_lagrange_reproduces_monomials(impl, dim, pmax):
                 assert np.allclose(np.einsum('qn,n->q', t['dphi'][:, :, k], mv), g, atol=1e-11)
 
 
-@pytest.mark.parametrize('dim,pmax', [(2, 5), (3, 5)])
+@pytest.mark.parametrize('dim,pmax', [(2, 10), (3, 5)])
 def test_product_tables_match_oracle(dim, pmax):
     for p in range(1, pmax + 1):
         a, b = hf.refel_tables(dim, p), orc.refel(dim, p)
         for k in a:
             assert a[k].shape == b[k].shape
-            assert np.allclose(a[k], b[k], atol=1e-13, rtol=0), (dim, p, k)
+            # relative to the table's size: the 2-D p = 10 gradients reach ~1e2
+            assert np.allclose(a[k], b[k], atol=1e-13 * max(1.0, np.abs(b[k]).max()), rtol=0), (dim, p, k)
