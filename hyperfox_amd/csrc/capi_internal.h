@@ -229,7 +229,7 @@ struct hfx_hdg {
   bool use_affine() const {
     if (kernel_choice == 1) return false;
     // the affine kernel keeps a cell's working set in LDS: 2-D p <= 5, 3-D p <= 3
-    const bool ok = affine && !(hasD && ndD == dim * dim) && (dim == 2 || order <= 3);
+    const bool ok = affine && !(hasD && ndD == dim * dim) && order <= (dim == 2 ? 5 : 3);
     if (kernel_choice == 2 && !ok)
       throw std::runtime_error("affine element kernel needs straight-sided cells, a scalar diffusion field and "
                                "2-D p <= 5 or 3-D p <= 3");

@@ -12,7 +12,8 @@ template <int D, int P>
 struct ESz {
   static constexpr int NN = D == 2 ? (P + 1) * (P + 2) / 2 : (P + 1) * (P + 2) * (P + 3) / 6;
   static constexpr int NNF = D == 2 ? P + 1 : (P + 1) * (P + 2) / 2;
-  static constexpr int NIP = D == 2 ? (P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : P == 4 ? 16 : 25)
+  static constexpr int NIP = D == 2 ? (P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : P == 4 ? 16 : P == 5 ? 25
+                                      : P == 6 ? 33 : P == 7 ? 42 : P == 8 ? 55 : P == 9 ? 67 : 79)
                                     : (P == 1 ? 4 : P == 2 ? 14 : P == 3 ? 24 : P == 4 ? 46 : 81);
   static constexpr int NIPF = D == 2 ? P + 1 : (P == 1 ? 3 : P == 2 ? 6 : P == 3 ? 12 : P == 4 ? 16 : 25);
   static constexpr int NF = D + 1;

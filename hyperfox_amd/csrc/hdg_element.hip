@@ -545,6 +545,7 @@ size_t element_lds_bytes(int dim, int order) {
   if (dim == D && order == P)                                                                      \
     return element_in_workspace<D, P>() ? 0 : (size_t)Layout<D, P, 64>::TOTAL * sizeof(double);
   HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
+  HFX_CASE(2, 6) HFX_CASE(2, 7) HFX_CASE(2, 8) HFX_CASE(2, 9) HFX_CASE(2, 10)
   HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3) HFX_CASE(3, 4) HFX_CASE(3, 5)
 #undef HFX_CASE
   throw std::runtime_error("element kernel: (dim, order) not supported on the GPU path");
@@ -554,6 +555,7 @@ size_t element_workspace_doubles(int dim, int order) {
 #define HFX_CASE(D, P) \
   if (dim == D && order == P) return element_in_workspace<D, P>() ? (size_t)Layout<D, P, 64>::TOTAL : 0;
   HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
+  HFX_CASE(2, 6) HFX_CASE(2, 7) HFX_CASE(2, 8) HFX_CASE(2, 9) HFX_CASE(2, 10)
   HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3) HFX_CASE(3, 4) HFX_CASE(3, 5)
 #undef HFX_CASE
   throw std::runtime_error("element kernel: (dim, order) not supported on the GPU path");
@@ -563,6 +565,7 @@ void launch_element(int dim, int order, const ElemArgs& a, int nt, int grid_cap,
 #define HFX_CASE(D, P) \
   if (dim == D && order == P) { launch_dp<D, P>(a, nt, grid_cap, s); return; }
   HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
+  HFX_CASE(2, 6) HFX_CASE(2, 7) HFX_CASE(2, 8) HFX_CASE(2, 9) HFX_CASE(2, 10)
   HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3) HFX_CASE(3, 4) HFX_CASE(3, 5)
 #undef HFX_CASE
   throw std::runtime_error("element kernel: (dim, order) not supported on the GPU path");
@@ -579,6 +582,7 @@ void launch_ip_coords(int dim, int order, int nEl, const double* tabs, const dou
     return;                                                                                              \
   }
   HFX_CASE(2, 1) HFX_CASE(2, 2) HFX_CASE(2, 3) HFX_CASE(2, 4) HFX_CASE(2, 5)
+  HFX_CASE(2, 6) HFX_CASE(2, 7) HFX_CASE(2, 8) HFX_CASE(2, 9) HFX_CASE(2, 10)
   HFX_CASE(3, 1) HFX_CASE(3, 2) HFX_CASE(3, 3) HFX_CASE(3, 4) HFX_CASE(3, 5)
 #undef HFX_CASE
   throw std::runtime_error("ip coords: (dim, order) not supported on the GPU path");

@@ -335,7 +335,7 @@ void bsr_spmv(const BsrDev& A, const double* x, double* y, const double* b, hipS
                        A.vals.p, x, b, y);                                                                \
     break;
   switch (A.bs) {
-    HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
+    HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(7) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(11) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
     default: throw std::runtime_error("spmv: unsupported block size");
   }
 #undef HFX_BS
@@ -350,7 +350,7 @@ void bsr_invert_diag(const BsrDev& A, double* dinv, hipStream_t s) {
                        dinv);                                                                                  \
     break;
   switch (A.bs) {
-    HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
+    HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(7) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(11) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
     default: throw std::runtime_error("block jacobi: unsupported block size");
   }
 #undef HFX_BS
@@ -372,7 +372,7 @@ void apply_block_diag(int nb, int bs, const double* dinv, const double* x, doubl
     hipLaunchKernelGGL((block_diag_apply_kernel<B>), dim3(ceil_div(n, 256)), dim3(256), 0, s, nb, dinv, x, y); \
     break;
   switch (bs) {
-    HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
+    HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(7) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(11) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
     default: throw std::runtime_error("block jacobi: unsupported block size");
   }
 #undef HFX_BS
@@ -600,7 +600,7 @@ void bicgstab(const BsrDev& A, const double* b, double* x, const std::function<v
                        A.vals.p, xin, yout, w1, both ? 1 : 0, fpart);                                    \
     break;
     switch (A.bs) {
-      HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
+      HFX_BS(1) HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(7) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(11) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
       default: throw std::runtime_error("bicgstab: unsupported block size");
     }
 #undef HFX_BS

@@ -321,7 +321,7 @@ void launch_scatter(const BsrDev& A, int nFpc, const double* S, const double* S0
     break;
   if (n > 0) {
     switch (A.bs) {
-      HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
+      HFX_BS(2) HFX_BS(3) HFX_BS(4) HFX_BS(5) HFX_BS(6) HFX_BS(7) HFX_BS(8) HFX_BS(9) HFX_BS(10) HFX_BS(11) HFX_BS(12) HFX_BS(15) HFX_BS(18) HFX_BS(21) HFX_BS(30)
       default: throw std::runtime_error("scatter: unsupported block size");
     }
     HFX_HIP(hipGetLastError());

@@ -36,17 +36,18 @@ def test_errors_are_reported_not_raised_through_the_abi():
 def test_unsupported_gpu_orders_rejected_before_any_device_call():
     v = ctypes.c_longlong()
     assert hf.lib.hfx_element_lds_bytes(3, 6, ctypes.byref(v)) == -1
-    assert hf.lib.hfx_element_lds_bytes(2, 6, ctypes.byref(v)) == -1
+    assert hf.lib.hfx_element_lds_bytes(2, 11, ctypes.byref(v)) == -1
     assert hf.lib.hfx_element_workspace_bytes(3, 6, ctypes.byref(v)) == -1
     assert hf.lib.hfx_element_lds_bytes(3, 3, ctypes.byref(v)) == 0 and v.value > 0
     assert hf.lib.hfx_element_workspace_bytes(3, 3, ctypes.byref(v)) == 0 and v.value == 0
 
 
-@pytest.mark.parametrize('p', [4, 5])
-def test_3d_high_orders_keep_the_element_in_a_device_workspace(p):
-    # 3-D p = 4, 5 (ReferenceElement.cpp:623): the element's working set
-    # exceeds the 160 KB of LDS, so the kernel takes a per-workgroup slab of
-    # device memory instead
+@pytest.mark.parametrize('dim,p', [(3, 4), (3, 5), (2, 7), (2, 8), (2, 9), (2, 10)])
+def test_high_orders_keep_the_element_in_a_device_workspace(dim, p):
+    # 3-D p = 4, 5 and 2-D p >= 7 (ReferenceElement.cpp:620-623): the
+    # element's working set exceeds the 160 KB of LDS, so the kernel takes a
+    # per-workgroup slab of device memory instead (2-D p = 6 still fits)
     v = ctypes.c_longlong()
-    assert hf.lib.hfx_element_lds_bytes(3, p, ctypes.byref(v)) == 0 and v.value == 0
-    assert hf.lib.hfx_element_workspace_bytes(3, p, ctypes.byref(v)) == 0 and v.value > 160 * 1024
+    assert hf.lib.hfx_element_lds_bytes(dim, p, ctypes.byref(v)) == 0 and v.value == 0
+    assert hf.lib.hfx_element_workspace_bytes(dim, p, ctypes.byref(v)) == 0 and v.value > 160 * 1024
+    assert hf.lib.hfx_element_lds_bytes(2, 6, ctypes.byref(v)) == 0 and 0 < v.value <= 160 * 1024

@@ -17,9 +17,10 @@ from hdg_helpers import l2_projection_error, refined_sparse_solve, rel
 pytestmark = pytest.mark.gpu
 
 CASES = [(2, p) for p in range(1, 6)] + [(3, p) for p in range(1, 4)]
-# 3-D p = 4, 5: the general kernel with the element's working set in a device
-# workspace (it exceeds the LDS); no affine kernel at these orders
-HIGH = [(3, 4), (3, 5)]
+# 3-D p = 4, 5 and 2-D p = 6..10 (ReferenceElement.cpp:620-623): the general
+# kernel, with the element's working set in a device workspace where it
+# exceeds the LDS; no affine kernel at these orders
+HIGH = [(3, 4), (3, 5)] + [(2, p) for p in range(6, 11)]
 CASES_ALL = CASES + HIGH
 
 
@@ -83,6 +84,18 @@ def oracle_local(w, f, e):
     return orc.local_system(w.dim, w.p, w.kind, X, D=D, ndD=w.ndD, b=b, tau=tau.reshape(-1), f_ip=fe, r_ip=re)
 
 
+def condense_bar(w, f, base):
+    """The condensed blocks' bar: `base`, or 2e-15 cond(local u-q block) where
+    that is larger -- the forward error of either side's elimination grows
+    with it: the worst 2-D cells reach cond 3e3 at p = 5, 2e4 at p = 9 and
+    3.5e4 at p = 10 (measured U distance 1.3e-11 at p = 9)."""
+    if (w.dim, w.p) not in HIGH:
+        return base
+    n = w.mesh.sizes['nN'] * (w.dim + 1)
+    kappa = max(np.linalg.cond(oracle_local(w, f, e)[0][:n, :n]) for e in range(w.mesh.n_cells))
+    return max(base, 2e-15 * kappa)
+
+
 @pytest.mark.parametrize('dim,p', CASES_ALL)
 def test_ip_coords(dim, p):
     w = small_workload(dim, p, 1)
@@ -122,13 +135,14 @@ def test_condensed_blocks_match_oracle(dim, p, kind, kernel):
     assert info['kernel'] == ('general' if (kernel == 1 or kind == 1 or (dim, p) in HIGH) else 'affine')
     prob.assemble()
     g = prob.condensed()
+    bar = condense_bar(w, f, 1e-11)
     for e in range(w.mesh.n_cells):
         Ko, Fo = oracle_local(w, f, e)
         c = orc.condense(dim, p, Ko, Fo)
         for k in ('S', 'U', 'Q'):
-            assert rel(g[k][e], c[k]) < 1e-11, (e, k, rel(g[k][e], c[k]))
+            assert rel(g[k][e], c[k]) < bar, (e, k, rel(g[k][e], c[k]))
         for k in ('S0', 'U0', 'Q0'):
-            assert np.allclose(g[k][e], c[k], rtol=1e-10, atol=1e-12 * max(1, np.abs(c[k]).max())), (e, k)
+            assert np.allclose(g[k][e], c[k], rtol=1e-10, atol=bar / 10 * max(1, np.abs(c[k]).max())), (e, k)
 
 
 @pytest.mark.parametrize('kind', [0, 1, 2])
@@ -171,7 +185,7 @@ def test_fused_scatter_equals_stored_scatter(dim, p, kind):
 
 @pytest.mark.parametrize('pc', [hf.PC_NONE, hf.PC_JACOBI, hf.PC_BJACOBI])
 @pytest.mark.parametrize('kind', [1, 2])
-@pytest.mark.parametrize('dim,p', [(2, 1), (2, 3), (2, 5), (3, 1), (3, 3), (3, 4), (3, 5)])
+@pytest.mark.parametrize('dim,p', [(2, 1), (2, 3), (2, 5), (2, 7), (2, 10), (3, 1), (3, 3), (3, 4), (3, 5)])
 def test_solution_matches_oracle(dim, p, kind, pc):
     w = small_workload(dim, p, kind)
     prob, f = run_gpu(w)
