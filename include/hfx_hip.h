@@ -333,7 +333,8 @@ int hfx_hdg_time(hfx_hdg* h, int what, int warmup, int reps, double* ms);
  * for the orders (3-D p = 4, 5) whose element working set does not fit the
  * LDS and lives in a device workspace instead (hfx_element_workspace_bytes).
  * The reference supports 2-D p <= 10 and 3-D p <= 5
- * (ReferenceElement.cpp:620-623); the GPU path 2-D p <= 5, 3-D p <= 5. */
+ * (ReferenceElement.cpp:620-623), and so does the GPU path (2-D p >= 7 and
+ * 3-D p >= 4 in the device workspace). */
 int hfx_element_lds_bytes(int dim, int order, long long* bytes);
 /* Device workspace bytes per workgroup of the general element kernel of
  * (dim, order) (0 when its working set is in LDS); a problem allocates it for
