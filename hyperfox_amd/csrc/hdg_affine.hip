@@ -145,6 +145,58 @@ __device__ inline int dpp_argmax(double v, bool valid) {
   return 63 - (int)(best & 63u);
 }
 
+// Next-cell gather held in registers, one entry per lane: fetch_idx issues
+// the loads that depend on the cell number only (cell nodes, faces, face
+// positions, BSR blocks, cubature-point data), fetch_data the loads through
+// those indices (coordinates, D, b, tau on both sides, face owners,
+// Dirichlet flags).  A wave issues fetch_idx for its next cell once the
+// current cell's gather is in LDS and fetch_data late in the current cell,
+// so both land while the current cell computes and the top of the next
+// iteration only writes them to LDS.
+template <int D, int P>
+struct GatherPf {
+  using Z = ESz<D, P>;
+  static constexpr int NN = Z::NN, NNF = Z::NNF, NIP = Z::NIP, NF = Z::NF;
+  static_assert((D + 1) * D <= 64 && NN * D <= 64 && NF * NNF <= 64 && NF * NF <= 64 && NIP <= 64,
+                "one gathered entry per lane");
+  int cn, cd, fc, pm, bp, fx;  // node of entry lane / D, node lane; face, position of entry lane; block; face lane
+  double fq, rq, vx, dn, bn, t0, t1;
+  int s2, s2x, dr;             // owner cell of face fc and of face fx, Dirichlet flag of fx
+
+  __device__ __forceinline__ void fetch_idx(const ElemArgs& a, int e, int lane, bool fused) {
+    const int* cell = a.cells + (size_t)e * NN;
+    if (lane < NN * D) cn = cell[lane / D];
+    if (lane < NN && a.ndD == 1) cd = cell[lane];
+    if (lane < NF * NNF) {
+      fc = a.cell2face[(size_t)e * NF + lane / NNF];
+      pm = a.face_perm[(size_t)e * NF * NNF + lane];
+    }
+    if (fused) {
+      if (lane < NF * NF) bp = a.bpos[(size_t)e * NF * NF + lane];
+      if (lane < NF) fx = a.cell2face[(size_t)e * NF + lane];
+    }
+    fq = (lane < NIP && a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + lane] : 0.0;
+    rq = (lane < NIP && a.hasR) ? a.r_ip[(size_t)e * NIP + lane] : 0.0;
+  }
+  __device__ __forceinline__ void fetch_data(const ElemArgs& a, int lane, bool fused) {
+    if (lane < (D + 1) * D) vx = a.coords[(size_t)cn * D + lane % D];
+    if (lane < NN * D && a.hasB) bn = a.bn[(size_t)cn * D + lane % D];
+    if (lane < NN && a.ndD == 1) dn = a.Dn[cd];
+    if (lane < NF * NNF) {
+      const size_t ti = ((size_t)fc * NNF + pm) * a.tau_sides;
+      t0 = a.tau[ti];
+      if (a.tau_sides == 2) {
+        t1 = a.tau[ti + 1];
+        s2 = a.face2cell[2 * (size_t)fc];
+      }
+    }
+    if (fused && lane < NF) {
+      s2x = a.face2cell[2 * (size_t)fx];
+      dr = a.is_dir ? a.is_dir[fx] : 0;
+    }
+  }
+};
+
 template <int D, int P, bool TD, int W>
 __device__ __forceinline__ void affine_load_tables(const double* __restrict__ tabs, double* sm, int* s_pos,
                                                    int* s_fn) {
@@ -208,44 +260,83 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
   int* IX = reinterpret_cast<int*>(ws + L::O_IX);
   const bool fused = a.bvals != nullptr;
 
+  // The gather is software-pipelined across a wave's cells (GatherPf) at the
+  // orders that run one wave per SIMD anyway; below them the prefetch
+  // registers would cost the second wave that hides the gather already.
+  GatherPf<D, P> pf;
+  constexpr bool PIPE = D == 3 ? P >= 2 : P >= 4;
+  if (PIPE && e_first + wv < e_end) {
+    pf.fetch_idx(a, e_first + wv, lane, fused);
+    pf.fetch_data(a, lane, fused);
+  }
   for (int e = e_first + wv; e < e_end; e += e_stride) {
-    const int* cell = a.cells + (size_t)e * NN;
     // ------------------------------------------------------------ gather
-    for (int t = lane; t < (D + 1) * D; t += 64) VX[t] = a.coords[(size_t)cell[t / D] * D + t % D];
-    for (int t = lane; t < NN * ND; t += 64) {
-      const int i = t / ND, c = t % ND;
-      double v;
-      (void)c;
-      v = a.ndD == 1 ? a.Dn[cell[i]] : (a.useDiff ? 1.0 : 0.0);
-      DN[t] = v;
-    }
-    for (int t = lane; t < NN * D; t += 64) BN[t] = a.hasB ? a.bn[(size_t)cell[t / D] * D + t % D] : 0.0;
-    for (int t = lane; t < NF * NNF; t += 64) {
-      const int f = t / NNF, j = t % NNF;
-      const int fc = a.cell2face[(size_t)e * NF + f];
-      const int side = (a.tau_sides == 2 && a.face2cell[2 * (size_t)fc] != e) ? 1 : 0;
-      const int p = a.face_perm[((size_t)e * NF + f) * NNF + j];
-      TAU[t] = a.tau[((size_t)fc * NNF + p) * a.tau_sides + side];
-      IX[L::IX_PERM + t] = p;
-    }
-    if (fused) {
-      for (int t = lane; t < NF * NF; t += 64) IX[L::IX_BPOS + t] = a.bpos[(size_t)e * NF * NF + t];
-      for (int t = lane; t < NF; t += 64) {
-        const int fc = a.cell2face[(size_t)e * NF + t];
-        IX[L::IX_FACE + t] = fc;
-        IX[L::IX_FLAG + t] = (a.face2cell[2 * (size_t)fc] == e ? 0 : 1) | (a.is_dir && a.is_dir[fc] ? 2 : 0);
+    if constexpr (!PIPE) {
+      const int* cell = a.cells + (size_t)e * NN;
+      for (int t = lane; t < (D + 1) * D; t += 64) VX[t] = a.coords[(size_t)cell[t / D] * D + t % D];
+      for (int t = lane; t < NN * ND; t += 64) {
+        const int i = t / ND, c = t % ND;
+        double v;
+        (void)c;
+        v = a.ndD == 1 ? a.Dn[cell[i]] : (a.useDiff ? 1.0 : 0.0);
+        DN[t] = v;
+      }
+      for (int t = lane; t < NN * D; t += 64) BN[t] = a.hasB ? a.bn[(size_t)cell[t / D] * D + t % D] : 0.0;
+      for (int t = lane; t < NF * NNF; t += 64) {
+        const int f = t / NNF, j = t % NNF;
+        const int fc = a.cell2face[(size_t)e * NF + f];
+        const int side = (a.tau_sides == 2 && a.face2cell[2 * (size_t)fc] != e) ? 1 : 0;
+        const int p = a.face_perm[((size_t)e * NF + f) * NNF + j];
+        TAU[t] = a.tau[((size_t)fc * NNF + p) * a.tau_sides + side];
+        IX[L::IX_PERM + t] = p;
+      }
+      if (fused) {
+        for (int t = lane; t < NF * NF; t += 64) IX[L::IX_BPOS + t] = a.bpos[(size_t)e * NF * NF + t];
+        for (int t = lane; t < NF; t += 64) {
+          const int fc = a.cell2face[(size_t)e * NF + t];
+          IX[L::IX_FACE + t] = fc;
+          IX[L::IX_FLAG + t] = (a.face2cell[2 * (size_t)fc] == e ? 0 : 1) | (a.is_dir && a.is_dir[fc] ? 2 : 0);
+        }
+      }
+      for (int t = lane; t < NIP; t += 64) {
+        double fq = (a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + t] : 0.0;
+        if (a.u_old && a.ts_idt != 0.0) {  // implicit Euler: + u_old(x_q) / dt
+          double uo = 0.0;
+          for (int j = 0; j < NN; ++j) uo += PHI[t * NN + j] * a.u_old[(size_t)e * NN + j];
+          fq += a.ts_idt * uo;
+        }
+        FQ[t] = fq;
+        RQ[t] = (a.hasR ? a.r_ip[(size_t)e * NIP + t] : 0.0) + a.ts_idt;
+      }
+    } else {
+      if (lane < (D + 1) * D) VX[lane] = pf.vx;
+      if (lane < NN) DN[lane] = a.ndD == 1 ? pf.dn : (a.useDiff ? 1.0 : 0.0);
+      if (lane < NN * D) BN[lane] = a.hasB ? pf.bn : 0.0;
+      if (lane < NF * NNF) {
+        TAU[lane] = (a.tau_sides == 2 && pf.s2 != e) ? pf.t1 : pf.t0;
+        IX[L::IX_PERM + lane] = pf.pm;
+      }
+      if (fused) {
+        if (lane < NF * NF) IX[L::IX_BPOS + lane] = pf.bp;
+        if (lane < NF) {
+          IX[L::IX_FACE + lane] = pf.fx;
+          IX[L::IX_FLAG + lane] = (pf.s2x == e ? 0 : 1) | (pf.dr ? 2 : 0);
+        }
+      }
+      if (lane < NIP) {
+        double fq = pf.fq;
+        if (a.u_old && a.ts_idt != 0.0) {  // implicit Euler: + u_old(x_q) / dt
+          double uo = 0.0;
+          for (int j = 0; j < NN; ++j) uo += PHI[lane * NN + j] * a.u_old[(size_t)e * NN + j];
+          fq += a.ts_idt * uo;
+        }
+        FQ[lane] = fq;
+        RQ[lane] = pf.rq + a.ts_idt;
       }
     }
-    for (int t = lane; t < NIP; t += 64) {
-      double fq = (a.hasF && a.kind != 0) ? a.f_ip[(size_t)e * NIP + t] : 0.0;
-      if (a.u_old && a.ts_idt != 0.0) {  // implicit Euler: + u_old(x_q) / dt
-        double uo = 0.0;
-        for (int j = 0; j < NN; ++j) uo += PHI[t * NN + j] * a.u_old[(size_t)e * NN + j];
-        fq += a.ts_idt * uo;
-      }
-      FQ[t] = fq;
-      RQ[t] = (a.hasR ? a.r_ip[(size_t)e * NIP + t] : 0.0) + a.ts_idt;
-    }
+    const int en = e + e_stride;
+    const bool more = en < e_end;
+    if (PIPE && more) pf.fetch_idx(a, en, lane, fused);
     wave_sync();
     // ------------------------------------------------ element geometry
     // J[r][c] = dx_c/dxi_r (transposed convention of Operator.cpp:31-36)
@@ -851,6 +942,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       }
     }
     wave_sync();  // QN complete before the S rows read it
+    if (PIPE && more) pf.fetch_data(a, lane, fused);
     // S rows of face f on FP64 MFMA: [Slu_f | Slq_f] (NNF x 2 NNF) times
     // [U at the face nodes; n.q at the face nodes] (2 NNF x NCOL), i.e.
     // S[(f, aa)][c] = sum_b FMt[aa][b] U[fn_b][c] - FMd[aa][b] QN[(f, b)][c],
