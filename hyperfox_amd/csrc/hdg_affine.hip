@@ -32,6 +32,7 @@ struct RefConst;
 namespace {
 
 typedef double dvec4 __attribute__((ext_vector_type(4)));
+typedef double dvec2 __attribute__((ext_vector_type(2)));
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -77,12 +78,15 @@ struct ALay {
   static constexpr int T_AH = T_FW + NIPF;
   static constexpr int T_G = T_AH + D * NN * NN;
   static constexpr int TAB_ALL = T_G + NF * NN * NNF;
-  static constexpr int TAB = TAB_ALL;              // all staged in LDS, shared by the waves
+  // all staged in LDS, shared by the waves; the per-wave scratch starts at
+  // an even offset (with O_KM and PER even, the LU factors are 16-byte
+  // aligned for paired reads)
+  static constexpr int TAB = (TAB_ALL + 1) & ~1;
   // per-wave scratch
   static constexpr int NSF = NS;                  // LDS face masses: tau, b.n, D
   static constexpr int O_FM = 0;
-  static constexpr int O_SUQ = O_FM + NF * NSF * NNF * NNF;
-  static constexpr int O_KM = O_SUQ + D * NN * NN;  // Suu, then K, then its LU factors
+  static constexpr int O_SUQ = (O_FM + NF * NSF * NNF * NNF + 1) & ~1;
+  static constexpr int O_KM = (O_SUQ + D * NN * NN + 1) & ~1;  // Suu, then K, then its LU factors
   // after the U solve, Suq and the K factors are dead: the face traces n.q of the Q
   // columns take their place
   static constexpr int NQN = NF * NNF * NCOL;
@@ -117,7 +121,7 @@ struct ALay {
   // overlay, phase 2: R columns, overwritten in place by the U columns
   static constexpr int O_RB = O_OV;
   static constexpr int END2 = O_RB + NN * NCOL;
-  static constexpr int PER = END1 > END2 ? END1 : END2;
+  static constexpr int PER = ((END1 > END2 ? END1 : END2) + 1) & ~1;
   static_assert(NF * NIPF * NS >= NCOL, "the face-IP weights hold one dead row of trace columns");
 };
 
@@ -202,7 +206,7 @@ __device__ __forceinline__ void affine_load_tables(const double* __restrict__ ta
                                                    int* s_fn) {
   using L = ALay<D, P, TD>;
   using RC = RefConst<D, P>;
-  for (int t = threadIdx.x; t < L::TAB; t += 64 * W) sm[t] = tabs[t];
+  for (int t = threadIdx.x; t < L::TAB_ALL; t += 64 * W) sm[t] = tabs[t];
   // s_pos[i]: byte f = position of node i on face f, 0xFF when i is not on f
   static_assert(L::NF <= 4 && L::NNF < 255, "face positions packed in one word");
   for (int t = threadIdx.x; t < L::NN; t += 64 * W) {
@@ -694,9 +698,11 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         for (int j = k + 1; j < NN; ++j) kr[j] = fma(-l, readlane_d(kr[j], p), kr[j]);
         kr[k] = elim ? l : kr[k];
       }
+      // stored column by column, KM[j * NN + k] = LU[k][j]: a substitution
+      // step reads one contiguous column
       if (lane < NN) {
 #pragma unroll
-        for (int j = 0; j < NN; ++j) KM[myk * NN + j] = kr[j];
+        for (int j = 0; j < NN; ++j) KM[j * NN + myk] = kr[j];
         ws[L::O_RD + myk] = myrd;
       }
     }
@@ -814,17 +820,38 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       for (int k = 0; k < NN; ++k) y[k] = RB[PL[k] * NCOL + c];
       // column-oriented (right-looking) substitution: once y[k] is final,
       // every later row takes its update at once, so each step issues
-      // independent FMAs and the dependent chain is NN long, not NN^2 / 2
+      // independent FMAs and the dependent chain is NN long, not NN^2 / 2;
+      // the factor column of the next step is read while the current
+      // step's FMAs run
+      // (factor columns read as aligned pairs: KM is 16-byte aligned)
+      double fc[NN], fn[NN];
+      auto column = [&](int k, int lo, int hi, double* f) {  // f[i] = LU[i][k], lo <= i < hi
 #pragma unroll
-      for (int k = 0; k < NN - 1; ++k)
+        for (int t = (k * NN + lo) & ~1; t < k * NN + hi; t += 2) {
+          const dvec2 v = *reinterpret_cast<const dvec2*>(KM + t);
+          if (t >= k * NN + lo) f[t - k * NN] = v.x;
+          if (t + 1 < k * NN + hi) f[t + 1 - k * NN] = v.y;
+        }
+      };
+      column(0, 1, NN, fc);
 #pragma unroll
-        for (int i = k + 1; i < NN; ++i) y[i] = fma(-KM[i * NN + k], y[k], y[i]);
+      for (int k = 0; k < NN - 1; ++k) {
+        if (k + 2 < NN) column(k + 1, k + 2, NN, fn);
+#pragma unroll
+        for (int i = k + 1; i < NN; ++i) y[i] = fma(-fc[i], y[k], y[i]);
+#pragma unroll
+        for (int i = k + 2; i < NN; ++i) fc[i] = fn[i];
+      }
+      column(NN - 1, 0, NN - 1, fc);
 #pragma unroll
       for (int kk = 0; kk < NN; ++kk) {
         const int k = NN - 1 - kk;
+        if (k >= 2) column(k - 1, 0, k - 1, fn);
         y[k] *= RDp[k];
 #pragma unroll
-        for (int i = 0; i < k; ++i) y[i] = fma(-KM[i * NN + k], y[k], y[i]);
+        for (int i = 0; i < k; ++i) y[i] = fma(-fc[i], y[k], y[i]);
+#pragma unroll
+        for (int i = 0; i + 1 < k; ++i) fc[i] = fn[i];
       }
 #pragma unroll
       for (int i = 0; i < NN; ++i) UB[i * NCOL + c] = -y[i];
@@ -1072,7 +1099,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 template <int D, int P, bool TD, int W, bool TSG>
 __global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
   using L = ALay<D, P, TD>;
-  extern __shared__ double sm[];
+  extern __shared__ __align__(16) double sm[];
   __shared__ int s_pos[L::NN];
   __shared__ int s_fn[L::NF * L::NNF];
   affine_load_tables<D, P, TD, W>(a.tabs, sm, s_pos, s_fn);
@@ -1089,7 +1116,7 @@ template <int P, int W>
 __global__ __launch_bounds__(64 * W) void hdg_affine_batch_kernel(const ElemArgs* __restrict__ probs,
                                                                   const BatchItem* __restrict__ items, int nitems) {
   using L = ALay<2, P, false>;
-  extern __shared__ double sm[];
+  extern __shared__ __align__(16) double sm[];
   __shared__ int s_pos[L::NN];
   __shared__ int s_fn[L::NF * L::NNF];
   if (blockIdx.x >= nitems) return;
