@@ -430,6 +430,7 @@ def batched(hf, pr, count=64, seed=7):
     best = b.tune(reps=3)
     cfg = {str(k): v for k, v in b.config().items()}
     cfg['concurrent_orders'] = b.concurrent
+    cfg['concurrent_orders_in_graph'] = b.graph_concurrent
     out = {'workload': '%d 2-D HDG diffusion problems, %d cells, orders %s' % (
         count, cells, sorted({sp['p'] for sp in specs})), 'u_dofs': dofs, 'tuned': cfg, 'tune_best_kernel_ms': best}
     for name, what in (('kernel_only', 0), ('eager', 1), ('graph', 2), ('one_by_one', 3)):

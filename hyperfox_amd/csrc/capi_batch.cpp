@@ -44,7 +44,8 @@ struct hfx_batch {
   int grid[NORD] = {};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  bool concurrent = true;  // order kernels on side streams (chosen by the tuner)
+  bool concurrent = true;        // order kernels on side streams (chosen by the tuner)
+  bool graph_concurrent = true;  // the same inside the captured graph (timed separately)
 
   void set_device() const { HFX_HIP(hipSetDevice(device)); }
 
@@ -142,15 +143,19 @@ struct hfx_batch {
   void capture() {
     if (exec) return;
     build_items();
+    const bool eager = concurrent;
+    concurrent = graph_concurrent;
     HFX_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
     try {
       run_all();
     } catch (...) {
+      concurrent = eager;
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(stream, &g);
       if (g) (void)hipGraphDestroy(g);
       throw;
     }
+    concurrent = eager;
     HFX_HIP(hipStreamEndCapture(stream, &graph));
     HFX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
   }
@@ -262,7 +267,7 @@ extern "C" int hfx_batch_set_config(hfx_batch* b, int order, int waves, int tile
 }
 
 // out[4 * (p - 1) + 0..3] = waves, tile, work items, workgroups of order p (1..5);
-// out[20] = 1 when the orders run concurrently
+// out[20] bit 0: the orders run concurrently, bit 1: they do in the graph
 extern "C" int hfx_batch_get_config(hfx_batch* b, int* out) {
   B_TRY
   need(b && out, "null argument");
@@ -274,14 +279,15 @@ extern "C" int hfx_batch_get_config(hfx_batch* b, int* out) {
     out[4 * (p - 1) + 2] = (int)b->items[p].size();
     out[4 * (p - 1) + 3] = b->has(p) ? b->grid[p] : 0;
   }
-  out[20] = b->concurrent ? 1 : 0;
+  out[20] = (b->concurrent ? 1 : 0) | (b->graph_concurrent ? 2 : 0);
   B_END
 }
 
 // Time every (waves, tile) candidate of each order's kernel on this batch
 // (alone on the batch stream) and keep the fastest per order, then choose
-// between running the orders concurrently or one after another; best_ms
-// returns the tuned launch of all orders.
+// between running the orders concurrently or one after another, eagerly and
+// (timed on its own: a graph's parallel branches schedule differently) in
+// the replayed graph; best_ms returns the tuned eager launch of all orders.
 extern "C" int hfx_batch_tune(hfx_batch* b, int reps, double* best_ms) {
   B_TRY
   need(b != nullptr && !b->probs.empty(), "empty batch");
@@ -334,6 +340,15 @@ extern "C" int hfx_batch_tune(hfx_batch* b, int reps, double* best_ms) {
   const double conc = timed([&] { b->run_kernels(); });
   b->concurrent = conc <= serial;
   const double all = std::min(conc, serial);
+  double gms[2];
+  for (int c = 0; c < 2; ++c) {
+    b->drop_graph();
+    b->graph_concurrent = c == 1;
+    b->capture();
+    gms[c] = timed([&] { HFX_HIP(hipGraphLaunch(b->exec, b->stream)); });
+  }
+  b->drop_graph();
+  b->graph_concurrent = gms[1] <= gms[0];
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (best_ms) *best_ms = all;

@@ -131,13 +131,12 @@ struct ALay {
 // row (row_shr 1, 2, 4, 8) leaves each row's maximum in its last lane, and
 // two readlanes combine rows 0 and 1.
 __device__ inline int dpp_argmax(double v, bool valid) {
-  unsigned key = 0u;
-  if (valid) {
-    float fv = (float)v;
-    if (!(fv >= 0.0f)) fv = 0.0f;
-    key = (__float_as_uint(fv) & ~63u) | (63u - (unsigned)(threadIdx.x & 63));
-    key += 1u << 6;
-  }
+  // branch-free, so that the LU's steps stay one basic block the compiler
+  // can overlap
+  float fv = (float)v;
+  fv = fv >= 0.0f ? fv : 0.0f;
+  unsigned key = ((__float_as_uint(fv) & ~63u) | (63u - (unsigned)(threadIdx.x & 63))) + (1u << 6);
+  key = valid ? key : 0u;
   unsigned m = key;
   m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x111, 0xF, 0xF, true));
   m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x112, 0xF, 0xF, true));
@@ -449,22 +448,39 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       constexpr int KSF = (NIPF + 3) / 4;
       static_assert(NNF <= 16, "one MFMA tile per face mass");
       const int li = lane & 15, lk = lane >> 4;
-#pragma unroll 1
+      // psi_a(q) is the same on every face: read once, at clamped (in-range)
+      // addresses with the padding zeroed after the load (no exec-masked
+      // loads, so the reads pipeline); the faces' products run back to back
+      double ps[KSF];
+#pragma unroll
+      for (int ks = 0; ks < KSF; ++ks) {
+        const int q = ks * 4 + lk;
+        const double x = FPHI[(q < NIPF ? q : 0) * NNF + (li < NNF ? li : 0)];
+        ps[ks] = (q < NIPF && li < NNF) ? x : 0.0;
+      }
+      // the next face's weights are read before this face's masses are
+      // stored (the compiler cannot move reads past the stores)
+      double wt[2][KSF][NSF];
+      auto wload = [&](int f, double (*w)[NSF]) {
+#pragma unroll
+        for (int ks = 0; ks < KSF; ++ks) {
+          const int q = ks * 4 + lk, qc = q < NIPF ? q : 0;
+#pragma unroll
+          for (int sf = 0; sf < NSF; ++sf) w[ks][sf] = FWT[(f * NIPF + qc) * NS + sf];  // ps is zero on padding q
+        }
+      };
+      wload(0, wt[0]);
+#pragma unroll
       for (int f = 0; f < NF; ++f) {
+        if (f + 1 < NF) wload(f + 1, wt[(f + 1) & 1]);
         dvec4 acc[NSF];
 #pragma unroll
         for (int sf = 0; sf < NSF; ++sf) acc[sf] = dvec4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int ks = 0; ks < KSF; ++ks) {
-          const int q = ks * 4 + lk;
-          const bool v = q < NIPF && li < NNF;
-          const double ps = v ? FPHI[q * NNF + li] : 0.0;
+        for (int ks = 0; ks < KSF; ++ks)
 #pragma unroll
-          for (int sf = 0; sf < NSF; ++sf) {
-            const double w = v ? FWT[(f * NIPF + q) * NS + sf] : 0.0;
-            acc[sf] = __builtin_amdgcn_mfma_f64_16x16x4f64(ps * w, ps, acc[sf], 0, 0, 0);
-          }
-        }
+          for (int sf = 0; sf < NSF; ++sf)
+            acc[sf] = __builtin_amdgcn_mfma_f64_16x16x4f64(ps[ks] * wt[f & 1][ks][sf], ps[ks], acc[sf], 0, 0, 0);
 #pragma unroll
         for (int sf = 0; sf < NSF; ++sf)
 #pragma unroll
@@ -474,18 +490,16 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           }
       }
     }
-    for (int i = lane; i < NN; i += 64) {
-      double s = 0.0;
-      for (int q = 0; q < NIP; ++q) s += WF[q] * PHI[q * NN + i];
-      FU[i] = s;
-    }
     wave_sync();
     // ----------------------------------------- Suq_k and Suu (entries)
     // On FP64 MFMA over the cubature points: Suq_k = G_k PHI^T and
     // Suu = G_u PHI^T with G_k[i][q] = SD[q] sum_c Jinv[k][c] dphi_c(i, q) and
     // G_u[i][q] = WR[q] phi_i(q) - sum_k BJ[q][k] dphi_k(i, q), K = NIP;
     // then the face terms, and Suq moved to the reference frame
-    // SUQ_a = sum_k Jinv[k][a] Suq_k.
+    // SUQ_a = sum_k Jinv[k][a] Suq_k.  The first padding row of G_u carries
+    // the source weights WF[q], so that the same product gives
+    // F_u[j] = sum_q WF[q] phi_j(q) in row NN of its result.
+    static_assert(NN % 16 != 0, "a padding row for F_u");
     {
       constexpr int MT = (NN + 15) / 16, KS = (NIP + 3) / 4, NX = D + 1;
       const int li = lane & 15, lk = lane >> 4;
@@ -508,7 +522,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           double g[D], bj[D];
           for (int c2 = 0; c2 < D; ++c2) g[c2] = DPHI[(qq * NN + ic) * D + c2];
           for (int k = 0; k < D; ++k) bj[k] = BJ[qq * D + k];
-          const double sd = SD[qq], wr = WR[qq], ph = PHI[qq * NN + ic];
+          const double sd = SD[qq], wr = WR[qq], ph = PHI[qq * NN + ic], wf = WF[qq];
           double bgs = 0.0;
           for (int k = 0; k < D; ++k) {
             double s2 = 0.0;
@@ -516,7 +530,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
             av[ks][k] = v ? sd * s2 : 0.0;
             bgs += bj[k] * g[k];
           }
-          av[ks][D] = v ? wr * ph - bgs : 0.0;
+          av[ks][D] = v ? wr * ph - bgs : ((ia == NN && q < NIP) ? wf : 0.0);
         }
 #pragma unroll 1
         for (int nt = 0; nt < MT; ++nt) {
@@ -536,6 +550,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = m * 16 + 4 * r + lk, j = jb;
+            if (i == NN && j < NN) FU[j] = acc[D][r];
             if (i >= NN || j >= NN) continue;
             double aq[D], au = acc[D][r];
             for (int k = 0; k < D; ++k) aq[k] = acc[k][r];
@@ -686,13 +701,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         const bool cand = lane < NN && !((used >> lane) & 1ull);
         const int p = __builtin_amdgcn_readfirstlane(dpp_argmax(fabs(kr[k]), cand));
         used |= 1ull << p;
-        if (lane == 0) PL[k] = p;
         const bool elim = cand && lane != p;
         const double rp = 1.0 / readlane_d(kr[k], p);
-        if (lane == p) {
-          myk = k;
-          myrd = rp;
-        }
+        myk = lane == p ? k : myk;
+        myrd = lane == p ? rp : myrd;
         const double l = elim ? kr[k] * rp : 0.0;
 #pragma unroll
         for (int j = k + 1; j < NN; ++j) kr[j] = fma(-l, readlane_d(kr[j], p), kr[j]);
@@ -704,6 +716,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll
         for (int j = 0; j < NN; ++j) KM[j * NN + myk] = kr[j];
         ws[L::O_RD + myk] = myrd;
+        PL[myk] = lane;
       }
     }
     wave_sync();
@@ -740,7 +753,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       // all NT x MT tiles accumulated together (independent MFMA chains)
       int ggs[NT], bbs[NT];
       bool cvs[NT];
-      double ntc[NT][3];
+      dvec4 ntc[NT];  // a vector value, so that the lane-dependent component index needs no memory
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
         const int cc = nt * 16 + li;
@@ -748,7 +761,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         ggs[nt] = cvs[nt] ? cc / NNF : 0;
         bbs[nt] = cvs[nt] ? cc % NNF : 0;
         const double rg = FG[ggs[nt] * (D + 1) + D] / det;
-        for (int a2 = 0; a2 < 3; ++a2) ntc[nt][a2] = 0.0;
+        ntc[nt] = dvec4{0.0, 0.0, 0.0, 0.0};
         for (int a2 = 0; a2 < D; ++a2) {
           double s = 0.0;
           for (int k = 0; k < D; ++k) s += J[a2 * D + k] * FG[ggs[nt] * (D + 1) + k];
@@ -775,7 +788,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
-          const double ntv = a2 == 0 ? ntc[nt][0] : (a2 == 1 ? ntc[nt][1] : ntc[nt][2]);
+          const double ntv = ntc[nt][a2];
           const double y = GT[(ggs[nt] * NN + j) * NNF + bbs[nt]];
           bv[nt] = kv ? ntv * y : 0.0;
         }
@@ -833,6 +846,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           if (t + 1 < k * NN + hi) f[t + 1 - k * NN] = v.y;
         }
       };
+      // (the prefetch registers would cost the second wave per SIMD at the
+      // low orders: there the FMAs read the factors directly)
+      constexpr bool SPF = D == 3 ? P >= 2 : P >= 4;
+      if constexpr (SPF) {
       column(0, 1, NN, fc);
 #pragma unroll
       for (int k = 0; k < NN - 1; ++k) {
@@ -852,6 +869,19 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         for (int i = 0; i < k; ++i) y[i] = fma(-fc[i], y[k], y[i]);
 #pragma unroll
         for (int i = 0; i + 1 < k; ++i) fc[i] = fn[i];
+      }
+      } else {
+#pragma unroll
+        for (int k = 0; k < NN - 1; ++k)
+#pragma unroll
+          for (int i = k + 1; i < NN; ++i) y[i] = fma(-KM[k * NN + i], y[k], y[i]);
+#pragma unroll
+        for (int kk = 0; kk < NN; ++kk) {
+          const int k = NN - 1 - kk;
+          y[k] *= RDp[k];
+#pragma unroll
+          for (int i = 0; i < k; ++i) y[i] = fma(-KM[k * NN + i], y[k], y[i]);
+        }
       }
 #pragma unroll
       for (int i = 0; i < NN; ++i) UB[i * NCOL + c] = -y[i];
@@ -1096,8 +1126,17 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
   }
 }
 
+// Orders whose registers fit two waves per SIMD (the compiler is asked to
+// keep them there: the second wave hides the LDS and MFMA latencies; the
+// accumulators then live in ordinary VGPRs)
+template <int D, int P, bool TSG>
+constexpr bool two_waves() {
+  return D == 2 ? P <= (TSG ? 2 : 3) : P == 1;
+}
+
 template <int D, int P, bool TD, int W, bool TSG>
-__global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(two_waves<D, P, TSG>() ? 2 : 1, 8))) void
+hdg_affine_kernel(ElemArgs a) {
   using L = ALay<D, P, TD>;
   extern __shared__ __align__(16) double sm[];
   __shared__ int s_pos[L::NN];
@@ -1113,7 +1152,8 @@ __global__ __launch_bounds__(64 * W) void hdg_affine_kernel(ElemArgs a) {
 // one problem.  Each order has its own kernel so that its register and LDS
 // budget is that of the order alone.
 template <int P, int W>
-__global__ __launch_bounds__(64 * W) void hdg_affine_batch_kernel(const ElemArgs* __restrict__ probs,
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(two_waves<2, P, false>() ? 2 : 1, 8))) void
+hdg_affine_batch_kernel(const ElemArgs* __restrict__ probs,
                                                                   const BatchItem* __restrict__ items, int nitems) {
   using L = ALay<2, P, false>;
   extern __shared__ __align__(16) double sm[];

@@ -425,7 +425,8 @@ class Batch:
         """{order: dict(waves, tile, items, workgroups)} for the orders present."""
         out = np.zeros(21, dtype=np.int32)
         check(lib.hfx_batch_get_config(self._h, _ptr(out)))
-        self.concurrent = bool(out[20])
+        self.concurrent = bool(out[20] & 1)
+        self.graph_concurrent = bool(out[20] & 2)
         return {p: dict(waves=int(out[4 * (p - 1)]), tile=int(out[4 * (p - 1) + 1]), items=int(out[4 * (p - 1) + 2]),
                         workgroups=int(out[4 * (p - 1) + 3])) for p in range(1, 6) if out[4 * (p - 1) + 2] > 0}
 

@@ -450,13 +450,15 @@ int hfx_batch_size(hfx_batch* b, int* n, long long* cells);
  * workgroup (1, 2 or 4) and cells per work item. */
 int hfx_batch_set_config(hfx_batch* b, int order, int waves, int tile);
 /* out[4*(p-1) + 0..3] = waves, tile, work items, workgroups of order p = 1..5
- * (zero items for an order not in the batch); out[20] = 1 when the orders run
- * concurrently on side streams (21 ints). */
+ * (zero items for an order not in the batch); out[20] bit 0 set when the
+ * orders run concurrently on side streams, bit 1 when they do inside the
+ * captured graph (21 ints). */
 int hfx_batch_get_config(hfx_batch* b, int* out);
 /* Autotune (waves, tile) of every order by timing each candidate launch
  * `reps` times on this batch, keeping the fastest per order, then choose
- * concurrent or one-after-another order launches; best_ms returns the time
- * of the tuned launch of all orders. */
+ * concurrent or one-after-another order launches, for eager launches and
+ * (timed separately) inside the replayed graph; best_ms returns the time of
+ * the tuned eager launch of all orders. */
 int hfx_batch_tune(hfx_batch* b, int reps, double* best_ms);
 /* Assemble every problem: the per-order launches + the scatters; use_graph
  * != 0 captures the sequence as a HIP graph on first use and replays it. */
