@@ -974,14 +974,14 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
                 for (int a2 = 0; a2 < D; ++a2) s -= JI[k * D + a2] * y[m][a2][r];
                 qk[k] = s + ng[k] * gb;
               }
-              if (trc) {
-                // streaming stores: U and Q are read again only by the reconstruction
-                __builtin_nontemporal_store(ubv[m][r], &Ue[(size_t)i * NL + cc]);
-                for (int k = 0; k < D; ++k) __builtin_nontemporal_store(qk[k], &Qe[(size_t)(i * D + k) * NL + cc]);
-              } else {
-                Ue[i] = ubv[m][r];
-                for (int k = 0; k < D; ++k) Qe[i * D + k] = qk[k];
-              }
+              // streaming stores (U and Q are read again only by the
+              // reconstruction), the F_u column's to U0 and Q0: addresses by
+              // select, no branch
+              const size_t su = trc ? (size_t)i * NL + cc : (size_t)i;
+              const size_t sq0 = trc ? (size_t)i * D * NL + cc : (size_t)i * D;
+              const size_t sqk = trc ? (size_t)NL : 1;
+              __builtin_nontemporal_store(ubv[m][r], &Ue[su]);
+              for (int k = 0; k < D; ++k) __builtin_nontemporal_store(qk[k], &Qe[sq0 + k * sqk]);
               // n_f . q on every face of node i; a node off face f writes its
               // value to a dead row instead (the face-IP weights, unused after
               // the face masses), which keeps the stores free of branches
@@ -1043,26 +1043,30 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       for (int f = 0; f < NF; ++f) {
         const double* fmt = FM + (f * NSF + 0) * NNF * NNF;
         const double* fmb = FM + (f * NSF + 1) * NNF * NNF;
-        const double* fmd = FM + (f * NSF + 2) * NNF * NNF;
+        // operands at computed offsets (no selected pointers: those become
+        // branches with a wait each); Slu_f = FMt, Slq_f = -FMd
         double av[KS], bv[KS][NT];
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
           const int kk = ks * 4 + lk;
           const bool kv = kk < KF, isu = kk < NNF;
           const int b = kv ? (isu ? kk : kk - NNF) : 0;
-          const double x = (isu ? fmt : fmd)[lic * NNF + b];
+          const double x = FM[((f * NSF + (isu ? 0 : 2)) * NNF + lic) * NNF + b];
           av[ks] = (kv && li < NNF) ? (isu ? x : -x) : 0.0;
-          const double* row = isu ? UB + s_fn[f * NNF + b] * NCOL : QN + (f * NNF + b) * NCOL;
+          const int node = s_fn[f * NNF + b];
+          const int roff = isu ? L::O_RB + node * NCOL : L::O_SUQ + (f * NNF + b) * NCOL;
 #pragma unroll
           for (int nt = 0; nt < NT; ++nt) {
-            const double y = row[ccl[nt]];
+            const double y = ws[roff + ccl[nt]];
             bv[ks][nt] = (kv && nt * 16 + li < NCOL) ? y : 0.0;
           }
         }
         // epilogue inputs, loaded while the MFMAs run: the face's flags and
         // node permutation, each column's block, the Sll = -FMt + FMb terms
-        const int Ff = fused ? IX[L::IX_FACE + f] : 0;
-        const int flg = fused ? IX[L::IX_FLAG + f] : 0;
+        // (the index words are read in both modes; only the fused scatter
+        // uses them)
+        const int Ff = IX[L::IX_FACE + f];
+        const int flg = IX[L::IX_FLAG + f];
         int prm[4], bpo[NT];
         double sll[NT][4];
 #pragma unroll
@@ -1072,7 +1076,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
-          bpo[nt] = fused ? IX[L::IX_BPOS + f * NF + gcol[nt]] : 0;
+          bpo[nt] = IX[L::IX_BPOS + f * NF + gcol[nt]];
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int aa = 4 * r + lk, ac = aa < NNF ? aa : 0;
@@ -1092,32 +1096,29 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         mfma_interleave<KS * NT, KS * (NT + 1)>();
         const bool dirf = (flg & 2) != 0;
         const int sidef = flg & 1;
+        // one streaming store per entry, its address and value chosen by
+        // selects: a trace column goes to its block of the trace matrix
+        // (fused; zero on a Dirichlet face's off-diagonal blocks) or to S,
+        // the F_u column to S0 (negated)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           const int cc = nt * 16 + li;
           const dvec4 acc = accs[nt];
-          if (cc <= NL) {
-            const bool trc = cc < NL;
-            const int gg = gcol[nt];
-            double* Se = trc ? a.S + (size_t)e * NL * NL : a.S0 + (size_t)e * NL;
-            double* blk = nullptr;  // fused: this lane's block, column already placed
-            if (fused && trc)
-              blk = (gg == f) ? a.dside + (size_t)(2 * Ff + sidef) * NNF * NNF + pcs[nt]
-                              : a.bvals + (size_t)bpo[nt] * NNF * NNF + pcs[nt];
+          const bool trc = cc < NL;
+          const int gg = gcol[nt];
+          double* blk = (gg == f) ? a.dside + (size_t)(2 * Ff + sidef) * NNF * NNF + pcs[nt]
+                                  : a.bvals + (size_t)bpo[nt] * NNF * NNF + pcs[nt];
+          double* srow = a.S + (size_t)e * NL * NL + cc;
+          double* s0 = a.S0 + (size_t)e * NL;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int aa = 4 * r + lk;
-              if (aa >= NNF) continue;
-              double sv = (expl && trc) ? 0.0 : acc[r];
-              if (trc) {
-                if (f == gg) sv += sll[nt][r];
-                if (fused)  // streaming store (measured faster, the diagonal sides included)
-                  __builtin_nontemporal_store((dirf && gg != f) ? 0.0 : sv, &blk[(size_t)prm[r] * NNF]);
-                else Se[(size_t)(f * NNF + aa) * NL + cc] = sv;
-              } else {
-                Se[f * NNF + aa] = -sv;
-              }
-            }
+          for (int r = 0; r < 4; ++r) {
+            const int aa = 4 * r + lk;
+            double sv = (expl && trc) ? 0.0 : acc[r];
+            sv += (trc && f == gg) ? sll[nt][r] : 0.0;
+            double* dst = !trc ? s0 + f * NNF + aa
+                               : (fused ? blk + (size_t)prm[r] * NNF : srow + (size_t)(f * NNF + aa) * NL);
+            const double val = !trc ? -sv : ((fused && dirf && gg != f) ? 0.0 : sv);
+            if (cc <= NL && aa < NNF) __builtin_nontemporal_store(val, dst);
           }
         }
       }
