@@ -551,24 +551,28 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           for (int r = 0; r < 4; ++r) {
             const int i = m * 16 + 4 * r + lk, j = jb;
             if (i == NN && j < NN) FU[j] = acc[D][r];
-            if (i >= NN || j >= NN) continue;
+            // face terms without branches: every face's masses are read (at
+            // clamped positions) and the entries off the face masked out
+            const bool ok = i < NN && j < NN;
+            const int ic = i < NN ? i : 0, jc = j < NN ? j : 0;
             double aq[D], au = acc[D][r];
             for (int k = 0; k < D; ++k) aq[k] = acc[k][r];
-            const unsigned pi = (unsigned)s_pos[i], pj = (unsigned)s_pos[j];
+            const unsigned pi = (unsigned)s_pos[ic], pj = (unsigned)s_pos[jc];
+#pragma unroll
             for (int f = 0; f < NF; ++f) {
               const int ai = (int)(pi >> (8 * f) & 0xFFu), bj = (int)(pj >> (8 * f) & 0xFFu);
-              if (ai != 0xFF && bj != 0xFF) {
-                const double* F0 = FM + (f * NSF) * NNF * NNF + ai * NNF + bj;
-                au += F0[0];
-                for (int k = 0; k < D; ++k) aq[k] -= nfe[f][k] * F0[2 * NNF * NNF];
-              }
+              const bool on = ai != 0xFF && bj != 0xFF;
+              const double* F0 = FM + (f * NSF) * NNF * NNF + (on ? ai : 0) * NNF + (on ? bj : 0);
+              const double t0 = F0[0], t2 = F0[2 * NNF * NNF];
+              au += on ? t0 : 0.0;
+              for (int k = 0; k < D; ++k) aq[k] -= nfe[f][k] * (on ? t2 : 0.0);
             }
             for (int r2 = 0; r2 < D; ++r2) {
               double s2 = 0.0;
               for (int k = 0; k < D; ++k) s2 += JI[k * D + r2] * aq[k];
-              SUQ[(r2 * NN + i) * NN + j] = s2;
+              if (ok) SUQ[(r2 * NN + i) * NN + j] = s2;
             }
-            KM[i * NN + j] = au;
+            if (ok) KM[i * NN + j] = au;
           }
         }
       }
@@ -725,23 +729,12 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     // Column vectors live in registers (compile-time row index) or in
     // lane-contiguous LDS rows [row][NCOL] when read at face-node positions.
     const int c = lane;
-    const bool tr = c < NL;
-    const int g = tr ? c / NNF : 0, bp = tr ? c % NNF : 0;
     double* UB = RB;               // U columns replace R columns lane by lane
     double* QN = ws + L::O_SUQ;    // face traces of n.q, over the dead Suq and K factors
     // R[:, c] = Sul[:, c] - sum_k Suq_k B_k[:, c] (or -F_u on lane NL), with
-    // B_k[:, (g,b)] = -rho_g n_gk G_g[:, b] and sum_k n_gk Suq_k = sum_a nt_a SUQ_a
-    if (c <= NL) {
-      const double sul_s = TSG ? a.ts_arest : 1.0;  // the time transform scales Sul
-      for (int i = 0; i < NN; ++i) RB[i * NCOL + c] = tr ? 0.0 : -FU[i];
-      if (tr) {
-#pragma unroll
-        for (int a2 = 0; a2 < NNF; ++a2)
-          RB[s_fn[g * NNF + a2] * NCOL + c] =
-              sul_s * (-FM[((g * NSF + 0) * NNF + a2) * NNF + bp] + FM[((g * NSF + 1) * NNF + a2) * NNF + bp]);
-      }
-    }
-    wave_sync();
+    // B_k[:, (g,b)] = -rho_g n_gk G_g[:, b] and sum_k n_gk Suq_k = sum_a nt_a SUQ_a;
+    // Sul[i][(g, b)] = -FMt_g[a][b] + FMb_g[a][b] when node i is node a of
+    // face g, else 0 (added to the product in its epilogue)
     // ... + sum_a nt_a(c) SUQ_a G_g(c)[:, bp(c)] for the trace columns on FP64
     // MFMA: X = [SUQ_0 | .. | SUQ_{D-1}] (NN x D NN) times Bt (D NN x NL),
     // Bt[(a, j)][c] = nt_a(c) G_g(c)[j][bp(c)], nt_a = rho_g sum_k J[a][k] n_gk.
@@ -798,9 +791,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           for (int m = 0; m < MT; ++m) acc[nt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[nt], acc[nt][m], 0, 0, 0);
       }
       mfma_interleave<KS * MT * NT, KS * (MT + NT)>();
-      // R += product: all the R entries are read before any is written (the
-      // compiler cannot tell the rows apart and would otherwise wait on every
-      // read-modify-write)
+      // R = Sul + product (F_u column: -F_u), every entry written once; the
+      // Sul entries are read at clamped face positions, masked off the face
+      static_assert(NL % 16 != 0, "the F_u column lies in the last column tile");
+      const double sul_s = TSG ? a.ts_arest : 1.0;  // the time transform scales Sul
       double rv[NT][MT][4];
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
@@ -808,8 +802,14 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         for (int m = 0; m < MT; ++m)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = m * 16 + 4 * r + lk, cc = nt * 16 + li;
-            rv[nt][m][r] = RB[(i < NN ? i : 0) * NCOL + (cc < NCOL ? cc : 0)];
+            const int i = m * 16 + 4 * r + lk, ic = i < NN ? i : 0;
+            const int pa = (int)((unsigned)s_pos[ic] >> (8 * ggs[nt]) & 0xFFu);
+            const bool on = pa != 0xFF;
+            const int pac = on ? pa : 0;
+            const double t = -FM[((ggs[nt] * NSF + 0) * NNF + pac) * NNF + bbs[nt]] +
+                             FM[((ggs[nt] * NSF + 1) * NNF + pac) * NNF + bbs[nt]];
+            const double fu = FU[ic];
+            rv[nt][m][r] = cvs[nt] ? (on ? sul_s * t : 0.0) : -fu;
           }
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
@@ -818,7 +818,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = m * 16 + 4 * r + lk, cc = nt * 16 + li;
-            if (cvs[nt] && i < NN) RB[i * NCOL + cc] = rv[nt][m][r] + acc[nt][m][r];
+            if (cc <= NL && i < NN) RB[i * NCOL + cc] = rv[nt][m][r] + acc[nt][m][r];
           }
     }
     wave_sync();
