@@ -390,53 +390,57 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
       FG[f * (D + 1) + D] = meas;
     }
     // ------------------------------------------------ cubature-point data
-    for (int q = lane; q < NIP; q += 64) {
-      double dq[TD ? D * D : 1], bq[D];
-      for (int u = 0; u < ND; ++u) dq[u] = 0.0;
+    // one cubature point (then one face point) per lane, as straight-line
+    // code at clamped indices with the stores masked: the reads pipeline
+    static_assert(NIP <= 64 && NF * NIPF <= 64, "one point per lane");
+    {
+      const int q = lane < NIP ? lane : 0;
+      double dq = 0.0, bq[D];
       for (int c = 0; c < D; ++c) bq[c] = 0.0;
+#pragma unroll
       for (int i = 0; i < NN; ++i) {
         const double ph = PHI[q * NN + i];
-        for (int u = 0; u < ND; ++u) dq[u] += ph * DN[i * ND + u];
+        dq += ph * DN[i];
         for (int c = 0; c < D; ++c) bq[c] += ph * BN[i * D + c];
       }
       const double wd = WQ[q] * det;
-      SD[q] = wd * dq[0];
+      double bj[D];
       for (int r = 0; r < D; ++r) {
         double s = 0.0;
         for (int k = 0; k < D; ++k) s += bq[k] * JI[k * D + r];
-        BJ[q * D + r] = wd * s;
+        bj[r] = wd * s;
       }
-      WR[q] = wd * RQ[q];
-      WF[q] = wd * FQ[q];
+      const double wr = wd * RQ[q], wf = wd * FQ[q];
+      if (lane < NIP) {
+        SD[q] = wd * dq;
+        for (int r = 0; r < D; ++r) BJ[q * D + r] = bj[r];
+        WR[q] = wr;
+        WF[q] = wf;
+      }
     }
     wave_sync();
-    for (int t = lane; t < NF * NIPF; t += 64) {
+    {
+      const int t = lane < NF * NIPF ? lane : 0;
       const int f = t / NIPF, q = t % NIPF;
-      double tq = 0.0, bq[D], dq[TD ? D * D : 1];
+      double tq = 0.0, bq[D], dq = 0.0;
       for (int c = 0; c < D; ++c) bq[c] = 0.0;
-      for (int u = 0; u < ND; ++u) dq[u] = 0.0;
+#pragma unroll
       for (int j = 0; j < NNF; ++j) {
         const double ps = FPHI[q * NNF + j];
         const int node = s_fn[f * NNF + j];
         tq += ps * TAU[f * NNF + j];
         for (int c = 0; c < D; ++c) bq[c] += ps * BN[node * D + c];
-        for (int u = 0; u < ND; ++u) dq[u] += ps * DN[node * ND + u];
+        dq += ps * DN[node];
       }
       const double* nf = FG + f * (D + 1);
       const double ds = FWQ[q] * nf[D];
       double bn = 0.0;
       for (int c = 0; c < D; ++c) bn += bq[c] * nf[c];
-      double* w = FWT + t * NS;
-      w[0] = ds * tq;
-      w[1] = ds * bn;
-      if constexpr (TD) {
-        for (int k = 0; k < D; ++k) {
-          double s = 0.0;
-          for (int c = 0; c < D; ++c) s += dq[k * D + c] * nf[c];
-          w[2 + k] = ds * s;
-        }
-      } else {
-        w[2] = ds * dq[0];
+      if (lane < NF * NIPF) {
+        double* w = FWT + t * NS;
+        w[0] = ds * tq;
+        w[1] = ds * bn;
+        w[2] = ds * dq;
       }
     }
     wave_sync();
@@ -667,6 +671,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           for (int m = 0; m < MT; ++m) acc[nt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[nt], acc[nt][m], 0, 0, 0);
       }
       mfma_interleave<KS * MT * MT, KS * 2 * MT>();
+      // Suu entries read (at clamped addresses) before any K entry is written:
+      // the compiler cannot tell the entries apart and would otherwise wait
+      // on every read-modify-write
+      double kv0[MT][MT][4];
 #pragma unroll
       for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
@@ -674,7 +682,16 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = m * 16 + 4 * r + lk, jb = nt * 16 + li;
-            if (i < NN && jb < NN) KM[i * NN + jb] -= acc[nt][m][r];
+            kv0[nt][m][r] = KM[(i < NN ? i : 0) * NN + (jb < NN ? jb : 0)];
+          }
+#pragma unroll
+      for (int nt = 0; nt < MT; ++nt)
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = m * 16 + 4 * r + lk, jb = nt * 16 + li;
+            if (i < NN && jb < NN) KM[i * NN + jb] = kv0[nt][m][r] - acc[nt][m][r];
           }
     }
     wave_sync();
