@@ -55,6 +55,17 @@ __device__ __forceinline__ void mfma_interleave() {
   }
 }
 
+// 1 / x from the hardware estimate and two Newton steps (within an ulp or so
+// of the rounded quotient): a shorter dependent chain than the IEEE division
+// sequence, on the pivot path of the LU
+__device__ inline double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
 // value of v in lane l (l uniform across the wave)
 __device__ inline double readlane_d(double v, int l) {
   const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
@@ -723,7 +734,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         const int p = __builtin_amdgcn_readfirstlane(dpp_argmax(fabs(kr[k]), cand));
         used |= 1ull << p;
         const bool elim = cand && lane != p;
-        const double rp = 1.0 / readlane_d(kr[k], p);
+        const double rp = recip(readlane_d(kr[k], p));
         myk = lane == p ? k : myk;
         myrd = lane == p ? rp : myrd;
         const double l = elim ? kr[k] * rp : 0.0;
@@ -1056,7 +1067,9 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         bcol[nt] = cc < NL ? cc % NNF : 0;
         pcs[nt] = IX[L::IX_PERM + gcol[nt] * NNF + bcol[nt]];  // block column of a trace column
       }
-#pragma unroll 1
+      // faces unrolled: the next face's operand reads overlap this face's
+      // MFMAs and stores
+#pragma unroll
       for (int f = 0; f < NF; ++f) {
         const double* fmt = FM + (f * NSF + 0) * NNF * NNF;
         const double* fmb = FM + (f * NSF + 1) * NNF * NNF;
