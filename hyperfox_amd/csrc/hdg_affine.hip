@@ -944,7 +944,9 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
           }
         }
       }
-#pragma unroll 1
+      // column tiles unrolled: the next tile's reads overlap this tile's
+      // MFMAs and stores
+#pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
         const int cc = nt * 16 + li, ccc = cc < NCOL ? cc : NCOL - 1;
         double bv[KS];
