@@ -1004,14 +1004,17 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
                 for (int a2 = 0; a2 < D; ++a2) s -= JI[k * D + a2] * y[m][a2][r];
                 qk[k] = s + ng[k] * gb;
               }
-              // streaming stores (U and Q are read again only by the
-              // reconstruction), the F_u column's to U0 and Q0: addresses by
-              // select, no branch
-              const size_t su = trc ? (size_t)i * NL + cc : (size_t)i;
-              const size_t sq0 = trc ? (size_t)i * D * NL + cc : (size_t)i * D;
-              const size_t sqk = trc ? (size_t)NL : 1;
-              __builtin_nontemporal_store(ubv[m][r], &Ue[su]);
-              for (int k = 0; k < D; ++k) __builtin_nontemporal_store(qk[k], &Qe[sq0 + k * sqk]);
+              // streaming stores for U and Q (read again only by the
+              // reconstruction); the F_u column's U0 and Q0 are small and
+              // go through the cache (a streaming store of a lone 8-byte
+              // entry writes a partial line)
+              if (trc) {
+                __builtin_nontemporal_store(ubv[m][r], &Ue[(size_t)i * NL + cc]);
+                for (int k = 0; k < D; ++k) __builtin_nontemporal_store(qk[k], &Qe[(size_t)(i * D + k) * NL + cc]);
+              } else {
+                Ue[i] = ubv[m][r];
+                for (int k = 0; k < D; ++k) Qe[i * D + k] = qk[k];
+              }
               // n_f . q on every face of node i; a node off face f writes its
               // value to a dead row instead (the face-IP weights, unused after
               // the face masses), which keeps the stores free of branches
@@ -1128,10 +1131,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         mfma_interleave<KS * NT, KS * (NT + 1)>();
         const bool dirf = (flg & 2) != 0;
         const int sidef = flg & 1;
-        // one streaming store per entry, its address and value chosen by
-        // selects: a trace column goes to its block of the trace matrix
-        // (fused; zero on a Dirichlet face's off-diagonal blocks) or to S,
-        // the F_u column to S0 (negated)
+        // one streaming store per trace-column entry, its address and value
+        // chosen by selects: to its block of the trace matrix (fused; zero on
+        // a Dirichlet face's off-diagonal blocks) or to S; the F_u column to
+        // S0 (negated)
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           const int cc = nt * 16 + li;
@@ -1147,10 +1150,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
             const int aa = 4 * r + lk;
             double sv = (expl && trc) ? 0.0 : acc[r];
             sv += (trc && f == gg) ? sll[nt][r] : 0.0;
-            double* dst = !trc ? s0 + f * NNF + aa
-                               : (fused ? blk + (size_t)prm[r] * NNF : srow + (size_t)(f * NNF + aa) * NL);
-            const double val = !trc ? -sv : ((fused && dirf && gg != f) ? 0.0 : sv);
-            if (cc <= NL && aa < NNF) __builtin_nontemporal_store(val, dst);
+            double* dst = fused ? blk + (size_t)prm[r] * NNF : srow + (size_t)(f * NNF + aa) * NL;
+            const double val = (fused && dirf && gg != f) ? 0.0 : sv;
+            if (trc && aa < NNF) __builtin_nontemporal_store(val, dst);
+            if (cc == NL && aa < NNF) s0[f * NNF + aa] = -sv;  // S0: small, through the cache
           }
         }
       }
