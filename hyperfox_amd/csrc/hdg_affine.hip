@@ -945,8 +945,9 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         }
       }
       // column tiles unrolled: the next tile's reads overlap this tile's
-      // MFMAs and stores
-#pragma unroll
+      // MFMAs and stores (not in the time-transform variant)
+      constexpr int UT = TSG ? 1 : NT;
+#pragma unroll UT
       for (int nt = 0; nt < NT; ++nt) {
         const int cc = nt * 16 + li, ccc = cc < NCOL ? cc : NCOL - 1;
         double bv[KS];
@@ -1073,8 +1074,10 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
         pcs[nt] = IX[L::IX_PERM + gcol[nt] * NNF + bcol[nt]];  // block column of a trace column
       }
       // faces unrolled: the next face's operand reads overlap this face's
-      // MFMAs and stores
-#pragma unroll
+      // MFMAs and stores (not in the time-transform variant, whose registers
+      // that would spill)
+      constexpr int UF = TSG ? 1 : NF;
+#pragma unroll UF
       for (int f = 0; f < NF; ++f) {
         const double* fmt = FM + (f * NSF + 0) * NNF * NNF;
         const double* fmb = FM + (f * NSF + 1) * NNF * NNF;
