@@ -550,6 +550,15 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll 1
         for (int nt = 0; nt < MT; ++nt) {
           const int jb = nt * 16 + li;
+          // face positions of the tile's rows and column, read before the
+          // MFMAs (the face-mass reads of the epilogue depend on them)
+          unsigned pir[4];
+          const unsigned pj = (unsigned)s_pos[jb < NN ? jb : 0];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int i = m * 16 + 4 * r + lk;
+            pir[r] = (unsigned)s_pos[i < NN ? i : 0];
+          }
           dvec4 acc[NX];
 #pragma unroll
           for (int x = 0; x < NX; ++x) acc[x] = dvec4{0.0, 0.0, 0.0, 0.0};
@@ -569,10 +578,9 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
             // face terms without branches: every face's masses are read (at
             // clamped positions) and the entries off the face masked out
             const bool ok = i < NN && j < NN;
-            const int ic = i < NN ? i : 0, jc = j < NN ? j : 0;
             double aq[D], au = acc[D][r];
             for (int k = 0; k < D; ++k) aq[k] = acc[k][r];
-            const unsigned pi = (unsigned)s_pos[ic], pj = (unsigned)s_pos[jc];
+            const unsigned pi = pir[r];
 #pragma unroll
             for (int f = 0; f < NF; ++f) {
               const int ai = (int)(pi >> (8 * f) & 0xFFu), bj = (int)(pj >> (8 * f) & 0xFFu);
