@@ -1123,10 +1123,12 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           bpo[nt] = IX[L::IX_BPOS + f * NF + gcol[nt]];
+          // (only a tile holding some of face f's own columns needs them)
+          const bool own = nt * 16 <= f * NNF + NNF - 1 && f * NNF <= nt * 16 + 15;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int aa = 4 * r + lk, ac = aa < NNF ? aa : 0;
-            sll[nt][r] = -fmt[ac * NNF + bcol[nt]] + fmb[ac * NNF + bcol[nt]];
+            sll[nt][r] = own ? -fmt[ac * NNF + bcol[nt]] + fmb[ac * NNF + bcol[nt]] : 0.0;
           }
         }
         // the NT column tiles of face f accumulated together (independent
