@@ -826,9 +826,9 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
 #pragma unroll
           for (int m = 0; m < MT; ++m) acc[nt][m] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[m], bv[nt], acc[nt][m], 0, 0, 0);
       }
-      mfma_interleave<KS * MT * NT, KS * (MT + NT)>();
       // R = Sul + product (F_u column: -F_u), every entry written once; the
       // Sul entries are read at clamped face positions, masked off the face
+      // (those reads are interleaved with the MFMAs too)
       static_assert(NL % 16 != 0, "the F_u column lies in the last column tile");
       const double sul_s = TSG ? a.ts_arest : 1.0;  // the time transform scales Sul
       double rv[NT][MT][4];
@@ -847,6 +847,7 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
             const double fu = FU[ic];
             rv[nt][m][r] = cvs[nt] ? (on ? sul_s * t : 0.0) : -fu;
           }
+      mfma_interleave<KS * MT * NT, KS * (MT + NT) + NT * MT * 4 * 4>();
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
