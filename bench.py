@@ -33,7 +33,7 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM KiB per element-kernel dispatch, (read, write), from rocprofv3 --pmc
 # FETCH_SIZE / WRITE_SIZE runs of this script, scaled by a 1 GiB copy kernel
 # (FETCH_SIZE x2.00, WRITE_SIZE x1.00); the summary is kept under profiles/.
-MEASURED_TRAFFIC_KIB = {(24, 3): (2 * 80525.0, 3920440.0)}
+MEASURED_TRAFFIC_KIB = {(24, 3): (2 * 80524.0, 3901410.0)}
 
 
 def elem_flops(dim, p, nN, nIP, nNf, nIPf, conv):
