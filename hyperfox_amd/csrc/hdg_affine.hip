@@ -720,7 +720,8 @@ __device__ __forceinline__ void affine_cells(const ElemArgs& a, int e_first, int
     // yet (DPP argmax); every other such row takes l = K[i][k] / K[p][k],
     // K[i][j] -= l K[p][j] for j > k and keeps l in K[i][k].  Row p is then
     // row k of U (with the multipliers of the earlier steps to its left) and
-    // is stored as row k of KM, its reciprocal pivot in RD[k].  The R columns
+    // is stored as column k of KM (KM[j * NN + k]), its reciprocal pivot in
+    // RD[k], and the pivot row of step k in PL[k].  The R columns
     // are solved against these factors below: forward and back substitution
     // is backward stable, whereas multiplying by an explicitly formed K^-1
     // leaves an error in U that does not cancel in S = Slu U + Slq Q + Sll
